@@ -1,0 +1,306 @@
+/*
+ * vulkpy_hip.h -- C ABI of the MI355X (gfx950) backend for the vulkpy array,
+ * PRNG and nn API.
+ *
+ * Every function returns an int status (VKH_OK on success).  On failure the
+ * message is kept in thread-local storage and read with vkh_last_error().
+ * Device memory is addressed by plain pointers obtained from vkh_alloc (or
+ * from any other allocator on the same device, e.g. torch); sizes are element
+ * counts unless a name says "nbytes".  All work is enqueued on the stream
+ * owned by (or attached to) a vkh_gpu context and is asynchronous with respect
+ * to the host: completion is observed through vkh_event objects or
+ * vkh_gpu_wait().
+ *
+ * The reference exposes one pybind11 module, vulkpy/_vkarray.cc:756-898, whose
+ * GPU.submit(spv, ...) takes a shader path and a parameter struct.  Here each
+ * shader family is a typed entry point; the comment above each entry names the
+ * reference interface (file:line) it replaces.  Reference paths are relative to
+ * the vulkpy source tree.
+ */
+#ifndef VULKPY_HIP_H
+#define VULKPY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define VKH_OK            0
+#define VKH_ERROR         1  /* generic failure (maps to RuntimeError)        */
+#define VKH_TIMEOUT       2  /* event wait timed out ("Timeout at Command Wait") */
+#define VKH_INVALID       3  /* bad argument (maps to ValueError)             */
+#define VKH_OUT_OF_MEMORY 4
+#define VKH_NO_DEVICE     5
+
+/* ---- operation codes ---------------------------------------------------- */
+/* binary ops: vulkpy/shader/{add,sub,mul,div,max,min,pow}*.comp,
+ * reflected scalar ops rsub/rdiv/rpow_scalar.comp */
+enum vkh_binop {
+  VKH_ADD = 0, VKH_SUB = 1, VKH_MUL = 2, VKH_DIV = 3,
+  VKH_MAX = 4, VKH_MIN = 5, VKH_POW = 6,
+  VKH_RSUB = 7, VKH_RDIV = 8, VKH_RPOW = 9
+};
+
+/* unary math: vulkpy/shader/{abs,sign,...,invsqrt}.comp (vkarray.py:680-1098) */
+enum vkh_unop {
+  VKH_ABS = 0, VKH_SIGN = 1, VKH_SIN = 2, VKH_COS = 3, VKH_TAN = 4,
+  VKH_ASIN = 5, VKH_ACOS = 6, VKH_ATAN = 7, VKH_SINH = 8, VKH_COSH = 9,
+  VKH_TANH = 10, VKH_ASINH = 11, VKH_ACOSH = 12, VKH_ATANH = 13,
+  VKH_EXP = 14, VKH_LOG = 15, VKH_EXP2 = 16, VKH_LOG2 = 17,
+  VKH_SQRT = 18, VKH_INVSQRT = 19
+};
+
+/* reductions: vulkpy/shader/{sum,prod,maximum,minimum}*.comp */
+enum vkh_redop { VKH_RSUM = 0, VKH_RPROD = 1, VKH_RMAX = 2, VKH_RMIN = 3 };
+
+/* GEMM arithmetic modes (precision of the products; accumulation is FP32) */
+enum vkh_gemm_mode {
+  VKH_GEMM_FP32 = 0,   /* FP32 MFMA                                   */
+  VKH_GEMM_BF16 = 1,   /* operands rounded to BF16, FP32 accumulate    */
+  VKH_GEMM_BF16X3 = 2  /* split-BF16: hi*hi + hi*lo + lo*hi            */
+};
+
+/* GEMM epilogue activation */
+enum vkh_act { VKH_ACT_NONE = 0, VKH_ACT_RELU = 1, VKH_ACT_SIGMOID = 2 };
+
+typedef struct vkh_gpu vkh_gpu;           /* device + stream + allocator cache */
+typedef struct vkh_event vkh_event;       /* completion marker of enqueued work */
+typedef struct vkh_xoshiro vkh_xoshiro;   /* Xoshiro128++ state buffer          */
+
+/* ---- errors / library --------------------------------------------------- */
+/* thread-local message of the last failed call (empty if none) */
+const char* vkh_last_error(void);
+/* ABI version of this library */
+int vkh_abi_version(int* out);
+
+/* ---- device context ------------------------------------------------------ */
+/* number of visible HIP devices */
+int vkh_device_count(int* out);
+/* replaces createGPU(idx, priority) (_vkarray.cc:759-763) and GPU::GPU
+ * (_vkarray.cc:472-501).  One in-order stream at the default priority; the
+ * priority argument is accepted and recorded only. */
+int vkh_gpu_create(int idx, float priority, vkh_gpu** out);
+/* context bound to an existing hipStream_t (e.g. torch's current stream);
+ * kernels launched through it go to that stream.  Has no counterpart in the
+ * reference (torch interop for the autograd wrappers). */
+int vkh_gpu_create_on_stream(int idx, void* stream, vkh_gpu** out);
+int vkh_gpu_destroy(vkh_gpu* gpu);
+/* replaces GPU::wait (_vkarray.cc:560-562): waits until the stream is idle */
+int vkh_gpu_wait(vkh_gpu* gpu);
+/* replaces GPU::canSubgroupArithmetic (_vkarray.cc:564-573): always 1 */
+int vkh_gpu_can_subgroup_arithmetic(vkh_gpu* gpu, int* out);
+int vkh_gpu_get_stream(vkh_gpu* gpu, void** out);
+/* device properties: compute units, LDS bytes per block, wavefront size */
+int vkh_gpu_info(vkh_gpu* gpu, int* num_cus, int* lds_bytes, int* wave_size);
+
+/* ---- device memory ------------------------------------------------------- */
+/* replaces Buffer<T>::Buffer (_vkarray.cc:49-79) and GPU::createBuffer /
+ * toBuffer (_vkarray.cc:511-525).  Stream-ordered caching allocator: a freed
+ * block may be handed out again to work enqueued later on the same stream. */
+int vkh_alloc(vkh_gpu* gpu, size_t nbytes, void** out);
+int vkh_free(vkh_gpu* gpu, void* ptr);
+/* releases every cached (free) block back to the driver */
+int vkh_empty_cache(vkh_gpu* gpu);
+/* bytes held by the cache: live allocations and free cached blocks */
+int vkh_memory_stats(vkh_gpu* gpu, size_t* live_bytes, size_t* cached_bytes);
+/* host -> device copy enqueued on the stream; the host buffer must stay valid
+ * and unchanged until an event recorded after this call completes.  Replaces
+ * Buffer::set / the host-visible mapping (_vkarray.cc:85, 98-105). */
+int vkh_upload(vkh_gpu* gpu, void* dst, const void* src, size_t nbytes);
+/* device -> host copy ordered after all work on the stream; returns once the
+ * data is in host memory (replaces reading the mapped pointer, _vkarray.cc:107) */
+int vkh_download(vkh_gpu* gpu, void* dst, const void* src, size_t nbytes);
+/* device -> device copy on the stream */
+int vkh_copy(vkh_gpu* gpu, void* dst, const void* src, size_t nbytes);
+/* dst[0..n) = value (vkarray.py:1541 `z[:] = 0.0`, initializers.py:48) */
+int vkh_fill_f32(vkh_gpu* gpu, float* dst, float value, size_t n);
+int vkh_fill_u32(vkh_gpu* gpu, uint32_t* dst, uint32_t value, size_t n);
+
+/* ---- jobs ---------------------------------------------------------------- */
+/* replaces Job (_vkarray.cc:392-457): an event recorded after the work
+ * enqueued so far on the stream */
+int vkh_event_record(vkh_gpu* gpu, vkh_event** out);
+/* replaces Job::wait(timeout_ns) (_vkarray.cc:446-456); UINT64_MAX waits
+ * forever; returns VKH_TIMEOUT with "Timeout at Command Wait" */
+int vkh_event_wait(vkh_event* ev, uint64_t timeout_ns);
+int vkh_event_query(vkh_event* ev, int* done);
+int vkh_event_destroy(vkh_event* ev);
+/* event with timing enabled (for measuring kernels on the context's stream;
+ * no counterpart in the reference) and the milliseconds between two of them */
+int vkh_timer_record(vkh_gpu* gpu, vkh_event** out);
+int vkh_event_elapsed_ms(vkh_event* start, vkh_event* end, float* ms);
+
+/* ---- elementwise (F1-F10) ------------------------------------------------ */
+/* F1/F2 same-shape binary and in-place (c may equal a):
+ * add.comp ... pow.comp, iadd.comp ... ipow.comp; caller vkarray.py:447-459 */
+int vkh_binary(vkh_gpu* gpu, int op, const float* a, const float* b, float* c,
+               uint64_t n);
+/* F3/F4 scalar and reflected scalar ops (c may equal a):
+ * *_scalar.comp, r*_scalar.comp; caller vkarray.py:469-483, 573-583 */
+int vkh_binary_scalar(vkh_gpu* gpu, int op, const float* a, float s, float* c,
+                      uint64_t n);
+/* F5/F6 broadcast binary (NumPy rule).  shapes are left-padded to ndim
+ * (ndim <= 16); in-place when c == a and shape_c == shape_a.
+ * *_broadcast.comp, i*_broadcast.comp; caller vkarray.py:499-519, 539-557 */
+int vkh_binary_bcast(vkh_gpu* gpu, int op, const float* a, const float* b,
+                     float* c, int ndim, const uint64_t* shape_a,
+                     const uint64_t* shape_b, const uint64_t* shape_c);
+/* F7/F8 unary math (b may equal a): abs.comp ... invsqrt.comp */
+int vkh_unary(vkh_gpu* gpu, int fn, const float* a, float* b, uint64_t n);
+/* F9 clamp(x, lo, hi) with scalar or array bounds, each array broadcast to
+ * shape_out in-kernel (no broadcast_to copies).  lo / hi NULL => scalar lo_s /
+ * hi_s.  clamp*.comp, iclamp*.comp; caller vkarray.py:1110-1191 */
+int vkh_clamp(vkh_gpu* gpu, const float* x, const float* lo, const float* hi,
+              float lo_s, float hi_s, float* out, int ndim,
+              const uint64_t* shape_x, const uint64_t* shape_lo,
+              const uint64_t* shape_hi, const uint64_t* shape_out);
+/* F10 materialised broadcast copy: broadcast.comp; caller vkarray.py:1434-1474 */
+int vkh_broadcast_to(vkh_gpu* gpu, const float* a, float* out, int ndim,
+                     const uint64_t* shape_a, const uint64_t* shape_out);
+
+/* ---- reductions (F11-F14) ----------------------------------------------- */
+/* F11/F12 full reduction of n elements into out[0].  Accumulates in FP64;
+ * mean != 0 divides the FP64 sum by n before rounding.  Unlike sum.comp /
+ * sum_v1.3.comp (vkarray.py:1246-1274) this is correct for every n.
+ * Uses workspace from the context's allocator. */
+int vkh_reduce_all(vkh_gpu* gpu, int op, const float* a, uint64_t n,
+                   float* out, int mean);
+/* F13/F14 axis reduction over a viewed as [prev, axis, post]:
+ * out[p, j] = op_k a[p, k, j]  (rebroadcast == 0, out is [prev, post]) or
+ * out[p, k, j] = op_k' a[p, k', j] (rebroadcast != 0, out is [prev, axis, post]).
+ * The result is multiplied by scale_mul (if != 1) and then divided by
+ * scale_div (if != 1), both in FP32, reproducing mean's `*= n_after/n_before`
+ * (vkarray.py:1429-1430) and `/= shape[axis]` (vkarray.py:1427).
+ * *_axis.comp, *_axis_rebroadcast.comp; caller vkarray.py:1194-1244 */
+int vkh_reduce_axis(vkh_gpu* gpu, int op, const float* a, float* out,
+                    uint64_t prev, uint64_t axis, uint64_t post,
+                    int rebroadcast, float scale_mul, float scale_div);
+
+/* ---- gather (F17) -------------------------------------------------------- */
+/* c[i] = a[min(idx[i], n_a - 1)]: gather.comp; caller vkarray.py:1493-1501 */
+int vkh_gather(vkh_gpu* gpu, const float* a, uint64_t n_a, const uint32_t* idx,
+               float* c, uint64_t n_idx);
+/* c[k, p, j] = a[p, min(idx[k], axis-1), j]: gather_axis.comp;
+ * caller vkarray.py:1502-1521 */
+int vkh_gather_axis(vkh_gpu* gpu, const float* a, const uint32_t* idx,
+                    float* c, uint64_t prev, uint64_t axis, uint64_t post,
+                    uint64_t n_idx);
+
+/* ---- PRNG (F18) ---------------------------------------------------------- */
+/* replaces PRNG::Xoshiro128pp ctor (_vkarray.cc:643-675): size states seeded
+ * from seed with the splitmix64 chain and the jump exactly as written there */
+int vkh_xoshiro_create(vkh_gpu* gpu, uint32_t size, uint64_t seed,
+                       vkh_xoshiro** out);
+int vkh_xoshiro_destroy(vkh_xoshiro* rng);
+int vkh_xoshiro_size(vkh_xoshiro* rng, uint32_t* out);
+/* device pointer of the [size][4] uint32 state (for tests) */
+int vkh_xoshiro_state(vkh_xoshiro* rng, uint32_t** out);
+/* replaces random_uint32 / random_float (_vkarray.cc:681-717):
+ * output k is next() of state k mod size, at that state's (k div size)-th
+ * step of this call; states >= n are not advanced.
+ * prng_xoshiro128pp_uint32.comp / prng_xoshiro128pp_float.comp */
+int vkh_xoshiro_uint32(vkh_xoshiro* rng, uint32_t* out, uint64_t n);
+int vkh_xoshiro_float(vkh_xoshiro* rng, float* out, uint64_t n);
+/* fused uniform + Box-Muller (random.py:60-124, prng_box_muller.comp,
+ * prng_ibox_muller.comp): draws n (n even) or n + 1 (n odd) uniforms and
+ * writes n normals */
+int vkh_xoshiro_normal(vkh_xoshiro* rng, float* out, uint64_t n, float mean,
+                       float stddev);
+/* fused uniform + prng_randrange.comp (random.py:126-186); high_incl = high-1 */
+int vkh_xoshiro_randrange(vkh_xoshiro* rng, uint32_t* out, uint64_t n,
+                          uint32_t low, uint32_t high_incl);
+/* Box-Muller of existing uniforms u[0..n_u) (n_u even) into out[0..n):
+ * prng_box_muller.comp (out may equal u: prng_ibox_muller.comp) */
+int vkh_box_muller(vkh_gpu* gpu, const float* u, float* out, uint64_t n,
+                   float mean, float stddev);
+
+/* ---- GEMM (F15/F16) ------------------------------------------------------ */
+/* C[M,N] = alpha * op(A) op(B) + beta * C + bias[N] (bias may be NULL), then
+ * act.  op(A) is A[M,K] row-major (trans_a=0) or A stored [K,M] (trans_a=1);
+ * op(B) is B[K,N] (trans_b=0) or B stored [N,K] (trans_b=1).  Leading
+ * dimensions are the row strides in elements.  If c_bf16 != NULL the result is
+ * also stored as BF16 bits.  Replaces matmul.comp (vkarray.py:585-605) and
+ * batch_affine.comp (nn/layers.py:94-101). */
+int vkh_gemm(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
+             uint64_t M, uint64_t N, uint64_t K, float alpha,
+             const float* A, uint64_t lda, const float* B, uint64_t ldb,
+             float beta, float* C, uint64_t ldc, const float* bias, int act,
+             uint16_t* c_bf16);
+
+/* ---- nn fused kernels (F16, F19 and the nn package compositions) ------- */
+/* y = act(x) elementwise: ReLU = x.max(0.0) (nn/layers.py:185),
+ * Sigmoid = 1/(1+exp(0-x)) (nn/layers.py:240-244) */
+int vkh_activation(vkh_gpu* gpu, int act, const float* x, float* y, uint64_t n);
+/* dx = dy * act'(y): ReLU dy*max(sign(y),0) (nn/layers.py:207-210),
+ * Sigmoid / Softmax diagonal dy*y*(1-y) (nn/layers.py:264-267, 320-323) */
+int vkh_activation_backward(vkh_gpu* gpu, int act, const float* y,
+                            const float* dy, float* dx, uint64_t n);
+/* row softmax of x[rows, cols] (nn/layers.py:297-300) */
+int vkh_softmax(vkh_gpu* gpu, const float* x, float* y, uint64_t rows,
+                uint64_t cols);
+/* per-row cross entropy L[r] = sum_c -y*log(x+1e-8) (nn_cross_entropy.comp +
+ * sum(axis=1), nn/losses.py:140-147) and dx = -y/(x+1e-8)
+ * (nn_cross_entropy_backward.comp) */
+int vkh_cross_entropy(vkh_gpu* gpu, const float* x, const float* y, float* L,
+                      uint64_t rows, uint64_t cols);
+int vkh_cross_entropy_backward(vkh_gpu* gpu, const float* x, const float* y,
+                               float* dx, uint64_t n, float scale);
+/* fused softmax + cross entropy (nn/losses.py:200-249): sm = softmax(x),
+ * L[r] = sum_c -y*log(sm+1e-8) */
+int vkh_softmax_cross_entropy(vkh_gpu* gpu, const float* x, const float* y,
+                              float* sm, float* L, uint64_t rows,
+                              uint64_t cols);
+/* per-row losses: MSE L[r] = sum_c (y-x)^2 (nn/losses.py:294-296);
+ * Huber L[r] = sum_c 0.5*min(|y-x|^2, |y-x|) (nn/losses.py:365-369) */
+int vkh_mse_rows(vkh_gpu* gpu, const float* x, const float* y, float* L,
+                 uint64_t rows, uint64_t cols);
+int vkh_huber_rows(vkh_gpu* gpu, const float* x, const float* y, float* L,
+                   uint64_t rows, uint64_t cols);
+/* loss gradients, each multiplied by scale in FP32 afterwards:
+ * kind 0 softmax-CE  dx = sm - y          (nn/losses.py:249)
+ * kind 1 MSE         dx = (x - y) * 2     (nn/losses.py:318-320)
+ * kind 2 Huber       dx = clamp(x-y,-1,1) (nn/losses.py:391-393) */
+int vkh_loss_grad(vkh_gpu* gpu, int kind, const float* x, const float* y,
+                  float* dx, uint64_t n, float scale);
+/* Adam update in the op order of AdamState.grad2diff + Parameter.update
+ * (nn/optimizers.py:235-253, nn/parameters.py:94-95), FP32 throughout;
+ * b1t / b2t are the already-updated beta powers (FP64, as the Python floats) */
+int vkh_adam(vkh_gpu* gpu, float* w, const float* g, float* m, float* v,
+             uint64_t n, float lr, float beta1, float beta2, float eps,
+             double b1t, double b2t);
+/* bpow[0] *= beta1, bpow[1] *= beta2 in FP64 on the device (the
+ * `beta1t *= beta1` of nn/optimizers.py:241-242), so a captured training
+ * step needs no host scalars */
+int vkh_adam_pow_advance(vkh_gpu* gpu, double* bpow, double beta1,
+                         double beta2);
+/* Adam update reading the beta powers from device memory bpow[0..1];
+ * w_bf16 (may be NULL) also receives the updated weights as BF16 */
+int vkh_adam_device(vkh_gpu* gpu, float* w, const float* g, float* m, float* v,
+                    uint64_t n, float lr, float beta1, float beta2, float eps,
+                    const double* bpow, uint16_t* w_bf16);
+/* SGD (nn/optimizers.py:53) and AdaGrad (nn/optimizers.py:134-140) */
+int vkh_sgd(vkh_gpu* gpu, float* w, const float* g, uint64_t n, float lr);
+int vkh_adagrad(vkh_gpu* gpu, float* w, const float* g, float* h, uint64_t n,
+                float lr, float eps);
+/* column sums of dy[rows, cols] accumulated into db (db += sum_r dy[r, :]):
+ * Dense bias gradient (nn/layers.py:126-127) */
+int vkh_bias_grad(vkh_gpu* gpu, const float* dy, float* db, uint64_t rows,
+                  uint64_t cols, float beta);
+/* dst[i] = BF16(src[i]) (round to nearest even) */
+int vkh_to_bf16(vkh_gpu* gpu, const float* src, uint16_t* dst, uint64_t n);
+
+/* ---- graphs ------------------------------------------------------------- */
+/* capture everything enqueued on the context's stream between begin and end
+ * into one executable hipGraph (no counterpart in the reference) */
+int vkh_graph_begin(vkh_gpu* gpu);
+int vkh_graph_end(vkh_gpu* gpu, void** graph_exec);
+int vkh_graph_launch(vkh_gpu* gpu, void* graph_exec);
+int vkh_graph_destroy(void* graph_exec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VULKPY_HIP_H */

@@ -1,0 +1,416 @@
+"""GPU parity of the array path (F1-F17) against the oracle and the reference's
+own known answers (test/test_vulkpy.py).  Integer-exact ops (+ - * /, max,
+min, clamp, copies, gather) are compared bit for bit; transcendentals against
+FP64 with ulp bounds; reductions against the oracle's sequential order or
+FP64 with stated tolerances."""
+import numpy as np
+import pytest
+
+import vulkpy_amd as vk
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ULP32 = np.float32(2.0 ** -23)
+
+
+def A(gpu, x):
+    return vk.Array(gpu, data=np.asarray(x, dtype=np.float32))
+
+
+def assert_bits(got, want):
+    got = np.asarray(got, dtype=np.float32)
+    want = np.asarray(want, dtype=np.float32)
+    assert got.shape == want.shape, (got.shape, want.shape)
+    same = (got.view(np.uint32) == want.view(np.uint32)) | (np.isnan(got) & np.isnan(want))
+    assert same.all(), f"{(~same).sum()} mismatches, first at {np.argwhere(~same)[:3].tolist()}"
+
+
+def ulp_err(got, exact64):
+    got = np.asarray(got, dtype=np.float64)
+    exact64 = np.asarray(exact64, dtype=np.float64)
+    spacing = np.spacing(np.abs(exact64).astype(np.float32)).astype(np.float64)
+    return np.abs(got - exact64) / spacing
+
+
+# ---- reference known answers (test/test_vulkpy.py) -------------------------------
+def test_reference_arithmetic_kats(gpu):
+    a, b = A(gpu, [5, 5, 5]), A(gpu, [1, 1, 1])
+    np.testing.assert_allclose(a + b, [6, 6, 6])
+    c = A(gpu, [4, 4, 4])
+    np.testing.assert_allclose(a + b + c, [10, 10, 10])
+    np.testing.assert_allclose(A(gpu, [4, 4, 4]) - A(gpu, [2, 2, 2]), [2, 2, 2])
+    np.testing.assert_allclose(A(gpu, [2, 2, 2]) * A(gpu, [3, 3, 3]), [6, 6, 6])
+    np.testing.assert_allclose(A(gpu, [8, 8, 8]) / A(gpu, [2, 2, 2]), [4, 4, 4])
+    x = A(gpu, [1, 2, 3])
+    np.testing.assert_allclose(x + 1, [2, 3, 4])
+    np.testing.assert_allclose(1 + x, [2, 3, 4])
+    np.testing.assert_allclose(5 - x, [4, 3, 2])
+    np.testing.assert_allclose(6 / x, [6, 3, 2])
+    y = A(gpu, [1, 2, 3])
+    y += 1
+    y *= 2
+    y -= 1
+    y /= 2
+    np.testing.assert_allclose(y, [1.5, 2.5, 3.5])
+    with pytest.raises(ValueError):
+        A(gpu, [1, 1]) + A(gpu, [1, 1, 1])
+
+
+def test_reference_matmul_kats(gpu):
+    a = A(gpu, [[1, 2], [3, 4]])
+    np.testing.assert_allclose(a @ a, [[7, 10], [15, 22]])
+    np.testing.assert_allclose(a @ A(gpu, [1, 3]), [7, 15])
+    np.testing.assert_allclose(A(gpu, [1, 2]) @ a, [7, 10])
+    with pytest.raises(ValueError):
+        A(gpu, [[1, 2, 3], [4, 5, 6]]) @ A(gpu, [[1, 2, 3], [4, 5, 6]])
+    v = A(gpu, [1, 2, 3]) @ A(gpu, [4, 5, 6])
+    assert v.shape == (1,)
+    np.testing.assert_allclose(v, [32])
+
+
+def test_reference_reshape_and_minmax(gpu):
+    a = A(gpu, [1, 2, 3, 4])
+    a.reshape((2, 2))
+    np.testing.assert_allclose(a, [[1, 2], [3, 4]])
+    with pytest.raises(ValueError):
+        A(gpu, [1, 2, 3]).reshape((2, 2))
+    a, b = A(gpu, [1, 2, 3]), A(gpu, [3, 2, 1])
+    np.testing.assert_allclose(a.max(b), [3, 2, 3])
+    np.testing.assert_allclose(a.min(b), [1, 2, 1])
+    np.testing.assert_allclose(a.max(1.5), [1.5, 2, 3])
+    np.testing.assert_allclose(a.min(1.5), [1, 1.5, 1.5])
+    a.max(b, inplace=True)
+    np.testing.assert_allclose(a, [3, 2, 3])
+    c = A(gpu, [1, 2, 3])
+    c.min(1.5, inplace=True)
+    np.testing.assert_allclose(c, [1, 1.5, 1.5])
+
+
+def test_reference_broadcast_kats(gpu):
+    one21, one22 = A(gpu, np.ones((2, 1))), A(gpu, np.ones((2, 2)))
+    np.testing.assert_allclose(one21 + one22, [[2, 2], [2, 2]])
+    np.testing.assert_allclose(A(gpu, np.ones(2)) + one22, [[2, 2], [2, 2]])
+    _a = np.arange(4).reshape((1, 2, 2))
+    _b = np.arange(4).reshape((2, 2, 1))
+    np.testing.assert_allclose(A(gpu, _a) + A(gpu, _b), _a + _b)
+    m = A(gpu, [[1, 2], [3, 4]])
+    np.testing.assert_allclose(m.max(A(gpu, [[2, 3]])), [[2, 3], [3, 4]])
+    np.testing.assert_allclose(m.min(A(gpu, [2, 3])), [[1, 2], [2, 3]])
+    np.testing.assert_allclose(m ** A(gpu, [[2, 3]]), [[1, 8], [9, 64]])
+    i = A(gpu, np.ones((2, 2)))
+    i += A(gpu, np.ones((2, 1)))
+    np.testing.assert_allclose(i, 2 * np.ones((2, 2)))
+    i = A(gpu, 2 * np.ones((2, 2)))
+    i /= A(gpu, np.ones(2))
+    np.testing.assert_allclose(i, 2 * np.ones((2, 2)))
+    b = A(gpu, np.ones((1, 2, 2))).broadcast_to((3, 2, 2))
+    np.testing.assert_allclose(b, np.ones((3, 2, 2)))
+    np.testing.assert_allclose(A(gpu, [1, 2, 3]).broadcast_to((2, 3)), [[1, 2, 3], [1, 2, 3]])
+    with pytest.raises(ValueError):
+        A(gpu, np.ones(3)).broadcast_to((4,))
+
+
+def test_reference_clamp_kats(gpu):
+    x = np.asarray([1.2, 3.5, 10])
+    lo, hi = np.asarray([3, 3, 4]), np.asarray([6, 6, 7])
+    np.testing.assert_allclose(A(gpu, x).clamp(A(gpu, lo), A(gpu, hi)), np.clip(x, lo, hi))
+    np.testing.assert_allclose(A(gpu, x).clamp(3, 7), np.clip(x, 3, 7))
+    a = A(gpu, x)
+    a.clamp(A(gpu, lo), 7, inplace=True)
+    np.testing.assert_allclose(a, np.clip(x, lo, 7))
+    m = A(gpu, [[1, 2], [3, 4]])
+    b, c = A(gpu, [[2, 3]]), A(gpu, [2, 3])
+    np.testing.assert_allclose(m.clamp(b, c), [[2, 3], [2, 3]])
+    np.testing.assert_allclose(m.clamp(b, 5), [[2, 3], [3, 4]])
+    np.testing.assert_allclose(m.clamp(2, c), [[2, 2], [2, 3]])
+    m.clamp(2, c, inplace=True)
+    np.testing.assert_allclose(m, [[2, 2], [2, 3]])
+
+
+def test_reference_reduction_kats(gpu):
+    a = A(gpu, [1, 2, 3])
+    np.testing.assert_allclose(a.sum(), (6,))
+    np.testing.assert_allclose(a.prod(), (6,))
+    np.testing.assert_allclose(a.maximum(), (3,))
+    np.testing.assert_allclose(a.minimum(), (1,))
+    np.testing.assert_allclose(a.mean(), (2,))
+    np.testing.assert_allclose(a.sum(axis=0), (6,))
+    ones = A(gpu, np.ones((2, 2)))
+    np.testing.assert_allclose(ones.sum(keepdims=True), [[4]])
+    np.testing.assert_allclose(ones.sum(axis=0, keepdims=True), [[2, 2]])
+    big = A(gpu, np.ones(65))
+    for f, v in (("sum", 65), ("prod", 1), ("maximum", 1), ("minimum", 1), ("mean", 1)):
+        np.testing.assert_allclose(getattr(big, f)(), (v,))
+    x = np.ones((2, 3, 4, 2, 2, 4, 3))
+    g = A(gpu, x)
+    np.testing.assert_allclose(g.sum(axis=(1, 2, 5)), x.sum(axis=(1, 2, 5)))
+    np.testing.assert_allclose(g.mean(axis=(1, 2, 5)), x.mean(axis=(1, 2, 5)))
+    m = A(gpu, [[1, 2], [3, 4]])
+    np.testing.assert_allclose(m.sum(axis=0, rebroadcast=True), [[4, 6], [4, 6]])
+    np.testing.assert_allclose(m.prod(axis=0, rebroadcast=True), [[3, 8], [3, 8]])
+    np.testing.assert_allclose(m.maximum(axis=0, rebroadcast=True), [[3, 4], [3, 4]])
+    np.testing.assert_allclose(m.minimum(axis=0, rebroadcast=True), [[1, 2], [1, 2]])
+    np.testing.assert_allclose(m.mean(axis=0, rebroadcast=True), [[2, 3], [2, 3]])
+    with pytest.raises(ValueError):
+        m.sum(axis=(0, 1), rebroadcast=True)
+
+
+def test_reference_gather_kats(gpu):
+    _a = np.arange(8).reshape((2, 2, 2))
+    for _idx in (np.asarray([0, 1, 6, 7]), np.asarray([[0, 1], [6, 7]])):
+        b = A(gpu, _a).gather(vk.U32Array(gpu, data=_idx))
+        assert tuple(b.shape) == _idx.shape
+        np.testing.assert_allclose(b, np.ravel(_a)[_idx])
+    _a = np.arange(30).reshape((3, 2, 5))
+    _idx = np.asarray([[0, 1], [1, 1]])
+    b = A(gpu, _a).gather(vk.U32Array(gpu, data=_idx), axis=1)
+    assert tuple(b.shape) == (2, 2, 3, 5)
+    np.testing.assert_allclose(b, np.moveaxis(np.take(_a, _idx, axis=1), (1, 2), (0, 1)))
+    oh = vk.U32Array(gpu, data=[0, 1, 2, 1, 0]).to_onehot(3)
+    np.testing.assert_allclose(oh, np.identity(3)[[0, 1, 2, 1, 0]])
+    oh = vk.U32Array(gpu, data=[[0, 1], [2, 1]]).to_onehot(3)
+    np.testing.assert_allclose(oh, np.identity(3)[[[0, 1], [2, 1]]])
+
+
+def test_reference_math_kats(gpu):
+    x = np.asarray([1, 2, 3])
+    for f, ref, tol in (("exp", np.exp, 1e-7), ("log", np.log, 1e-7), ("exp2", np.exp2, 1e-7),
+                        ("log2", np.log2, 1e-7), ("sqrt", np.sqrt, 1e-7),
+                        ("invsqrt", lambda v: np.sqrt(1 / v), 1e-7)):
+        np.testing.assert_allclose(getattr(A(gpu, x), f)(), ref(x), rtol=tol)
+        a = A(gpu, x)
+        getattr(a, f)(inplace=True)
+        np.testing.assert_allclose(a, ref(x), rtol=tol)
+    t = np.asarray([-0.9, 0.1, 0.5])
+    trig = {"sin": np.sin, "cos": np.cos, "tan": np.tan, "atan": np.arctan, "sinh": np.sinh,
+            "cosh": np.cosh, "tanh": np.tanh, "asinh": np.arcsinh, "atanh": np.arctanh}
+    for f, npf in trig.items():
+        np.testing.assert_allclose(getattr(A(gpu, t), f)(), npf(t), rtol=1e-5)
+    for f, npf in (("asin", np.arcsin), ("acos", np.arccos)):
+        np.testing.assert_allclose(getattr(A(gpu, t), f)(), npf(t), rtol=1e-3)
+    np.testing.assert_allclose(A(gpu, [1.5, 2.5]).acosh(), np.arccosh([1.5, 2.5]), rtol=1e-5)
+    np.testing.assert_allclose(A(gpu, [-1, 1, 3]).abs(), [1, 1, 3])
+    np.testing.assert_allclose(A(gpu, [-1, 0, 3]).sign(), [-1, 0, 1])
+    y = np.asarray([1.1, 2.2, 1.4])
+    np.testing.assert_allclose(A(gpu, x) ** A(gpu, y), x ** y, rtol=1e-7)
+    np.testing.assert_allclose(A(gpu, x) ** 2.7, x ** 2.7, rtol=1e-6)
+    np.testing.assert_allclose(1.3 ** A(gpu, y), 1.3 ** y, rtol=1e-6)
+
+
+# ---- oracle parity on the C1 workload ---------------------------------------------
+@pytest.fixture(scope="module")
+def c1():
+    rng = np.random.default_rng(0)
+    A_ = rng.uniform(-1, 1, (1024, 1024)).astype(np.float32)
+    B_ = rng.uniform(-1, 1, (1024, 1024)).astype(np.float32)
+    return A_, B_
+
+
+def test_c1_exact_ops(gpu, c1):
+    a, b = c1
+    ga, gb = A(gpu, a), A(gpu, b)
+    assert_bits(ga + gb, a + b)
+    assert_bits(ga * gb, a * b)
+    assert_bits(ga / (gb.abs() + 1), a / (np.abs(b) + np.float32(1)))
+    assert_bits(ga.clamp(-0.5, 0.5), np.clip(a, np.float32(-0.5), np.float32(0.5)))
+    assert_bits(ga - gb, a - b)
+    assert_bits(ga.max(gb), np.maximum(a, b))
+    assert_bits(ga.min(gb), np.minimum(a, b))
+    assert_bits(ga.sqrt(), np.sqrt(a))  # NaN for negative inputs on both sides
+
+
+def test_c1_transcendentals(gpu, c1):
+    a, b = c1
+    ga, gb = A(gpu, a), A(gpu, b)
+    a64 = a.astype(np.float64)
+    assert ulp_err(ga.exp(), np.exp(a64)).max() <= 0.51
+    assert ulp_err(ga.tanh(), np.tanh(a64)).max() <= 0.51
+    p = (np.abs(a64) + np.float64(np.float32(0.1))) ** b.astype(np.float64)
+    got = (ga.abs() + 0.1) ** gb
+    base = (np.abs(a) + np.float32(0.1)).astype(np.float64)
+    assert ulp_err(got, base ** b.astype(np.float64)).max() <= 0.51
+    assert np.allclose(np.asarray(got), p, rtol=1e-6)
+
+
+def test_c1_reductions(gpu, c1):
+    a, _ = c1
+    ga = A(gpu, a)
+    s0 = np.asarray(ga.sum(axis=0))
+    np.testing.assert_allclose(s0, a.astype(np.float64).sum(axis=0), rtol=0, atol=1e-4)
+    np.testing.assert_allclose(s0, O.reduce_axis("sum", a, 0), rtol=0, atol=1e-4)
+    m = np.asarray(ga.mean())
+    assert abs(float(m[0]) - a.astype(np.float64).mean()) <= 1e-7
+    assert_bits(ga.maximum(axis=1), a.max(axis=1))
+    assert_bits(ga.minimum(axis=0), a.min(axis=0))
+
+
+def test_c1_matmul(gpu):
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-1, 1, (512, 512)).astype(np.float32)
+    y = rng.uniform(-1, 1, (512, 512)).astype(np.float32)
+    got = np.asarray(A(gpu, x) @ A(gpu, y))
+    ref = x.astype(np.float64) @ y.astype(np.float64)
+    bound = np.abs(x).astype(np.float64) @ np.abs(y).astype(np.float64)
+    assert (np.abs(got - ref) / bound).max() <= 512 * 2 ** -24
+
+
+def test_c1_gather(gpu):
+    rng = np.random.default_rng(0)
+    src = rng.standard_normal(2 ** 20, dtype=np.float32)
+    idx = rng.integers(0, 2 ** 20, 2 ** 18, dtype=np.uint32)
+    got = A(gpu, src).gather(vk.U32Array(gpu, data=idx))
+    assert_bits(got, src[idx])
+    assert_bits(got, O.gather(src, idx))
+
+
+# ---- broadcasting, random shapes ---------------------------------------------------
+def _random_bcast_pair(rng):
+    nd = int(rng.integers(1, 6))
+    shape = [int(rng.integers(1, 7)) for _ in range(nd)]
+    sa = [s if rng.random() < 0.6 else 1 for s in shape]
+    sb = [s if rng.random() < 0.6 else 1 for s in shape]
+    da = int(rng.integers(0, nd))
+    db = int(rng.integers(0, nd))
+    return tuple(sa[da:]) or (1,), tuple(sb[db:]) or (1,)
+
+
+@pytest.mark.parametrize("op", ["add", "sub", "mul", "div", "max", "min"])
+def test_broadcast_random_shapes(gpu, op):
+    rng = np.random.default_rng(123)
+    fn = {"add": lambda x, y: x + y, "sub": lambda x, y: x - y, "mul": lambda x, y: x * y,
+          "div": lambda x, y: x / y, "max": lambda x, y: x.max(y), "min": lambda x, y: x.min(y)}[op]
+    for _ in range(40):
+        sa, sb = _random_bcast_pair(rng)
+        a = rng.standard_normal(sa).astype(np.float32)
+        b = rng.standard_normal(sb).astype(np.float32)
+        got = fn(A(gpu, a), A(gpu, b))
+        assert_bits(got, O.binary_bcast(op, a, b))
+
+
+def test_broadcast_c2_shapes_small(gpu):
+    # the C2 operand patterns at a reduced size: (M,N)+(M,1), (M,N)+(1,N), in-place
+    rng = np.random.default_rng(2)
+    for M, N in ((64, 256), (33, 100), (1, 7), (5, 1)):
+        a = rng.standard_normal((M, N)).astype(np.float32)
+        col = rng.standard_normal((M, 1)).astype(np.float32)
+        row = rng.standard_normal((1, N)).astype(np.float32)
+        for op, f in (("add", np.add), ("sub", np.subtract), ("mul", np.multiply), ("div", np.divide)):
+            assert_bits(A(gpu, a)._op(A(gpu, col), op), f(a, col))
+            assert_bits(A(gpu, a)._op(A(gpu, row), op), f(a, row))
+            assert_bits(A(gpu, col)._op(A(gpu, a), op), f(col, a))
+            g = A(gpu, a)
+            g._iop(A(gpu, row), op)
+            assert_bits(g, f(a, row))
+            g = A(gpu, a)
+            g._iop(A(gpu, col), op)
+            assert_bits(g, f(a, col))
+            g = A(gpu, a)
+            g._iop(1.25, op)
+            assert_bits(g, f(a, np.float32(1.25)))
+
+
+def test_broadcast_to_random(gpu):
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        sa, _ = _random_bcast_pair(rng)
+        target = tuple(int(rng.integers(2, 4)) if s == 1 else s for s in sa)
+        lead = tuple(int(x) for x in rng.integers(1, 3, int(rng.integers(0, 3))))
+        a = rng.standard_normal(sa).astype(np.float32)
+        got = A(gpu, a).broadcast_to(lead + target)
+        assert_bits(got, np.broadcast_to(a, lead + target))
+        assert_bits(got, O.broadcast_to(a, lead + target))
+
+
+def test_clamp_broadcast_random(gpu):
+    rng = np.random.default_rng(6)
+    for _ in range(20):
+        sa, sb = _random_bcast_pair(rng)
+        x = rng.standard_normal(sa).astype(np.float32)
+        lo = (rng.standard_normal(sb) - 0.5).astype(np.float32)
+        hi = (rng.standard_normal(sa) + 0.5).astype(np.float32)
+        got = A(gpu, x).clamp(A(gpu, lo), A(gpu, hi))
+        assert_bits(got, np.minimum(np.maximum(x, lo), hi))
+
+
+# ---- reductions, random shapes ----------------------------------------------------
+@pytest.mark.parametrize("op", ["sum", "prod", "maximum", "minimum", "mean"])
+def test_reduce_axis_random(gpu, op):
+    rng = np.random.default_rng(7)
+    shapes = [(3, 5), (4, 70), (2, 3000), (5000, 3), (7, 6, 5), (2, 3, 4, 2), (130, 2, 300), (1, 9)]
+    for shape in shapes:
+        lo, hi = (0.99, 1.01) if op == "prod" else (0.5, 1.5)
+        x = rng.uniform(lo, hi, shape).astype(np.float32)
+        for axis in list(range(len(shape))) + [(0, len(shape) - 1)]:
+            got = np.asarray(getattr(A(gpu, x), op)(axis=axis))
+            if op in ("maximum", "minimum"):
+                assert_bits(got, O.reduce_axis(op, x, axis))
+                continue
+            x64 = x.astype(np.float64)
+            ref = {"sum": x64.sum, "prod": x64.prod, "mean": x64.mean}[op](axis=axis)
+            k = x.size // max(np.asarray(ref).size, 1)
+            np.testing.assert_allclose(got, ref, rtol=4 * k * 2 ** -24 + 1e-7)
+
+
+def test_reduce_all_large_and_edge(gpu):
+    rng = np.random.default_rng(8)
+    for n in (1, 63, 64, 65, 4097, 100003, 1 << 22):
+        x = rng.standard_normal(n).astype(np.float32)
+        g = A(gpu, x)
+        s = float(np.asarray(g.sum())[0])
+        exact = float(np.sum(x.astype(np.float64)))
+        assert abs(s - exact) <= abs(exact) * 2 ** -24 + 1e-6 * np.sqrt(n)
+        assert float(np.asarray(g.maximum())[0]) == x.max()
+        assert float(np.asarray(g.minimum())[0]) == x.min()
+        mean = float(np.asarray(g.mean())[0])
+        assert abs(mean - exact / n) <= abs(exact / n) * 2 ** -23 + 1e-12
+    p = rng.uniform(0.999, 1.001, 1 << 20).astype(np.float32)
+    got = float(np.asarray(A(gpu, p).prod())[0])
+    ref = float(np.exp(np.sum(np.log(p.astype(np.float64)))))
+    assert abs(got - ref) <= ref * 2 ** -23
+
+
+def test_rebroadcast_random(gpu):
+    rng = np.random.default_rng(9)
+    for shape in ((4, 70), (2, 3000), (6, 5, 4), (300, 2, 3)):
+        x = rng.uniform(0.5, 1.5, shape).astype(np.float32)
+        for axis in range(len(shape)):
+            got = A(gpu, x).maximum(axis=axis, rebroadcast=True)
+            assert_bits(got, O.reduce_axis("maximum", x, axis, rebroadcast=True))
+            got = np.asarray(A(gpu, x).sum(axis=axis, rebroadcast=True))
+            ref = np.broadcast_to(x.astype(np.float64).sum(axis=axis, keepdims=True), shape)
+            np.testing.assert_allclose(got, ref, rtol=shape[axis] * 2 ** -22)
+
+
+def test_negative_axis_and_keepdims(gpu):
+    x = np.arange(24, dtype=np.float32).reshape(2, 3, 4)
+    g = A(gpu, x)
+    np.testing.assert_allclose(g.sum(axis=-1), x.sum(axis=-1))
+    np.testing.assert_allclose(g.sum(axis=(-1, 0), keepdims=True), x.sum(axis=(-1, 0), keepdims=True))
+    assert g.sum(axis=(-1, 0), keepdims=True).shape == (1, 3, 1)
+
+
+def test_setitem_getitem_and_flush(gpu):
+    a = A(gpu, np.arange(6).reshape(2, 3))
+    a[0, 1] = 10
+    b = a + 1
+    np.testing.assert_allclose(b, [[1, 11, 3], [4, 5, 6]])
+    assert a[1, 2] == 5
+    a[:] = 2.0
+    np.testing.assert_allclose(a + 0, np.full((2, 3), 2.0))
+    host = a.array
+    host[1, 1] = -1
+    a.flush()
+    np.testing.assert_allclose(a * 1, [[2, 2, 2], [2, -1, 2]])
+    z = vk.zeros(gpu, (3, 2))
+    np.testing.assert_allclose(z, np.zeros((3, 2)))
+
+
+def test_job_wait_semantics(gpu):
+    a = A(gpu, np.ones(1 << 16))
+    b = a + a
+    assert b.job is not None
+    b.job.wait(10 ** 10)
+    b.wait()
+    assert b.job is None
+    gpu.wait()

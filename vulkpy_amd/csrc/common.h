@@ -1,0 +1,220 @@
+// Internal helpers shared by the HIP translation units of libvulkpy_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/vulkpy_hip.h"
+
+namespace vkh {
+
+// ---- errors ---------------------------------------------------------------
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_error(const std::string& msg);
+void clear_error();
+
+[[noreturn]] inline void fail(int code, const std::string& msg) {
+  throw Error(code, msg);
+}
+
+inline void check_hip(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    int code = (e == hipErrorOutOfMemory) ? VKH_OUT_OF_MEMORY : VKH_ERROR;
+    fail(code, std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+#define VKH_HIP(call) ::vkh::check_hip((call), #call)
+
+// Every exported function body runs inside guarded(): exceptions become codes.
+template <typename F>
+int guarded(F&& f) {
+  try {
+    clear_error();
+    f();
+    return VKH_OK;
+  } catch (const Error& e) {
+    set_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error("host out of memory");
+    return VKH_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return VKH_ERROR;
+  }
+}
+
+#define VKH_REQUIRE(cond, msg) \
+  do { if (!(cond)) ::vkh::fail(VKH_INVALID, (msg)); } while (0)
+
+// ---- stream-ordered caching allocator -------------------------------------
+// Blocks are rounded to a size class and kept in per-class free lists after
+// vkh_free.  Reuse is safe without synchronisation because every user of a
+// context enqueues on the context's single in-order stream.
+class Allocator {
+ public:
+  explicit Allocator(int device) : device_(device) {}
+  int device() const { return device_; }
+  ~Allocator();
+  void* allocate(size_t nbytes);
+  void release(void* p);
+  void empty_cache();
+  size_t live_bytes() const { return live_bytes_; }
+  size_t cached_bytes() const { return cached_bytes_; }
+  static size_t round_size(size_t nbytes);
+
+ private:
+  int device_;
+  std::mutex mu_;
+  std::unordered_map<void*, size_t> live_;
+  std::unordered_map<size_t, std::vector<void*>> free_;
+  size_t live_bytes_ = 0;
+  size_t cached_bytes_ = 0;
+};
+
+}  // namespace vkh
+
+// ---- context ---------------------------------------------------------------
+struct vkh_gpu {
+  int device = 0;
+  float priority = 0.0f;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  vkh::Allocator* alloc = nullptr;  // null for contexts on external streams
+  int num_cus = 0;
+  int lds_bytes = 0;
+  int wave_size = 64;
+  bool capturing = false;        // stream capture in progress (vkh_graph_begin)
+  std::vector<void*> retained;   // workspaces allocated during the capture
+};
+
+struct vkh_event {
+  hipEvent_t ev = nullptr;
+  int device = 0;
+};
+
+namespace vkh {
+
+// Makes the context's device current for the calling thread.
+inline void use(vkh_gpu* g) {
+  VKH_REQUIRE(g != nullptr, "null GPU handle");
+  VKH_HIP(hipSetDevice(g->device));
+}
+
+// Temporary device buffer taken from the context allocator (or hipMalloc for
+// contexts without one) and returned when the holder goes out of scope; the
+// return is stream ordered, so kernels enqueued before release may still use
+// it.
+class Workspace {
+ public:
+  Workspace(vkh_gpu* g, size_t nbytes);
+  ~Workspace();
+  void* get() const { return p_; }
+  Workspace(const Workspace&) = delete;
+  Workspace& operator=(const Workspace&) = delete;
+
+ private:
+  vkh_gpu* g_;
+  void* p_ = nullptr;
+  bool graph_owned_ = false;
+};
+
+// Executable graph plus the workspace blocks its kernels reference.
+struct Graph {
+  vkh_gpu* gpu = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<void*> retained;
+};
+
+inline void check_launch(const char* name) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) fail(VKH_ERROR, std::string("launch of ") + name + ": " + hipGetErrorString(e));
+}
+
+inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// Grid size for a grid-stride streaming kernel: enough blocks to cover the
+// work, capped at `per_cu` resident blocks per compute unit.
+inline unsigned stream_grid(const vkh_gpu* g, uint64_t work_items, unsigned block,
+                            unsigned per_cu = 8) {
+  uint64_t need = ceil_div(work_items, block);
+  uint64_t cap = (uint64_t)(g->num_cus > 0 ? g->num_cus : 256) * per_cu;
+  if (need < 1) need = 1;
+  return (unsigned)(need < cap ? need : cap);
+}
+
+// ---- device math shared by elementwise, reduction and nn kernels ----------
+// max / min follow numpy.maximum / numpy.minimum (NaN propagates, ties keep a).
+__device__ __forceinline__ float np_max(float a, float b) {
+  return (a >= b || a != a) ? a : b;
+}
+__device__ __forceinline__ float np_min(float a, float b) {
+  return (a <= b || a != a) ? a : b;
+}
+
+// Transcendentals are evaluated in FP64 and rounded once to FP32, which keeps
+// them within one rounding of the exact value (the FP64 oracle).
+__device__ __forceinline__ float dexp(float x) { return (float)exp((double)x); }
+__device__ __forceinline__ float dlog(float x) { return (float)log((double)x); }
+
+__device__ __forceinline__ float binop_apply(int op, float a, float b) {
+  switch (op) {
+    case VKH_ADD: return a + b;
+    case VKH_SUB: return a - b;
+    case VKH_MUL: return a * b;
+    case VKH_DIV: return a / b;
+    case VKH_MAX: return np_max(a, b);
+    case VKH_MIN: return np_min(a, b);
+    case VKH_POW: return (float)pow((double)a, (double)b);
+    case VKH_RSUB: return b - a;
+    case VKH_RDIV: return b / a;
+    case VKH_RPOW: return (float)pow((double)b, (double)a);
+  }
+  return 0.0f;
+}
+
+template <int OP>
+__device__ __forceinline__ float binop(float a, float b) {
+  if constexpr (OP == VKH_ADD) return a + b;
+  else if constexpr (OP == VKH_SUB) return a - b;
+  else if constexpr (OP == VKH_MUL) return a * b;
+  else if constexpr (OP == VKH_DIV) return a / b;
+  else if constexpr (OP == VKH_MAX) return np_max(a, b);
+  else if constexpr (OP == VKH_MIN) return np_min(a, b);
+  else if constexpr (OP == VKH_POW) return (float)pow((double)a, (double)b);
+  else if constexpr (OP == VKH_RSUB) return b - a;
+  else if constexpr (OP == VKH_RDIV) return b / a;
+  else return (float)pow((double)b, (double)a);
+}
+
+// Dispatch a runtime binary op code to a template parameter.
+template <template <int> class K, typename... Args>
+void dispatch_binop(int op, Args&&... args) {
+  switch (op) {
+    case VKH_ADD: K<VKH_ADD>::run(args...); break;
+    case VKH_SUB: K<VKH_SUB>::run(args...); break;
+    case VKH_MUL: K<VKH_MUL>::run(args...); break;
+    case VKH_DIV: K<VKH_DIV>::run(args...); break;
+    case VKH_MAX: K<VKH_MAX>::run(args...); break;
+    case VKH_MIN: K<VKH_MIN>::run(args...); break;
+    case VKH_POW: K<VKH_POW>::run(args...); break;
+    case VKH_RSUB: K<VKH_RSUB>::run(args...); break;
+    case VKH_RDIV: K<VKH_RDIV>::run(args...); break;
+    case VKH_RPOW: K<VKH_RPOW>::run(args...); break;
+    default: fail(VKH_INVALID, "unknown binary op " + std::to_string(op));
+  }
+}
+
+}  // namespace vkh

@@ -1,0 +1,493 @@
+// Fused kernels of the nn path: activations and their backward, row softmax,
+// cross entropy (F19), MSE / Huber losses, loss gradients, optimizers and the
+// Dense bias gradient.
+//
+// Each kernel performs, per element, the same FP32 operations in the same
+// order as the Array-op composition in the reference's nn package (cited per
+// kernel), so the fused result equals the composed one bit for bit; row sums
+// over short rows are sequential like vulkpy/shader/sum_axis.comp:28-30.
+#include "common.h"
+
+namespace vkh {
+
+__device__ __forceinline__ float sign_np(float x) {
+  return x > 0.f ? 1.f : (x < 0.f ? -1.f : (x == 0.f ? 0.f : x));
+}
+
+// nn/layers.py:185 (x.max(0.0)) and 240-244 (0-x, exp, +1, 1/y)
+__device__ __forceinline__ float act_fwd(int act, float x) {
+  if (act == VKH_ACT_RELU) return np_max(x, 0.0f);
+  if (act == VKH_ACT_SIGMOID) {
+    float y = 0.0f - x;
+    y = dexp(y);
+    y = y + 1.0f;
+    return 1.0f / y;
+  }
+  return x;
+}
+
+// nn/layers.py:207-210 and 264-267
+__device__ __forceinline__ float act_bwd(int act, float y, float dy) {
+  if (act == VKH_ACT_RELU) {
+    float d = np_max(sign_np(y), 0.0f);
+    return d * dy;
+  }
+  if (act == VKH_ACT_SIGMOID) {
+    float d = 1.0f - y;
+    d = d * y;
+    return d * dy;
+  }
+  return dy;
+}
+
+__global__ void __launch_bounds__(256) k_act(int act, const float* x, float* y, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) y[i] = act_fwd(act, x[i]);
+}
+
+__global__ void __launch_bounds__(256) k_act_bwd(int act, const float* y, const float* dy, float* dx, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) dx[i] = act_bwd(act, y[i], dy[i]);
+}
+
+// ---- row softmax and cross entropy --------------------------------------------------
+// exp(x - max) with FP32 denormal results flushed to zero: the reference runs
+// on Vulkan drivers that flush denormals, and its own test expects
+// softmax([100, 0]) == [1, 0] exactly (test/test_nn.py:124-131), where the
+// IEEE result of exp(-100) is the denormal 3.8e-44.
+__device__ __forceinline__ float softmax_exp(float v) {
+  const float e = dexp(v);
+  return e < 1.17549435e-38f ? 0.0f : e;
+}
+
+// nn/layers.py:297-300: X = x - max(x); X = exp(X); X /= sum(X)
+__global__ void __launch_bounds__(256)
+k_softmax_rows_thread(const float* x, float* y, uint64_t rows, uint64_t cols) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x; r < rows; r += stride) {
+    const float* xr = x + r * cols;
+    float* yr = y + r * cols;
+    float m = xr[0];
+    for (uint64_t j = 0; j < cols; ++j) m = np_max(m, xr[j]);
+    float s = 0.0f;
+    for (uint64_t j = 0; j < cols; ++j) {
+      float e = softmax_exp(xr[j] - m);
+      yr[j] = e;
+      s += e;
+    }
+    for (uint64_t j = 0; j < cols; ++j) yr[j] = yr[j] / s;
+  }
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = np_max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ float block_max(float v, float* lds) {
+  v = wave_max(v);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  float r = lds[0];
+  for (int i = 1; i < 4; ++i) r = np_max(r, lds[i]);
+  __syncthreads();
+  return r;
+}
+__device__ __forceinline__ float block_sum(float v, float* lds) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  float r = lds[0] + lds[1] + lds[2] + lds[3];
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(256)
+k_softmax_rows_block(const float* x, float* y, uint64_t rows, uint64_t cols) {
+  __shared__ float lds[4];
+  for (uint64_t r = blockIdx.x; r < rows; r += gridDim.x) {
+    const float* xr = x + r * cols;
+    float* yr = y + r * cols;
+    float m = -__builtin_huge_valf();
+    for (uint64_t j = threadIdx.x; j < cols; j += 256) m = np_max(m, xr[j]);
+    m = block_max(m, lds);
+    float s = 0.0f;
+    for (uint64_t j = threadIdx.x; j < cols; j += 256) {
+      float e = softmax_exp(xr[j] - m);
+      yr[j] = e;
+      s += e;
+    }
+    s = block_sum(s, lds);
+    for (uint64_t j = threadIdx.x; j < cols; j += 256) yr[j] = yr[j] / s;
+  }
+}
+
+// nn_cross_entropy.comp:26 + sum(axis=1): L[r] = sum_j (-y) * log(x + 1e-8)
+__device__ __forceinline__ float xent_term(float x, float y) { return (-y) * dlog(x + 1e-8f); }
+
+__global__ void __launch_bounds__(256)
+k_xent_rows(const float* x, const float* y, float* L, uint64_t rows, uint64_t cols) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x; r < rows; r += stride) {
+    float acc = 0.0f;
+    for (uint64_t j = 0; j < cols; ++j) acc += xent_term(x[r * cols + j], y[r * cols + j]);
+    L[r] = acc;
+  }
+}
+
+// softmax + cross entropy per row (nn/losses.py:227), sequential rows
+__global__ void __launch_bounds__(256)
+k_sm_xent_rows(const float* x, const float* y, float* sm, float* L, uint64_t rows, uint64_t cols) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x; r < rows; r += stride) {
+    const float* xr = x + r * cols;
+    const float* yr = y + r * cols;
+    float* sr = sm + r * cols;
+    float m = xr[0];
+    for (uint64_t j = 0; j < cols; ++j) m = np_max(m, xr[j]);
+    float s = 0.0f;
+    for (uint64_t j = 0; j < cols; ++j) {
+      float e = softmax_exp(xr[j] - m);
+      sr[j] = e;
+      s += e;
+    }
+    float acc = 0.0f;
+    for (uint64_t j = 0; j < cols; ++j) {
+      float p = sr[j] / s;
+      sr[j] = p;
+      acc += xent_term(p, yr[j]);
+    }
+    L[r] = acc;
+  }
+}
+
+// nn/losses.py:294-296 (MSE) and 365-369 (Huber), per row, sequential sum
+template <int KIND>
+__global__ void __launch_bounds__(256)
+k_loss_rows(const float* x, const float* y, float* L, uint64_t rows, uint64_t cols) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x; r < rows; r += stride) {
+    float acc = 0.0f;
+    for (uint64_t j = 0; j < cols; ++j) {
+      float d = y[r * cols + j] - x[r * cols + j];
+      float t;
+      if (KIND == 0) {
+        t = d * d;  // (y-x) ** 2.0, exact square rounded once
+      } else {
+        float a = fabsf(d);
+        t = np_min(a, a * a);
+        t = t * 0.5f;
+      }
+      acc += t;
+    }
+    L[r] = acc;
+  }
+}
+
+// nn/losses.py:249, 318-320, 391-393 then ReduceLoss.grad's `dx *= scale`
+__global__ void __launch_bounds__(256)
+k_loss_grad(int kind, const float* x, const float* y, float* dx, uint64_t n, float scale) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+    float d = x[i] - y[i];
+    if (kind == 1) d = d * 2.0f;
+    else if (kind == 2) d = np_min(np_max(d, -1.0f), 1.0f);
+    if (scale != 1.0f) d = d * scale;
+    dx[i] = d;
+  }
+}
+
+// nn_cross_entropy_backward.comp:26 then `dx *= scale`
+__global__ void __launch_bounds__(256)
+k_xent_bwd(const float* x, const float* y, float* dx, uint64_t n, float scale) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+    float d = (-y[i]) / (x[i] + 1e-8f);
+    if (scale != 1.0f) d = d * scale;
+    dx[i] = d;
+  }
+}
+
+// ---- optimizers ----------------------------------------------------------------------
+struct AdamScalars {
+  float beta1, one_m_b1, beta2, one_m_b2, eps, neg_lr, c1, c2;
+};
+
+// nn/optimizers.py:235-253 and nn/parameters.py:95, in that op order
+__device__ __forceinline__ void adam_elem(float& w, float g, float& m, float& v, const AdamScalars& s) {
+  m = m * s.beta1;
+  m = m + g * s.one_m_b1;
+  v = v * s.beta2;
+  v = v + (g * g) * s.one_m_b2;
+  float mhat = m / s.c1;
+  float vhat = v / s.c2;
+  vhat = sqrtf(vhat);
+  vhat = vhat + s.eps;
+  mhat = mhat * s.neg_lr;
+  mhat = mhat / vhat;
+  w = w + mhat;
+}
+
+__global__ void __launch_bounds__(256)
+k_adam(float* w, const float* g, float* m, float* v, uint64_t n, AdamScalars s) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+    float wi = w[i], mi = m[i], vi = v[i];
+    adam_elem(wi, g[i], mi, vi, s);
+    w[i] = wi; m[i] = mi; v[i] = vi;
+  }
+}
+
+__device__ __forceinline__ uint16_t f2bf_nn(float x) {
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+// Beta powers live in device memory: bpow[0] = beta1^t, bpow[1] = beta2^t
+// (FP64 like the Python floats of AdamState).  k_adam_pow advances them once.
+__global__ void k_adam_pow(double* bpow, double beta1, double beta2) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    bpow[0] = bpow[0] * beta1;
+    bpow[1] = bpow[1] * beta2;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_adam_dev(float* w, const float* g, float* m, float* v, uint64_t n, AdamScalars s,
+           const double* bpow, uint16_t* wbf) {
+  s.c1 = (float)(1.0 - bpow[0]);
+  s.c2 = (float)(1.0 - bpow[1]);
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+    float wi = w[i], mi = m[i], vi = v[i];
+    adam_elem(wi, g[i], mi, vi, s);
+    w[i] = wi; m[i] = mi; v[i] = vi;
+    if (wbf) wbf[i] = f2bf_nn(wi);
+  }
+}
+
+// nn/optimizers.py:53: w += (-lr) * g
+__global__ void __launch_bounds__(256) k_sgd(float* w, const float* g, uint64_t n, float neg_lr) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) w[i] = w[i] + g[i] * neg_lr;
+}
+
+// nn/optimizers.py:134-140
+__global__ void __launch_bounds__(256)
+k_adagrad(float* w, const float* g, float* h, uint64_t n, float neg_lr, float eps) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+    float gi = g[i];
+    float hi = h[i] + gi * gi;
+    h[i] = hi;
+    float s = sqrtf(hi);
+    s = s + eps;
+    float r = gi / s;
+    r = r * neg_lr;
+    w[i] = w[i] + r;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_axpby(float* db, const float* t, uint64_t n, float beta) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+    db[i] = beta == 0.0f ? t[i] : db[i] * beta + t[i];
+}
+
+}  // namespace vkh
+
+using namespace vkh;
+
+static unsigned grid_for(vkh_gpu* g, uint64_t n) { return stream_grid(g, n, 256, 8); }
+
+extern "C" {
+
+int vkh_activation(vkh_gpu* g, int act, const float* x, float* y, uint64_t n) {
+  return guarded([&] {
+    use(g);
+    if (n == 0) return;
+    VKH_REQUIRE(x && y, "activation: null pointer");
+    hipLaunchKernelGGL(k_act, dim3(grid_for(g, n)), dim3(256), 0, g->stream, act, x, y, n);
+    check_launch("activation");
+  });
+}
+
+int vkh_activation_backward(vkh_gpu* g, int act, const float* y, const float* dy, float* dx, uint64_t n) {
+  return guarded([&] {
+    use(g);
+    if (n == 0) return;
+    VKH_REQUIRE(y && dy && dx, "activation_backward: null pointer");
+    hipLaunchKernelGGL(k_act_bwd, dim3(grid_for(g, n)), dim3(256), 0, g->stream, act, y, dy, dx, n);
+    check_launch("activation_backward");
+  });
+}
+
+int vkh_softmax(vkh_gpu* g, const float* x, float* y, uint64_t rows, uint64_t cols) {
+  return guarded([&] {
+    use(g);
+    if (rows == 0 || cols == 0) return;
+    VKH_REQUIRE(x && y, "softmax: null pointer");
+    if (cols <= 64) {
+      unsigned grid = (unsigned)std::min<uint64_t>(ceil_div(rows, 256), (uint64_t)g->num_cus * 8);
+      hipLaunchKernelGGL(k_softmax_rows_thread, dim3(grid), dim3(256), 0, g->stream, x, y, rows, cols);
+    } else {
+      unsigned grid = (unsigned)std::min<uint64_t>(rows, (uint64_t)g->num_cus * 16);
+      hipLaunchKernelGGL(k_softmax_rows_block, dim3(grid), dim3(256), 0, g->stream, x, y, rows, cols);
+    }
+    check_launch("softmax");
+  });
+}
+
+int vkh_cross_entropy(vkh_gpu* g, const float* x, const float* y, float* L, uint64_t rows, uint64_t cols) {
+  return guarded([&] {
+    use(g);
+    if (rows == 0) return;
+    VKH_REQUIRE(x && y && L, "cross_entropy: null pointer");
+    unsigned grid = (unsigned)std::min<uint64_t>(ceil_div(rows, 256), (uint64_t)g->num_cus * 8);
+    hipLaunchKernelGGL(k_xent_rows, dim3(grid), dim3(256), 0, g->stream, x, y, L, rows, cols);
+    check_launch("cross_entropy");
+  });
+}
+
+int vkh_cross_entropy_backward(vkh_gpu* g, const float* x, const float* y, float* dx, uint64_t n, float scale) {
+  return guarded([&] {
+    use(g);
+    if (n == 0) return;
+    VKH_REQUIRE(x && y && dx, "cross_entropy_backward: null pointer");
+    hipLaunchKernelGGL(k_xent_bwd, dim3(grid_for(g, n)), dim3(256), 0, g->stream, x, y, dx, n, scale);
+    check_launch("cross_entropy_backward");
+  });
+}
+
+int vkh_softmax_cross_entropy(vkh_gpu* g, const float* x, const float* y, float* sm, float* L,
+                              uint64_t rows, uint64_t cols) {
+  return guarded([&] {
+    use(g);
+    if (rows == 0 || cols == 0) return;
+    VKH_REQUIRE(x && y && sm && L, "softmax_cross_entropy: null pointer");
+    if (cols <= 64) {
+      unsigned grid = (unsigned)std::min<uint64_t>(ceil_div(rows, 256), (uint64_t)g->num_cus * 8);
+      hipLaunchKernelGGL(k_sm_xent_rows, dim3(grid), dim3(256), 0, g->stream, x, y, sm, L, rows, cols);
+      check_launch("softmax_cross_entropy");
+      return;
+    }
+    int r = vkh_softmax(g, x, sm, rows, cols);
+    if (r != VKH_OK) fail(r, vkh_last_error());
+    r = vkh_cross_entropy(g, sm, y, L, rows, cols);
+    if (r != VKH_OK) fail(r, vkh_last_error());
+  });
+}
+
+int vkh_mse_rows(vkh_gpu* g, const float* x, const float* y, float* L, uint64_t rows, uint64_t cols) {
+  return guarded([&] {
+    use(g);
+    if (rows == 0) return;
+    VKH_REQUIRE(x && y && L, "mse_rows: null pointer");
+    unsigned grid = (unsigned)std::min<uint64_t>(ceil_div(rows, 256), (uint64_t)g->num_cus * 8);
+    hipLaunchKernelGGL(k_loss_rows<0>, dim3(grid), dim3(256), 0, g->stream, x, y, L, rows, cols);
+    check_launch("mse_rows");
+  });
+}
+
+int vkh_huber_rows(vkh_gpu* g, const float* x, const float* y, float* L, uint64_t rows, uint64_t cols) {
+  return guarded([&] {
+    use(g);
+    if (rows == 0) return;
+    VKH_REQUIRE(x && y && L, "huber_rows: null pointer");
+    unsigned grid = (unsigned)std::min<uint64_t>(ceil_div(rows, 256), (uint64_t)g->num_cus * 8);
+    hipLaunchKernelGGL(k_loss_rows<1>, dim3(grid), dim3(256), 0, g->stream, x, y, L, rows, cols);
+    check_launch("huber_rows");
+  });
+}
+
+int vkh_loss_grad(vkh_gpu* g, int kind, const float* x, const float* y, float* dx, uint64_t n, float scale) {
+  return guarded([&] {
+    use(g);
+    if (n == 0) return;
+    VKH_REQUIRE(x && y && dx, "loss_grad: null pointer");
+    VKH_REQUIRE(kind >= 0 && kind <= 2, "loss_grad: unknown kind");
+    hipLaunchKernelGGL(k_loss_grad, dim3(grid_for(g, n)), dim3(256), 0, g->stream, kind, x, y, dx, n, scale);
+    check_launch("loss_grad");
+  });
+}
+
+int vkh_adam(vkh_gpu* g, float* w, const float* gr, float* m, float* v, uint64_t n, float lr,
+             float beta1, float beta2, float eps, double b1t, double b2t) {
+  return guarded([&] {
+    use(g);
+    if (n == 0) return;
+    VKH_REQUIRE(w && gr && m && v, "adam: null pointer");
+    AdamScalars s{beta1, (float)(1.0 - (double)beta1), beta2, (float)(1.0 - (double)beta2),
+                  eps, -lr, (float)(1.0 - b1t), (float)(1.0 - b2t)};
+    hipLaunchKernelGGL(k_adam, dim3(grid_for(g, n)), dim3(256), 0, g->stream, w, gr, m, v, n, s);
+    check_launch("adam");
+  });
+}
+
+int vkh_adam_pow_advance(vkh_gpu* g, double* bpow, double beta1, double beta2) {
+  return guarded([&] {
+    use(g);
+    VKH_REQUIRE(bpow, "adam_pow_advance: null pointer");
+    hipLaunchKernelGGL(k_adam_pow, dim3(1), dim3(64), 0, g->stream, bpow, beta1, beta2);
+    check_launch("adam_pow_advance");
+  });
+}
+
+int vkh_adam_device(vkh_gpu* g, float* w, const float* gr, float* m, float* v, uint64_t n, float lr,
+                    float beta1, float beta2, float eps, const double* bpow, uint16_t* w_bf16) {
+  return guarded([&] {
+    use(g);
+    VKH_REQUIRE(w && gr && m && v && bpow, "adam_device: null pointer");
+    if (n == 0) return;
+    AdamScalars s{beta1, (float)(1.0 - (double)beta1), beta2, (float)(1.0 - (double)beta2),
+                  eps, -lr, 1.0f, 1.0f};
+    hipLaunchKernelGGL(k_adam_dev, dim3(grid_for(g, n)), dim3(256), 0, g->stream, w, gr, m, v, n, s,
+                       bpow, w_bf16);
+    check_launch("adam_device");
+  });
+}
+
+int vkh_sgd(vkh_gpu* g, float* w, const float* gr, uint64_t n, float lr) {
+  return guarded([&] {
+    use(g);
+    if (n == 0) return;
+    VKH_REQUIRE(w && gr, "sgd: null pointer");
+    hipLaunchKernelGGL(k_sgd, dim3(grid_for(g, n)), dim3(256), 0, g->stream, w, gr, n, -lr);
+    check_launch("sgd");
+  });
+}
+
+int vkh_adagrad(vkh_gpu* g, float* w, const float* gr, float* h, uint64_t n, float lr, float eps) {
+  return guarded([&] {
+    use(g);
+    if (n == 0) return;
+    VKH_REQUIRE(w && gr && h, "adagrad: null pointer");
+    hipLaunchKernelGGL(k_adagrad, dim3(grid_for(g, n)), dim3(256), 0, g->stream, w, gr, h, n, -lr, eps);
+    check_launch("adagrad");
+  });
+}
+
+int vkh_bias_grad(vkh_gpu* g, const float* dy, float* db, uint64_t rows, uint64_t cols, float beta) {
+  return guarded([&] {
+    use(g);
+    if (cols == 0) return;
+    VKH_REQUIRE(dy && db, "bias_grad: null pointer");
+    Workspace ws(g, sizeof(float) * cols);
+    float* t = (float*)ws.get();
+    int r = vkh_reduce_axis(g, VKH_RSUM, dy, t, 1, rows, cols, 0, 1.0f, 1.0f);
+    if (r != VKH_OK) fail(r, vkh_last_error());
+    hipLaunchKernelGGL(k_axpby, dim3(grid_for(g, cols)), dim3(256), 0, g->stream, db, (const float*)t, cols, beta);
+    check_launch("bias_grad");
+  });
+}
+
+}  // extern "C"

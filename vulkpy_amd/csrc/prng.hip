@@ -1,0 +1,419 @@
+// Xoshiro128++ PRNG (F18), Box-Muller, randrange, and gather (F17).
+//
+// Seeding follows vulkpy/_vkarray.cc:643-675 bit for bit: four chained
+// splitmix64 outputs (589-595, 658-663) give state 0 and state i is the
+// reference's jump() (597-621) applied to state i-1.  That jump assigns the
+// state inside the loop over the four JUMP words without resetting its
+// accumulators, so it is not Vigna's canonical 2^64 jump, but it is still
+// linear over GF(2): the 128x128 bit matrix of one jump lets every host thread
+// start at J^lo(state 0) and walk its own range of states sequentially.
+//
+// Generation replaces the chunked host loop of _vkarray.cc:697-717 (one
+// dispatch of `size` threads per chunk, host wait between chunks) by one
+// kernel: thread i keeps state i in registers and produces outputs
+// i, i + S, i + 2S, ... < n, which is exactly the reference's mapping.
+#include "common.h"
+
+#include <array>
+#include <thread>
+
+namespace vkh {
+
+// ---- host seeding ---------------------------------------------------------------
+struct St {
+  uint32_t w[4];
+};
+
+static inline uint32_t rotl32(uint32_t x, int k) { return (x << k) | (x >> (32 - k)); }
+
+static inline void next_host(St& s) {
+  const uint32_t t = s.w[1] << 9;
+  s.w[2] ^= s.w[0];
+  s.w[3] ^= s.w[1];
+  s.w[1] ^= s.w[2];
+  s.w[0] ^= s.w[3];
+  s.w[2] ^= t;
+  s.w[3] = rotl32(s.w[3], 11);
+}
+
+static inline uint64_t splitmix64(uint64_t x) {
+  uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// the jump exactly as _vkarray.cc:597-621 writes it
+static inline void jump_as_written(St& s) {
+  static const uint32_t JUMP[4] = {0x8764000b, 0xf542d2d3, 0x6fa035c3, 0x77f2db5b};
+  uint32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  for (uint32_t j : JUMP) {
+    for (int b = 0; b < 32; ++b) {
+      if (j & (1u << b)) { a0 ^= s.w[0]; a1 ^= s.w[1]; a2 ^= s.w[2]; a3 ^= s.w[3]; }
+      next_host(s);
+    }
+    s.w[0] = a0; s.w[1] = a1; s.w[2] = a2; s.w[3] = a3;
+  }
+}
+
+using Mat = std::array<St, 128>;  // column k = image of basis vector e_k
+
+static St mat_apply(const Mat& M, const St& x) {
+  St y{{0, 0, 0, 0}};
+  for (int k = 0; k < 128; ++k)
+    if (x.w[k >> 5] & (1u << (k & 31)))
+      for (int q = 0; q < 4; ++q) y.w[q] ^= M[k].w[q];
+  return y;
+}
+
+static Mat compose(const Mat& A, const Mat& B) {  // A after B
+  Mat C;
+  for (int k = 0; k < 128; ++k) C[k] = mat_apply(A, B[k]);
+  return C;
+}
+
+static Mat identity() {
+  Mat I;
+  for (int k = 0; k < 128; ++k) {
+    I[k] = St{{0, 0, 0, 0}};
+    I[k].w[k >> 5] = 1u << (k & 31);
+  }
+  return I;
+}
+
+static Mat jump_matrix() {
+  Mat J;
+  for (int k = 0; k < 128; ++k) {
+    St e{{0, 0, 0, 0}};
+    e.w[k >> 5] = 1u << (k & 31);
+    jump_as_written(e);
+    J[k] = e;
+  }
+  return J;
+}
+
+static Mat mat_pow(Mat base, uint64_t p) {
+  Mat r = identity();
+  while (p) {
+    if (p & 1) r = compose(base, r);
+    base = compose(base, base);
+    p >>= 1;
+  }
+  return r;
+}
+
+static void seed_states(uint32_t S, uint64_t seed, std::vector<uint32_t>& out) {
+  out.resize((size_t)4 * S);
+  St s0;
+  for (int i = 0; i < 4; ++i) {
+    seed = splitmix64(seed);
+    s0.w[i] = (uint32_t)seed;
+  }
+  unsigned T = std::thread::hardware_concurrency();
+  if (T < 1) T = 1;
+  if (T > 16) T = 16;
+  if (S < 4096) T = 1;
+  const uint64_t per = ceil_div(S, T);
+  Mat J = T > 1 ? jump_matrix() : Mat{};
+  auto work = [&](uint64_t lo, uint64_t hi) {
+    St s = lo == 0 ? s0 : mat_apply(mat_pow(J, lo), s0);
+    for (uint64_t i = lo; i < hi; ++i) {
+      if (i > lo) jump_as_written(s);
+      std::memcpy(&out[4 * i], s.w, 16);
+    }
+  };
+  if (T == 1) { work(0, S); return; }
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < T; ++t) {
+    uint64_t lo = t * per, hi = std::min<uint64_t>(S, lo + per);
+    if (lo >= hi) break;
+    th.emplace_back(work, lo, hi);
+  }
+  for (auto& x : th) x.join();
+}
+
+// ---- device generation ------------------------------------------------------------
+__device__ __forceinline__ uint32_t drotl(uint32_t x, int k) { return (x << k) | (x >> (32 - k)); }
+
+// prng_xoshiro128pp_uint32.comp:32-42
+__device__ __forceinline__ uint32_t dnext(uint4& s) {
+  const uint32_t result = drotl(s.x + s.w, 7) + s.x;
+  const uint32_t t = s.y << 9;
+  s.z ^= s.x;
+  s.w ^= s.y;
+  s.y ^= s.z;
+  s.x ^= s.w;
+  s.z ^= t;
+  s.w = drotl(s.w, 11);
+  return result;
+}
+
+// prng_xoshiro128pp_float.comp:33
+__device__ __forceinline__ float to_unit(uint32_t r) {
+  return __uint_as_float((r >> 9) | 0x3f800000u) - 1.0f;
+}
+
+constexpr int PRNG_BLOCK = 256;
+
+// OUT: 0 = uint32, 1 = float in [0, 1), 2 = randrange
+template <int OUT>
+__global__ void __launch_bounds__(PRNG_BLOCK)
+k_xoshiro(uint4* state, void* out, uint32_t S, uint64_t n, uint32_t low, uint32_t range) {
+  const uint64_t i = (uint64_t)blockIdx.x * PRNG_BLOCK + threadIdx.x;
+  if (i >= S || i >= n) return;
+  uint4 s = state[i];
+  for (uint64_t k = i; k < n; k += S) {
+    const uint32_t r = dnext(s);
+    if (OUT == 0) static_cast<uint32_t*>(out)[k] = r;
+    else if (OUT == 1) static_cast<float*>(out)[k] = to_unit(r);
+    else {
+      // prng_randrange.comp:24-26: low + uint(float(range) * u)
+      const float u = to_unit(r);
+      static_cast<uint32_t*>(out)[k] = low + __float2uint_rz((float)range * u);
+    }
+  }
+  state[i] = s;
+}
+
+// Box-Muller of one pair of uniforms (prng_box_muller.comp:25-31), with the
+// transcendentals evaluated in FP64 and rounded once.
+__device__ __forceinline__ void box_muller(float u1, float u2, float mean, float sd, float& o1, float& o2) {
+  const float t = 1.0f - u1;
+  const float l = (float)log((double)t);
+  const float r = sqrtf(-2.0f * l) * sd;
+  const float angle = 6.28318530718f * u2;
+  o1 = mean + r * (float)sin((double)angle);
+  o2 = mean + r * (float)cos((double)angle);
+}
+
+// Fused uniform + Box-Muller: thread p owns states 2p and 2p+1 (S even) and
+// produces normal pairs at uniform indices j = c*S + 2p < nu (nu even).
+__global__ void __launch_bounds__(PRNG_BLOCK)
+k_xoshiro_normal(uint4* state, float* out, uint32_t S, uint64_t nu, uint64_t n, float mean, float sd) {
+  const uint64_t p = (uint64_t)blockIdx.x * PRNG_BLOCK + threadIdx.x;
+  const uint64_t i = 2 * p;
+  if (i >= S || i >= nu) return;
+  uint4 s1 = state[i], s2 = state[i + 1];
+  for (uint64_t j = i; j < nu; j += S) {
+    const float u1 = to_unit(dnext(s1));
+    const float u2 = to_unit(dnext(s2));
+    float o1, o2;
+    box_muller(u1, u2, mean, sd, o1, o2);
+    if (j < n) out[j] = o1;
+    if (j + 1 < n) out[j + 1] = o2;
+  }
+  state[i] = s1;
+  state[i + 1] = s2;
+}
+
+__global__ void __launch_bounds__(PRNG_BLOCK)
+k_box_muller(const float* u, float* out, uint64_t npairs, uint64_t n, float mean, float sd) {
+  const uint64_t stride = (uint64_t)gridDim.x * PRNG_BLOCK;
+  for (uint64_t p = (uint64_t)blockIdx.x * PRNG_BLOCK + threadIdx.x; p < npairs; p += stride) {
+    const uint64_t j = 2 * p;
+    const float u1 = u[j], u2 = u[j + 1];
+    float o1, o2;
+    box_muller(u1, u2, mean, sd, o1, o2);
+    if (j < n) out[j] = o1;
+    if (j + 1 < n) out[j + 1] = o2;
+  }
+}
+
+// ---- gather --------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_gather(const float* a, uint64_t na, const uint32_t* idx, float* c, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+    uint64_t k = idx[i];
+    c[i] = a[k < na ? k : na - 1];
+  }
+}
+
+// c[k, p, j] = a[p, min(idx[k], axis-1), j]
+__global__ void __launch_bounds__(256)
+k_gather_axis(const float* a, const uint32_t* idx, float* c, uint64_t prev, uint64_t axis,
+              uint64_t post, uint64_t nidx) {
+  const uint64_t total = nidx * prev * post;
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += stride) {
+    const uint64_t j = t % post;
+    const uint64_t q = t / post;
+    const uint64_t p = q % prev;
+    const uint64_t k = q / prev;
+    uint64_t b = idx[k];
+    if (b >= axis) b = axis - 1;
+    c[t] = a[(p * axis + b) * post + j];
+  }
+}
+
+}  // namespace vkh
+
+struct vkh_xoshiro {
+  vkh_gpu* gpu = nullptr;
+  uint32_t size = 0;
+  uint32_t* state = nullptr;  // [size][4]
+  bool from_alloc = false;
+};
+
+using namespace vkh;
+
+static unsigned prng_grid(uint64_t threads) { return (unsigned)ceil_div(threads, PRNG_BLOCK); }
+
+extern "C" {
+
+int vkh_xoshiro_create(vkh_gpu* g, uint32_t size, uint64_t seed, vkh_xoshiro** out) {
+  return guarded([&] {
+    use(g);
+    VKH_REQUIRE(out, "null out");
+    VKH_REQUIRE(size >= 1, "Xoshiro128pp size must be at least 1");
+    std::vector<uint32_t> host;
+    seed_states(size, seed, host);
+    auto* r = new vkh_xoshiro();
+    r->gpu = g;
+    r->size = size;
+    const size_t nbytes = sizeof(uint32_t) * 4 * (size_t)size;
+    try {
+      if (g->alloc) {
+        r->state = (uint32_t*)g->alloc->allocate(nbytes);
+        r->from_alloc = true;
+      } else {
+        VKH_HIP(hipMalloc(&r->state, nbytes));
+      }
+      // synchronous copy: the host vector dies at the end of this call
+      VKH_HIP(hipMemcpyAsync(r->state, host.data(), nbytes, hipMemcpyHostToDevice, g->stream));
+      VKH_HIP(hipStreamSynchronize(g->stream));
+    } catch (...) {
+      if (r->state) { if (r->from_alloc) g->alloc->release(r->state); else (void)hipFree(r->state); }
+      delete r;
+      throw;
+    }
+    *out = r;
+  });
+}
+
+int vkh_xoshiro_destroy(vkh_xoshiro* r) {
+  return guarded([&] {
+    if (!r) return;
+    use(r->gpu);
+    if (r->from_alloc) r->gpu->alloc->release(r->state);
+    else { VKH_HIP(hipStreamSynchronize(r->gpu->stream)); VKH_HIP(hipFree(r->state)); }
+    delete r;
+  });
+}
+
+int vkh_xoshiro_size(vkh_xoshiro* r, uint32_t* out) {
+  return guarded([&] { VKH_REQUIRE(r && out, "null argument"); *out = r->size; });
+}
+
+int vkh_xoshiro_state(vkh_xoshiro* r, uint32_t** out) {
+  return guarded([&] { VKH_REQUIRE(r && out, "null argument"); *out = r->state; });
+}
+
+int vkh_xoshiro_uint32(vkh_xoshiro* r, uint32_t* out, uint64_t n) {
+  return guarded([&] {
+    VKH_REQUIRE(r, "null generator");
+    use(r->gpu);
+    if (n == 0) return;
+    VKH_REQUIRE(out, "null output");
+    uint64_t th = std::min<uint64_t>(r->size, n);
+    hipLaunchKernelGGL(k_xoshiro<0>, dim3(prng_grid(th)), dim3(PRNG_BLOCK), 0, r->gpu->stream,
+                       (uint4*)r->state, (void*)out, r->size, n, 0u, 0u);
+    check_launch("xoshiro uint32");
+  });
+}
+
+int vkh_xoshiro_float(vkh_xoshiro* r, float* out, uint64_t n) {
+  return guarded([&] {
+    VKH_REQUIRE(r, "null generator");
+    use(r->gpu);
+    if (n == 0) return;
+    VKH_REQUIRE(out, "null output");
+    uint64_t th = std::min<uint64_t>(r->size, n);
+    hipLaunchKernelGGL(k_xoshiro<1>, dim3(prng_grid(th)), dim3(PRNG_BLOCK), 0, r->gpu->stream,
+                       (uint4*)r->state, (void*)out, r->size, n, 0u, 0u);
+    check_launch("xoshiro float");
+  });
+}
+
+int vkh_xoshiro_randrange(vkh_xoshiro* r, uint32_t* out, uint64_t n, uint32_t low, uint32_t high_incl) {
+  return guarded([&] {
+    VKH_REQUIRE(r, "null generator");
+    use(r->gpu);
+    if (n == 0) return;
+    VKH_REQUIRE(out, "null output");
+    uint32_t range = high_incl - low + 1u;  // uint32 arithmetic as in the shader
+    uint64_t th = std::min<uint64_t>(r->size, n);
+    hipLaunchKernelGGL(k_xoshiro<2>, dim3(prng_grid(th)), dim3(PRNG_BLOCK), 0, r->gpu->stream,
+                       (uint4*)r->state, (void*)out, r->size, n, low, range);
+    check_launch("xoshiro randrange");
+  });
+}
+
+int vkh_box_muller(vkh_gpu* g, const float* u, float* out, uint64_t n, float mean, float sd) {
+  return guarded([&] {
+    use(g);
+    if (n == 0) return;
+    VKH_REQUIRE(u && out, "null pointer");
+    uint64_t npairs = ceil_div(n, 2);
+    unsigned grid = stream_grid(g, npairs, PRNG_BLOCK, 8);
+    hipLaunchKernelGGL(k_box_muller, dim3(grid), dim3(PRNG_BLOCK), 0, g->stream, u, out, npairs, n, mean, sd);
+    check_launch("box_muller");
+  });
+}
+
+int vkh_xoshiro_normal(vkh_xoshiro* r, float* out, uint64_t n, float mean, float sd) {
+  return guarded([&] {
+    VKH_REQUIRE(r, "null generator");
+    vkh_gpu* g = r->gpu;
+    use(g);
+    if (n == 0) return;
+    VKH_REQUIRE(out, "null output");
+    const uint64_t nu = n + (n & 1);  // random.py:105-123: n even, or n + 1 uniforms
+    if (r->size % 2 == 0) {
+      uint64_t th = std::min<uint64_t>(r->size, nu) / 2;
+      hipLaunchKernelGGL(k_xoshiro_normal, dim3(prng_grid(th)), dim3(PRNG_BLOCK), 0, g->stream,
+                         (uint4*)r->state, out, r->size, nu, n, mean, sd);
+      check_launch("xoshiro normal");
+      return;
+    }
+    // odd number of states: uniforms first, then the transform
+    Workspace ws(g, sizeof(float) * nu);
+    float* u = (float*)ws.get();
+    uint64_t th = std::min<uint64_t>(r->size, nu);
+    hipLaunchKernelGGL(k_xoshiro<1>, dim3(prng_grid(th)), dim3(PRNG_BLOCK), 0, g->stream,
+                       (uint4*)r->state, (void*)u, r->size, nu, 0u, 0u);
+    check_launch("xoshiro float");
+    uint64_t npairs = nu / 2;
+    unsigned grid = stream_grid(g, npairs, PRNG_BLOCK, 8);
+    hipLaunchKernelGGL(k_box_muller, dim3(grid), dim3(PRNG_BLOCK), 0, g->stream, u, out, npairs, n, mean, sd);
+    check_launch("box_muller");
+  });
+}
+
+int vkh_gather(vkh_gpu* g, const float* a, uint64_t na, const uint32_t* idx, float* c, uint64_t n) {
+  return guarded([&] {
+    use(g);
+    if (n == 0) return;
+    VKH_REQUIRE(a && idx && c, "gather: null pointer");
+    VKH_REQUIRE(na >= 1, "gather from an empty array");
+    unsigned grid = stream_grid(g, n, 256, 8);
+    hipLaunchKernelGGL(k_gather, dim3(grid), dim3(256), 0, g->stream, a, na, idx, c, n);
+    check_launch("gather");
+  });
+}
+
+int vkh_gather_axis(vkh_gpu* g, const float* a, const uint32_t* idx, float* c, uint64_t prev,
+                    uint64_t axis, uint64_t post, uint64_t nidx) {
+  return guarded([&] {
+    use(g);
+    uint64_t total = nidx * prev * post;
+    if (total == 0) return;
+    VKH_REQUIRE(a && idx && c, "gather_axis: null pointer");
+    VKH_REQUIRE(axis >= 1, "gather along an empty axis");
+    unsigned grid = stream_grid(g, total, 256, 8);
+    hipLaunchKernelGGL(k_gather_axis, dim3(grid), dim3(256), 0, g->stream, a, idx, c, prev, axis, post, nidx);
+    check_launch("gather_axis");
+  });
+}
+
+}  // extern "C"
