@@ -230,6 +230,18 @@ int vkh_gemm(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
              float beta, float* C, uint64_t ldc, const float* bias, int act,
              uint16_t* c_bf16);
 
+/* vkh_gemm whose epilogue finally multiplies by act'(y) with y = aux[m, n]
+ * (aux_act: VKH_ACT_RELU -> max(sign(y), 0), VKH_ACT_SIGMOID -> (1-y)*y):
+ * the backward of an activation fused into the GEMM that produces its input
+ * gradient, i.e. Dense.backward's `dy @ W` (nn/layers.py:141) followed by
+ * ReLU.backward / Sigmoid.backward (nn/layers.py:207-210, 264-267) */
+int vkh_gemm_ex(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
+                uint64_t M, uint64_t N, uint64_t K, float alpha,
+                const float* A, uint64_t lda, const float* B, uint64_t ldb,
+                float beta, float* C, uint64_t ldc, const float* bias, int act,
+                uint16_t* c_bf16, const float* aux, uint64_t ldaux,
+                int aux_act);
+
 /* ---- nn fused kernels (F16, F19 and the nn package compositions) ------- */
 /* y = act(x) elementwise: ReLU = x.max(0.0) (nn/layers.py:185),
  * Sigmoid = 1/(1+exp(0-x)) (nn/layers.py:240-244) */

@@ -1,0 +1,94 @@
+"""The captured training step (nn.GraphSequence) against the eager
+Sequence.train path: identical weights in FP32 after every replay (with eager
+work and synchronisations between replays), and a BF16 run that tracks the
+FP32 loss."""
+import numpy as np
+import pytest
+
+import vulkpy_amd as vk
+from vulkpy_amd import nn
+
+pytestmark = pytest.mark.gpu
+
+
+def make_net(gpu, dims, acts, seed, reg=None, opt="adam"):
+    layers = []
+    for k, (i, o) in enumerate(dims):
+        if opt == "adam":
+            wo, bo = nn.Adam(gpu, lr=1e-3), nn.Adam(gpu, lr=1e-3)
+        else:
+            wo, bo = nn.SGD(0.05), nn.SGD(0.05)
+        layers.append(nn.Dense(gpu, i, o, w_init=nn.HeNormal(gpu, i, seed=seed + k),
+                               w_opt=wo, b_opt=bo, w_reg=reg() if reg else None))
+        if acts[k] == "relu":
+            layers.append(nn.ReLU())
+        elif acts[k] == "sigmoid":
+            layers.append(nn.Sigmoid())
+    return layers
+
+
+def weights(layers):
+    return [np.asarray(p).copy() for L in layers if isinstance(L, nn.Dense) for p in (L.w.value, L.b.value)]
+
+
+CASES = [
+    # (dims, acts, loss, regularizer, optimizer)
+    ([(20, 32), (32, 24), (24, 10)], ["relu", "sigmoid", None], "sce", None, "adam"),
+    ([(16, 64), (64, 64), (64, 1)], ["relu", "relu", None], "huber", lambda: nn.Ridge(1e-4), "adam"),
+    ([(8, 16), (16, 3)], ["sigmoid", None], "mse", lambda: nn.Lasso(1e-3), "sgd"),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_graph_fp32_matches_eager(gpu, case):
+    dims, acts, loss_name, reg, opt = CASES[case]
+    B = 96
+    rng = np.random.default_rng(case)
+    x = rng.standard_normal((B, dims[0][0])).astype(np.float32)
+    O = dims[-1][1]
+    if loss_name == "sce":
+        y = np.identity(O, dtype=np.float32)[rng.integers(0, O, B)]
+    else:
+        y = rng.standard_normal((B, O)).astype(np.float32)
+    mk_loss = {"sce": nn.SoftmaxCrossEntropyLoss, "huber": nn.HuberLoss, "mse": nn.MSELoss}[loss_name]
+    eager_layers = make_net(gpu, dims, acts, 100, reg, opt)
+    graph_layers = make_net(gpu, dims, acts, 100, reg, opt)
+    eager = nn.Sequence(eager_layers, mk_loss(), regularize=True)
+    graph = nn.GraphSequence(graph_layers, mk_loss(), precision="fp32", regularize=True)
+    X, Y = vk.Array(gpu, data=x), vk.Array(gpu, data=y)
+    for step in range(4):
+        _, le = eager.train(X, Y)
+        _, lg = graph.train(X, Y)
+        gpu.wait()
+        assert np.asarray(le).item() == np.asarray(lg).item(), step
+        for a, b in zip(weights(eager_layers), weights(graph_layers)):
+            np.testing.assert_array_equal(a, b)
+        # eager work between replays
+        (X * 2.0).sum()
+        gpu.wait()
+
+
+def test_graph_bf16_tracks_fp32(gpu):
+    dims, acts = [(32, 128), (128, 128), (128, 1)], ["relu", "relu", None]
+    B = 512
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal((B, 32)).astype(np.float32)
+    P = rng.standard_normal((32, 1)).astype(np.float32)
+    y = (np.sin(x @ P) + 0.1 * rng.standard_normal((B, 1))).astype(np.float32)
+    X, Y = vk.Array(gpu, data=x), vk.Array(gpu, data=y)
+    f32 = nn.GraphSequence(make_net(gpu, dims, acts, 5, lambda: nn.Ridge(1e-4)), nn.HuberLoss(),
+                           precision="fp32")
+    b16 = nn.GraphSequence(make_net(gpu, dims, acts, 5, lambda: nn.Ridge(1e-4)), nn.HuberLoss(),
+                           precision="bf16")
+    l32 = [float(np.asarray(f32.train(X, Y)[1])) for _ in range(60)]
+    l16 = [float(np.asarray(b16.train(X, Y)[1])) for _ in range(60)]
+    assert l32[-1] < 0.7 * l32[0]
+    assert abs(l16[-1] - l32[-1]) <= 0.05 * l32[-1] + 1e-3, (l16[-1], l32[-1])
+
+
+def test_graph_rejects_unsupported(gpu):
+    with pytest.raises(ValueError):
+        nn.GraphSequence([nn.ReLU()], nn.MSELoss())
+    net = nn.GraphSequence(make_net(gpu, [(4, 3)], [None], 1), nn.MSELoss(), precision="fp32")
+    with pytest.raises(ValueError):
+        net.train(vk.Array(gpu, shape=(8, 5)), vk.Array(gpu, shape=(8, 3)))

@@ -49,9 +49,11 @@ def test_normal_matches_oracle(gpu, size, n):
     r.random(shape=(5,)); o.random(5)
     got = np.asarray(r.normal(shape=(n,), mean=0.5, stddev=2.0))
     want = o.normal(n, 0.5, 2.0)
-    # FP64 libm vs device FP64 math can differ in the last FP64 bit, which
-    # moves the FP32 rounding by at most one ulp in rare cases
-    assert np.all(np.abs(got - want) <= np.spacing(np.abs(want)) * 1.01), np.abs(got - want).max()
+    # The kernel evaluates log / sin / cos in FP32 device math (within 2 ulp);
+    # the oracle uses correctly rounded values.  The last step, mean + r*s,
+    # can cancel, so the bound is 4 ulp of its larger operand.
+    scale = np.abs(want - np.float32(0.5)) + np.float32(0.5)
+    assert np.all(np.abs(got - want) <= 4 * np.spacing(scale)), np.abs(got - want).max()
     np.testing.assert_array_equal(r.state(), o.state)
 
 

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <mutex>
@@ -97,7 +98,11 @@ struct vkh_gpu {
   int lds_bytes = 0;
   int wave_size = 64;
   bool capturing = false;        // stream capture in progress (vkh_graph_begin)
-  std::vector<void*> retained;   // workspaces allocated during the capture
+  // Workspaces of kernels captured into a graph are bump-allocated from this
+  // arena, reserved before the capture begins (no allocation is legal while
+  // the stream captures) and owned by the resulting graph.
+  char* arena = nullptr;
+  size_t arena_size = 0, arena_used = 0, arena_hint = 0;
 };
 
 struct vkh_event {
@@ -131,11 +136,11 @@ class Workspace {
   bool graph_owned_ = false;
 };
 
-// Executable graph plus the workspace blocks its kernels reference.
+// Executable graph plus the workspace arena its kernels reference.
 struct Graph {
   vkh_gpu* gpu = nullptr;
   hipGraphExec_t exec = nullptr;
-  std::vector<void*> retained;
+  void* arena = nullptr;
 };
 
 inline void check_launch(const char* name) {
@@ -152,6 +157,34 @@ inline unsigned stream_grid(const vkh_gpu* g, uint64_t work_items, unsigned bloc
   uint64_t need = ceil_div(work_items, block);
   uint64_t cap = (uint64_t)(g->num_cus > 0 ? g->num_cus : 256) * per_cu;
   if (need < 1) need = 1;
+  return (unsigned)(need < cap ? need : cap);
+}
+
+// ---- streaming memory access ----------------------------------------------
+// Nontemporal 128-bit loads / stores for data touched once (operands larger
+// than the caches): measured 6.5-6.9 TB/s on MI355X against 5.8-6.2 TB/s for
+// ordinary accesses in the same one-float4-per-thread kernels.
+typedef float vkh_f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float4 ld_stream(const float4* p) {
+  const vkh_f4v v = __builtin_nontemporal_load(reinterpret_cast<const vkh_f4v*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st_stream(float4* p, const float4& v) {
+  vkh_f4v w = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(w, reinterpret_cast<vkh_f4v*>(p));
+}
+template <typename T>
+__device__ __forceinline__ void st_stream1(T* p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+
+// Grid for a one-work-item-per-thread kernel (with a grid-stride loop as a
+// guard for very large arrays).
+inline unsigned full_grid(uint64_t work_items, unsigned block) {
+  uint64_t need = ceil_div(work_items, block);
+  if (need < 1) need = 1;
+  const uint64_t cap = uint64_t(1) << 24;
   return (unsigned)(need < cap ? need : cap);
 }
 

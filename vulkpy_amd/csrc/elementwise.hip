@@ -74,34 +74,25 @@ __device__ __forceinline__ float4 apply4(const F& f, float4 a, float4 b, float4 
 }
 
 constexpr int EW_BLOCK = 256;
-constexpr int EW_UNROLL = 4;
 
 // Vectorised 2-D kernel: the first total4 float4 units of out, where out is
-// [rows, cols] with cols % 4 == 0 (or rows == 1) and cols4 = cols / 4.  Row
+// [rows, cols] with cols % 4 == 0 (or rows == 1) and cols4 = cols / 4.  One
+// unit per thread over a full grid (the loop only runs again for arrays
+// beyond 2^24 blocks); full-size operands and the output stream with
+// nontemporal accesses, broadcast operands use ordinary (cached) loads.  Row
 // and column of unit t come from one fast division (total4 < 2^31).
 template <class F, bool NEED_ROWCOL>
 __global__ void __launch_bounds__(EW_BLOCK)
 k_ew2d_vec(F f, Opnd a, Opnd b, Opnd c, float* out, uint32_t total4, FastDiv cols4) {
-  const uint32_t stride = gridDim.x * EW_BLOCK * EW_UNROLL;
-  for (uint32_t base = blockIdx.x * EW_BLOCK * EW_UNROLL + threadIdx.x; base < total4; base += stride) {
-    float4 r[EW_UNROLL];
-#pragma unroll
-    for (int u = 0; u < EW_UNROLL; ++u) {
-      uint32_t t = base + u * EW_BLOCK;
-      if (t < total4) {
-        uint32_t row = 0, col4 = t;
-        if constexpr (NEED_ROWCOL) { row = cols4.div(t); col4 = t - row * cols4.d; }
-        float4 va = load4(a, t, row, col4);
-        float4 vb = F::nin > 1 ? load4(b, t, row, col4) : va;
-        float4 vc = F::nin > 2 ? load4(c, t, row, col4) : va;
-        r[u] = apply4(f, va, vb, vc);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < EW_UNROLL; ++u) {
-      uint32_t t = base + u * EW_BLOCK;
-      if (t < total4) reinterpret_cast<float4*>(out)[t] = r[u];
-    }
+  const uint32_t stride = gridDim.x * EW_BLOCK;
+  for (uint32_t t = blockIdx.x * EW_BLOCK + threadIdx.x; t < total4; t += stride) {
+    uint32_t row = 0, col4 = t;
+    if constexpr (NEED_ROWCOL) { row = cols4.div(t); col4 = t - row * cols4.d; }
+    const float4 va = a.kind == K_FULL ? ld_stream(reinterpret_cast<const float4*>(a.p) + t) : load4(a, t, row, col4);
+    float4 vb = va, vc = va;
+    if (F::nin > 1) vb = b.kind == K_FULL ? ld_stream(reinterpret_cast<const float4*>(b.p) + t) : load4(b, t, row, col4);
+    if (F::nin > 2) vc = c.kind == K_FULL ? ld_stream(reinterpret_cast<const float4*>(c.p) + t) : load4(c, t, row, col4);
+    st_stream(reinterpret_cast<float4*>(out) + t, apply4(f, va, vb, vc));
   }
 }
 
@@ -112,10 +103,11 @@ k_ew2d_vec64(F f, Opnd a, Opnd b, Opnd c, float* out, uint64_t total4, uint64_t 
   const uint64_t stride = (uint64_t)gridDim.x * EW_BLOCK;
   for (uint64_t t = (uint64_t)blockIdx.x * EW_BLOCK + threadIdx.x; t < total4; t += stride) {
     uint64_t row = t / cols4, col4 = t - row * cols4;
-    float4 va = load4(a, t, row, col4);
-    float4 vb = F::nin > 1 ? load4(b, t, row, col4) : va;
-    float4 vc = F::nin > 2 ? load4(c, t, row, col4) : va;
-    reinterpret_cast<float4*>(out)[t] = apply4(f, va, vb, vc);
+    const float4 va = a.kind == K_FULL ? ld_stream(reinterpret_cast<const float4*>(a.p) + t) : load4(a, t, row, col4);
+    float4 vb = va, vc = va;
+    if (F::nin > 1) vb = b.kind == K_FULL ? ld_stream(reinterpret_cast<const float4*>(b.p) + t) : load4(b, t, row, col4);
+    if (F::nin > 2) vc = c.kind == K_FULL ? ld_stream(reinterpret_cast<const float4*>(c.p) + t) : load4(c, t, row, col4);
+    st_stream(reinterpret_cast<float4*>(out) + t, apply4(f, va, vb, vc));
   }
 }
 
@@ -259,7 +251,7 @@ static void run_ew(vkh_gpu* g, const F& f, const std::vector<InSpec>& ins,
     if (total4 > 0) {
       const uint64_t cols4 = rows == 1 ? total4 : cols / 4;
       if (total4 < (uint64_t(1) << 31)) {
-        unsigned grid = stream_grid(g, ceil_div(total4, EW_UNROLL), EW_BLOCK, 8);
+        unsigned grid = full_grid(total4, EW_BLOCK);
         FastDiv fd((uint32_t)cols4);
         if (need_rc)
           hipLaunchKernelGGL((k_ew2d_vec<F, true>), dim3(grid), dim3(EW_BLOCK), 0, g->stream,
@@ -340,8 +332,8 @@ __global__ void __launch_bounds__(EW_BLOCK)
 k_unary_vec(const float4* a, float4* b, uint64_t n4) {
   const uint64_t stride = (uint64_t)gridDim.x * EW_BLOCK;
   for (uint64_t t = (uint64_t)blockIdx.x * EW_BLOCK + threadIdx.x; t < n4; t += stride) {
-    float4 v = a[t];
-    b[t] = make_float4(unop<FN>(v.x), unop<FN>(v.y), unop<FN>(v.z), unop<FN>(v.w));
+    const float4 v = ld_stream(a + t);
+    st_stream(b + t, make_float4(unop<FN>(v.x), unop<FN>(v.y), unop<FN>(v.z), unop<FN>(v.w)));
   }
 }
 
@@ -360,7 +352,7 @@ static void run_unary(vkh_gpu* g, const float* a, float* b, uint64_t n) {
     uint64_t n4 = n / 4;
     vec = n4 * 4;
     if (n4) {
-      unsigned grid = stream_grid(g, n4, EW_BLOCK, 8);
+      unsigned grid = full_grid(n4, EW_BLOCK);
       hipLaunchKernelGGL(k_unary_vec<FN>, dim3(grid), dim3(EW_BLOCK), 0, g->stream,
                          (const float4*)a, (float4*)b, n4);
       check_launch("unary (vector)");

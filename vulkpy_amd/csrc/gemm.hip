@@ -60,6 +60,9 @@ struct GemmArgs {
   uint64_t M, N, K, lda, ldb, ldc;
   float alpha, beta;
   int act;
+  const float* aux;  // activation output whose derivative multiplies the result
+  uint64_t ldaux;
+  int aux_act;
   uint32_t tiles_m, tiles_n;
 };
 
@@ -67,6 +70,22 @@ __device__ __forceinline__ float act_apply(int act, float v) {
   if (act == VKH_ACT_RELU) return (v >= 0.0f || v != v) ? v : 0.0f;  // np.maximum(v, 0)
   if (act == VKH_ACT_SIGMOID) return 1.0f / (1.0f + (float)exp((double)(-v)));
   return v;
+}
+
+// dy * act'(y) given the activation's output y, in the op order of
+// nn/layers.py:207-210 (ReLU) and 264-267 (Sigmoid)
+__device__ __forceinline__ float act_deriv_mul(int act, float y, float dy) {
+  if (act == VKH_ACT_RELU) {
+    const float s = y > 0.f ? 1.f : (y < 0.f ? -1.f : (y == 0.f ? 0.f : y));
+    const float d = (s >= 0.f || s != s) ? s : 0.f;
+    return d * dy;
+  }
+  if (act == VKH_ACT_SIGMOID) {
+    float d = 1.0f - y;
+    d = d * y;
+    return d * dy;
+  }
+  return dy;
 }
 
 // Reads 4 consecutive elements of a row (guarded at the row end).
@@ -278,6 +297,7 @@ k_gemm(GemmArgs p, int vec_a, int vec_b) {
         if (p.beta != 0.f) v += p.beta * *cp;
         if (p.bias) v += bn;
         v = act_apply(p.act, v);
+        if (p.aux) v = act_deriv_mul(p.aux_act, p.aux[m * p.ldaux + n], v);
         *cp = v;
         if (p.Cbf) p.Cbf[m * p.ldc + n] = f2bf(v);
       }
@@ -311,30 +331,51 @@ using namespace vkh;
 
 extern "C" {
 
+static void gemm_impl(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M, uint64_t N,
+                      uint64_t K, float alpha, const float* A, uint64_t lda, const float* B,
+                      uint64_t ldb, float beta, float* Cp, uint64_t ldc, const float* bias, int act,
+                      uint16_t* c_bf16, const float* aux, uint64_t ldaux, int aux_act) {
+  use(g);
+  if (M == 0 || N == 0) return;
+  VKH_REQUIRE(Cp, "gemm: null C");
+  VKH_REQUIRE(K == 0 || (A && B), "gemm: null operand");
+  VKH_REQUIRE(ldc >= N, "gemm: ldc < N");
+  VKH_REQUIRE(trans_a ? lda >= M : lda >= K, "gemm: lda too small");
+  VKH_REQUIRE(trans_b ? ldb >= K : ldb >= N, "gemm: ldb too small");
+  VKH_REQUIRE(act >= VKH_ACT_NONE && act <= VKH_ACT_SIGMOID, "gemm: unknown activation");
+  VKH_REQUIRE(aux_act >= VKH_ACT_NONE && aux_act <= VKH_ACT_SIGMOID, "gemm: unknown aux activation");
+  VKH_REQUIRE(!aux || ldaux >= N, "gemm: ldaux < N");
+  GemmArgs a{};
+  a.A = A; a.B = B; a.C = Cp; a.bias = bias; a.Cbf = c_bf16;
+  a.M = M; a.N = N; a.K = K; a.lda = lda; a.ldb = ldb; a.ldc = ldc;
+  a.alpha = alpha; a.beta = beta; a.act = act;
+  a.aux = aux; a.ldaux = ldaux; a.aux_act = aux_act;
+  const int va = ((uintptr_t)A & 15u) == 0 && lda % 4 == 0;
+  const int vb = ((uintptr_t)B & 15u) == 0 && ldb % 4 == 0;
+  switch (mode) {
+    case VKH_GEMM_FP32: launch_mode<VKH_GEMM_FP32>(g, trans_a, trans_b, a, va, vb); break;
+    case VKH_GEMM_BF16: launch_mode<VKH_GEMM_BF16>(g, trans_a, trans_b, a, va, vb); break;
+    case VKH_GEMM_BF16X3: launch_mode<VKH_GEMM_BF16X3>(g, trans_a, trans_b, a, va, vb); break;
+    default: fail(VKH_INVALID, "gemm: unknown mode");
+  }
+}
+
 int vkh_gemm(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M, uint64_t N, uint64_t K,
              float alpha, const float* A, uint64_t lda, const float* B, uint64_t ldb, float beta,
              float* Cp, uint64_t ldc, const float* bias, int act, uint16_t* c_bf16) {
   return guarded([&] {
-    use(g);
-    if (M == 0 || N == 0) return;
-    VKH_REQUIRE(Cp, "gemm: null C");
-    VKH_REQUIRE(K == 0 || (A && B), "gemm: null operand");
-    VKH_REQUIRE(ldc >= N, "gemm: ldc < N");
-    VKH_REQUIRE(trans_a ? lda >= M : lda >= K, "gemm: lda too small");
-    VKH_REQUIRE(trans_b ? ldb >= K : ldb >= N, "gemm: ldb too small");
-    VKH_REQUIRE(act >= VKH_ACT_NONE && act <= VKH_ACT_SIGMOID, "gemm: unknown activation");
-    GemmArgs a{};
-    a.A = A; a.B = B; a.C = Cp; a.bias = bias; a.Cbf = c_bf16;
-    a.M = M; a.N = N; a.K = K; a.lda = lda; a.ldb = ldb; a.ldc = ldc;
-    a.alpha = alpha; a.beta = beta; a.act = act;
-    const int va = ((uintptr_t)A & 15u) == 0 && lda % 4 == 0;
-    const int vb = ((uintptr_t)B & 15u) == 0 && ldb % 4 == 0;
-    switch (mode) {
-      case VKH_GEMM_FP32: launch_mode<VKH_GEMM_FP32>(g, trans_a, trans_b, a, va, vb); break;
-      case VKH_GEMM_BF16: launch_mode<VKH_GEMM_BF16>(g, trans_a, trans_b, a, va, vb); break;
-      case VKH_GEMM_BF16X3: launch_mode<VKH_GEMM_BF16X3>(g, trans_a, trans_b, a, va, vb); break;
-      default: fail(VKH_INVALID, "gemm: unknown mode");
-    }
+    gemm_impl(g, mode, trans_a, trans_b, M, N, K, alpha, A, lda, B, ldb, beta, Cp, ldc, bias, act,
+              c_bf16, nullptr, 0, VKH_ACT_NONE);
+  });
+}
+
+int vkh_gemm_ex(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M, uint64_t N, uint64_t K,
+                float alpha, const float* A, uint64_t lda, const float* B, uint64_t ldb, float beta,
+                float* Cp, uint64_t ldc, const float* bias, int act, uint16_t* c_bf16,
+                const float* aux, uint64_t ldaux, int aux_act) {
+  return guarded([&] {
+    gemm_impl(g, mode, trans_a, trans_b, M, N, K, alpha, A, lda, B, ldb, beta, Cp, ldc, bias, act,
+              c_bf16, aux, ldaux, aux_act);
   });
 }
 

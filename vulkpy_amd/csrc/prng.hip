@@ -164,26 +164,30 @@ k_xoshiro(uint4* state, void* out, uint32_t S, uint64_t n, uint32_t low, uint32_
   uint4 s = state[i];
   for (uint64_t k = i; k < n; k += S) {
     const uint32_t r = dnext(s);
-    if (OUT == 0) static_cast<uint32_t*>(out)[k] = r;
-    else if (OUT == 1) static_cast<float*>(out)[k] = to_unit(r);
+    if (OUT == 0) st_stream1(static_cast<uint32_t*>(out) + k, r);
+    else if (OUT == 1) st_stream1(static_cast<float*>(out) + k, to_unit(r));
     else {
       // prng_randrange.comp:24-26: low + uint(float(range) * u)
       const float u = to_unit(r);
-      static_cast<uint32_t*>(out)[k] = low + __float2uint_rz((float)range * u);
+      st_stream1(static_cast<uint32_t*>(out) + k, low + __float2uint_rz((float)range * u));
     }
   }
   state[i] = s;
 }
 
-// Box-Muller of one pair of uniforms (prng_box_muller.comp:25-31), with the
-// transcendentals evaluated in FP64 and rounded once.
+// Box-Muller of one pair of uniforms (prng_box_muller.comp:25-31) in the
+// reference's FP32 operation order.  log, sin and cos are the device's FP32
+// functions (within 2 ulp, like the GLSL.std.450 ones the reference relies
+// on); FP64 versions would make the fused normal compute-bound.
 __device__ __forceinline__ void box_muller(float u1, float u2, float mean, float sd, float& o1, float& o2) {
   const float t = 1.0f - u1;
-  const float l = (float)log((double)t);
+  const float l = logf(t);
   const float r = sqrtf(-2.0f * l) * sd;
   const float angle = 6.28318530718f * u2;
-  o1 = mean + r * (float)sin((double)angle);
-  o2 = mean + r * (float)cos((double)angle);
+  float s, c;
+  sincosf(angle, &s, &c);
+  o1 = mean + r * s;
+  o2 = mean + r * c;
 }
 
 // Fused uniform + Box-Muller: thread p owns states 2p and 2p+1 (S even) and
@@ -199,8 +203,8 @@ k_xoshiro_normal(uint4* state, float* out, uint32_t S, uint64_t nu, uint64_t n, 
     const float u2 = to_unit(dnext(s2));
     float o1, o2;
     box_muller(u1, u2, mean, sd, o1, o2);
-    if (j < n) out[j] = o1;
-    if (j + 1 < n) out[j + 1] = o2;
+    if (j < n) st_stream1(out + j, o1);
+    if (j + 1 < n) st_stream1(out + j + 1, o2);
   }
   state[i] = s1;
   state[i + 1] = s2;

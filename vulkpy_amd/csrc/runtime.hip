@@ -86,16 +86,22 @@ Allocator::~Allocator() {
 
 Workspace::Workspace(vkh_gpu* g, size_t nbytes) : g_(g) {
   if (nbytes == 0) nbytes = 1;
-  if (g->alloc) {
-    p_ = g->alloc->allocate(nbytes);
-    // A captured graph keeps using its workspace on every replay, so the
-    // block stays out of the cache until the graph is destroyed.
-    if (g->capturing) {
-      g->retained.push_back(p_);
-      graph_owned_ = true;
+  if (g->capturing) {
+    // A captured graph uses this memory on every replay: take it from the
+    // graph's arena (nodes of one graph run in stream order, so a plain bump
+    // allocation never overlaps live data).
+    const size_t need = ceil_div(nbytes, 256) * 256;
+    if (!g->arena || g->arena_used + need > g->arena_size) {
+      g->arena_hint = g->arena_used + need;
+      fail(VKH_OUT_OF_MEMORY, "graph workspace arena exhausted (" + std::to_string(g->arena_size) +
+                                  " bytes); capture again to reserve a larger arena");
     }
+    p_ = g->arena + g->arena_used;
+    g->arena_used += need;
+    graph_owned_ = true;
+  } else if (g->alloc) {
+    p_ = g->alloc->allocate(nbytes);
   } else {
-    VKH_REQUIRE(!g->capturing, "workspace on an external stream during capture");
     VKH_HIP(hipMallocAsync(&p_, nbytes, g->stream));
   }
 }
@@ -239,6 +245,7 @@ int vkh_alloc(vkh_gpu* g, size_t nbytes, void** out) {
     use(g);
     VKH_REQUIRE(out, "null out");
     VKH_REQUIRE(g->alloc, "context on an external stream has no allocator");
+    VKH_REQUIRE(!g->capturing, "device allocation while a graph is being captured");
     *out = g->alloc->allocate(nbytes);
   });
 }
@@ -386,30 +393,47 @@ int vkh_graph_begin(vkh_gpu* g) {
   return guarded([&] {
     use(g);
     VKH_REQUIRE(!g->capturing, "graph capture already active on this context");
-    VKH_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+    VKH_REQUIRE(g->alloc, "graph capture needs a context with its own allocator");
+    // reserve the workspace arena before the stream starts capturing
+    const size_t want = std::max<size_t>(size_t(64) << 20, g->arena_hint);
+    g->arena = (char*)g->alloc->allocate(want);
+    g->arena_size = Allocator::round_size(want);
+    g->arena_used = 0;
+    hipError_t r = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal);
+    if (r != hipSuccess) {
+      g->alloc->release(g->arena);
+      g->arena = nullptr;
+      check_hip(r, "hipStreamBeginCapture");
+    }
     g->capturing = true;
-    g->retained.clear();
   });
 }
 
 int vkh_graph_end(vkh_gpu* g, void** exec) {
   return guarded([&] {
-    use(g);
-    VKH_REQUIRE(exec, "null out");
+    VKH_REQUIRE(g, "null GPU handle");
     VKH_REQUIRE(g->capturing, "no graph capture active on this context");
+    // End the capture before anything that may fail, so the stream is usable
+    // again whatever happened while capturing.
     g->capturing = false;
-    auto* gr = new vkh::Graph();
-    gr->gpu = g;
-    gr->retained.swap(g->retained);
     hipGraph_t graph = nullptr;
     hipError_t r = hipStreamEndCapture(g->stream, &graph);
+    auto* gr = new vkh::Graph();
+    gr->gpu = g;
+    gr->arena = g->arena;
+    g->arena = nullptr;
+    g->arena_size = g->arena_used = 0;
     if (r == hipSuccess) {
       r = hipGraphInstantiate(&gr->exec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
+    } else if (graph) {
+      (void)hipGraphDestroy(graph);
     }
-    if (r != hipSuccess) {
-      for (void* p : gr->retained) g->alloc->release(p);
+    if (r != hipSuccess || !exec) {
+      if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+      g->alloc->release(gr->arena);
       delete gr;
+      VKH_REQUIRE(exec, "null out");
       check_hip(r, "graph capture");
     }
     *exec = (void*)gr;
@@ -433,7 +457,7 @@ int vkh_graph_destroy(void* exec) {
     use(gr->gpu);
     VKH_HIP(hipStreamSynchronize(gr->gpu->stream));
     if (gr->exec) VKH_HIP(hipGraphExecDestroy(gr->exec));
-    for (void* p : gr->retained) gr->gpu->alloc->release(p);
+    if (gr->arena) gr->gpu->alloc->release(gr->arena);
     delete gr;
   });
 }

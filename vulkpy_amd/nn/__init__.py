@@ -43,3 +43,4 @@ from .regularizers import (
     Elastic,
 )
 from .models import Sequence
+from .graph import GraphSequence

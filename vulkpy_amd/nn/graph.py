@@ -1,0 +1,352 @@
+"""
+Captured training step (:mod:`vulkpy_amd.nn.graph`)
+===================================================
+
+``GraphSequence`` runs the whole ``Sequence.train`` step -- forward, loss,
+backward, regularizer gradients and optimizer update -- as ONE hipGraph replay
+over preallocated buffers:
+
+* forward: one GEMM per Dense with bias and the following ReLU / Sigmoid in
+  its epilogue;
+* loss: one fused row kernel plus the batch reduction; the loss gradient is
+  one kernel;
+* backward: per Dense, ``dW = dY^T X`` accumulated (GEMM beta = 1) onto the
+  regularizer gradient, ``db`` by a column sum, and ``dX = dY W`` with the
+  previous activation's derivative in the GEMM epilogue;
+* update: one kernel per parameter; Adam's beta powers live in device memory
+  and are advanced inside the graph (nn/optimizers.py:241-242), so no host
+  scalar changes between replays.
+
+In ``precision="fp32"`` every operation is the one ``Sequence.train`` performs
+(vulkpy/nn/models.py:55-78 with ``regularize=True``), in the same FP32 order,
+so both give identical weights.  ``"bf16"`` runs the GEMMs on BF16 matrix
+cores with FP32 accumulation and FP32 master weights.
+
+The arrays returned by ``train`` are the graph's own output buffers and are
+overwritten by the next call.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Iterable, List, Optional, Tuple
+
+import numpy as np
+
+from vulkpy_amd._lib import ACT, BINOP, GEMM_MODE, UNOP, call
+from vulkpy_amd.vkarray import Array
+from .core import Loss, Module
+from .layers import Dense, ReLU, Sigmoid
+from .losses import CrossEntropyLoss, HuberLoss, MSELoss, SoftmaxCrossEntropyLoss
+from .optimizers import AdaGradState, AdamState, SGDState
+from .parameters import Parameter, _has_fused_update
+from .regularizers import Elastic, Lasso, Ridge
+
+__all__ = ["GraphSequence"]
+
+
+class _Unit:
+    """One Dense layer and the activation that follows it (if any)."""
+
+    def __init__(self, dense: Dense, act: Optional[str]):
+        self.dense = dense
+        self.act = act
+
+
+def _units(layers) -> List[_Unit]:
+    units: List[_Unit] = []
+    for L in layers:
+        if isinstance(L, Dense):
+            units.append(_Unit(L, None))
+        elif isinstance(L, (ReLU, Sigmoid)):
+            if not units or units[-1].act is not None:
+                raise ValueError("GraphSequence: an activation must follow a Dense layer")
+            units[-1].act = "relu" if isinstance(L, ReLU) else "sigmoid"
+        else:
+            raise ValueError(f"GraphSequence does not support layer {type(L).__name__}")
+    if not units:
+        raise ValueError("GraphSequence needs at least one Dense layer")
+    return units
+
+
+class _Plan:
+    def __init__(self, seq: "GraphSequence", x_shape, y_shape):
+        self.key = (tuple(x_shape), tuple(y_shape))
+        self.seq = seq
+        gpu = seq.gpu
+        self.gpu = gpu
+        h = gpu._handle
+        self.h = h
+        self.mode = GEMM_MODE[seq.precision]
+        units = _units(seq.L)
+        self.units = units
+        B = int(x_shape[0])
+        self.B = B
+        if len(x_shape) != 2 or int(x_shape[1]) != units[0].dense.input_dim:
+            raise ValueError(f"input shape {x_shape} does not match the first Dense layer")
+        for a, b in zip(units, units[1:]):
+            if a.dense.output_dim != b.dense.input_dim:
+                raise ValueError("consecutive Dense dimensions do not match")
+        out_dim = units[-1].dense.output_dim
+        if tuple(y_shape) != (B, out_dim):
+            raise ValueError(f"target shape {y_shape} != {(B, out_dim)}")
+        loss = seq.loss
+        if isinstance(loss, SoftmaxCrossEntropyLoss):
+            self.loss_kind = "sce"
+        elif isinstance(loss, CrossEntropyLoss):
+            self.loss_kind = "ce"
+        elif isinstance(loss, MSELoss):
+            self.loss_kind = "mse"
+        elif isinstance(loss, HuberLoss):
+            self.loss_kind = "huber"
+        else:
+            raise ValueError(f"GraphSequence does not support loss {type(loss).__name__}")
+        self.mean = loss.scale_backward is not None
+        self.scale = float(np.float32(1 / B)) if self.mean else 1.0
+
+        # static buffers
+        self.x = Array(gpu, shape=(B, units[0].dense.input_dim))
+        self.y = Array(gpu, shape=(B, out_dim))
+        self.hs = [Array(gpu, shape=(B, u.dense.output_dim)) for u in units]
+        self.ds = [Array(gpu, shape=(B, u.dense.output_dim)) for u in units]
+        self.sm = Array(gpu, shape=(B, out_dim)) if self.loss_kind == "sce" else None
+        self.g_out = Array(gpu, shape=(B, out_dim)) if units[-1].act else None
+        self.L_rows = Array(gpu, shape=(B,))
+        self.loss = Array(gpu, shape=())
+        # parameters, their optimizer states and device beta powers
+        self.params: List[Parameter] = []
+        for u in units:
+            for p in (u.dense.w, u.dense.b):
+                if p.grad is None:
+                    continue
+                st = p.opt_state
+                if not _has_fused_update(st) or not isinstance(st, (AdamState, SGDState, AdaGradState)):
+                    raise ValueError(f"GraphSequence does not support optimizer state {type(st).__name__}")
+                self.params.append(p)
+        self.bpow = {}
+        for p in self.params:
+            st = p.opt_state
+            if isinstance(st, AdamState):
+                bp = np.array([st.beta1t, st.beta2t], dtype=np.float64)
+                buf = _DeviceDoubles(gpu, bp)
+                self.bpow[id(st)] = buf
+        gpu.wait()
+        self.graph = None
+        self._capture()
+
+    # ---- the step -------------------------------------------------------------------
+    def _enqueue(self):
+        h, mode, B = self.h, self.mode, self.B
+        units = self.units
+        # forward
+        src = self.x
+        for u, hout in zip(units, self.hs):
+            d = u.dense
+            call("vkh_gemm", h, mode, 0, 1, B, d.output_dim, d.input_dim, 1.0, src._ptr, d.input_dim,
+                 d.w.value._ptr, d.input_dim, 0.0, hout._ptr, d.output_dim, d.b.value._ptr,
+                 ACT[u.act], None)
+            src = hout
+        out = self.hs[-1]
+        O = units[-1].dense.output_dim
+        # loss rows, batch reduction and gradient w.r.t. the network output
+        gl = self.g_out if self.g_out is not None else self.ds[-1]
+        if self.loss_kind == "sce":
+            call("vkh_softmax_cross_entropy", h, out._ptr, self.y._ptr, self.sm._ptr, self.L_rows._ptr, B, O)
+            call("vkh_loss_grad", h, 0, self.sm._ptr, self.y._ptr, gl._ptr, B * O, self.scale)
+        elif self.loss_kind == "ce":
+            call("vkh_cross_entropy", h, out._ptr, self.y._ptr, self.L_rows._ptr, B, O)
+            call("vkh_cross_entropy_backward", h, out._ptr, self.y._ptr, gl._ptr, B * O, self.scale)
+        elif self.loss_kind == "mse":
+            call("vkh_mse_rows", h, out._ptr, self.y._ptr, self.L_rows._ptr, B, O)
+            call("vkh_loss_grad", h, 1, out._ptr, self.y._ptr, gl._ptr, B * O, self.scale)
+        else:
+            call("vkh_huber_rows", h, out._ptr, self.y._ptr, self.L_rows._ptr, B, O)
+            call("vkh_loss_grad", h, 2, out._ptr, self.y._ptr, gl._ptr, B * O, self.scale)
+        mul = float(np.float32(1 / B)) if self.mean else 1.0
+        call("vkh_reduce_axis", h, 0, self.L_rows._ptr, self.loss._ptr, 1, B, 1, 0, mul, 1.0)
+        if self.g_out is not None:
+            call("vkh_activation_backward", h, ACT[units[-1].act], out._ptr, self.g_out._ptr,
+                 self.ds[-1]._ptr, B * O)
+        # backward
+        for i in range(len(units) - 1, -1, -1):
+            d = units[i].dense
+            dy = self.ds[i]
+            xin = self.x if i == 0 else self.hs[i - 1]
+            for p, is_w in ((d.w, True), (d.b, False)):
+                if p.grad is None:
+                    continue
+                beta = 0.0
+                if self.seq.regularize and p.R is not None:
+                    self._enqueue_reg_grad(p)
+                    beta = 1.0
+                if is_w:
+                    call("vkh_gemm", h, mode, 1, 0, d.output_dim, d.input_dim, B, 1.0, dy._ptr,
+                         d.output_dim, xin._ptr, d.input_dim, beta, p.grad._ptr, d.input_dim,
+                         None, 0, None)
+                else:
+                    call("vkh_bias_grad", h, dy._ptr, p.grad._ptr, B, d.output_dim, beta)
+            if i > 0:
+                prev = units[i - 1]
+                call("vkh_gemm_ex", h, mode, 0, 0, B, d.input_dim, d.output_dim, 1.0, dy._ptr,
+                     d.output_dim, d.w.value._ptr, d.input_dim, 0.0, self.ds[i - 1]._ptr,
+                     d.input_dim, None, 0, None,
+                     self.hs[i - 1]._ptr if prev.act else None, d.input_dim, ACT[prev.act])
+        # optimizer
+        for p in self.params:
+            st = p.opt_state
+            n = p.value.size
+            if isinstance(st, AdamState):
+                bp = self.bpow[id(st)]
+                o = st.opt
+                call("vkh_adam_pow_advance", h, bp.ptr, float(o.beta1), float(o.beta2))
+                call("vkh_adam_device", h, p.value._ptr, p.grad._ptr, st.m._ptr, st.v._ptr, n,
+                     float(o.lr), float(o.beta1), float(o.beta2), float(o.eps), bp.ptr, None)
+            elif isinstance(st, SGDState):
+                call("vkh_sgd", h, p.value._ptr, p.grad._ptr, n, float(st.opt.lr))
+            else:
+                call("vkh_adagrad", h, p.value._ptr, p.grad._ptr, st.h._ptr, n,
+                     float(st.opt.lr), float(st.opt.eps))
+
+    def _enqueue_reg_grad(self, p: Parameter):
+        """grad = R.grad(value) (regularizers.py:23-192), before dW / db are added"""
+        h, n, R = self.h, p.value.size, p.R
+        if isinstance(R, Ridge):
+            call("vkh_binary_scalar", h, BINOP["mul"], p.value._ptr, float(2 * R.coeff), p.grad._ptr, n)
+        elif isinstance(R, Lasso):
+            call("vkh_unary", h, UNOP["sign"], p.value._ptr, p.grad._ptr, n)
+            call("vkh_binary_scalar", h, BINOP["mul"], p.grad._ptr, float(R.coeff), p.grad._ptr, n)
+        elif isinstance(R, Elastic):
+            tmp = self._scratch(p)
+            call("vkh_unary", h, UNOP["sign"], p.value._ptr, p.grad._ptr, n)
+            call("vkh_binary_scalar", h, BINOP["mul"], p.grad._ptr, float(R.L1.coeff), p.grad._ptr, n)
+            call("vkh_binary_scalar", h, BINOP["mul"], p.value._ptr, float(2 * R.L2.coeff), tmp._ptr, n)
+            call("vkh_binary", h, BINOP["add"], p.grad._ptr, tmp._ptr, p.grad._ptr, n)
+        else:
+            raise ValueError(f"GraphSequence does not support regularizer {type(R).__name__}")
+
+    def _scratch(self, p: Parameter) -> Array:
+        if not hasattr(self, "_scr"):
+            self._scr = {}
+        if id(p) not in self._scr:
+            self._scr[id(p)] = Array(self.gpu, shape=p.value.shape)
+        return self._scr[id(p)]
+
+    def _capture(self):
+        # buffers the step may allocate are created before the capture begins
+        for u in self.units:
+            for p in (u.dense.w, u.dense.b):
+                if p.grad is not None and self.seq.regularize and isinstance(p.R, Elastic):
+                    self._scratch(p)
+        self.gpu.wait()
+        call("vkh_graph_begin", self.h)
+        try:
+            self._enqueue()
+        except BaseException:
+            ex = ctypes.c_void_p()
+            from vulkpy_amd import _lib
+            _lib.lib.vkh_graph_end(self.h, ctypes.byref(ex))
+            if ex.value:
+                _lib.lib.vkh_graph_destroy(ex.value)
+            raise
+        ex = ctypes.c_void_p()
+        call("vkh_graph_end", self.h, ctypes.byref(ex))
+        self.graph = ex.value
+
+    def run(self, x: Array, y: Array) -> Tuple[Array, Array]:
+        h = self.h
+        call("vkh_copy", h, self.x._ptr, x._ptr, self.x.buffer.nbytes)
+        call("vkh_copy", h, self.y._ptr, y._ptr, self.y.buffer.nbytes)
+        call("vkh_graph_launch", h, self.graph)
+        for p in self.params:
+            st = p.opt_state
+            if isinstance(st, AdamState):
+                st.beta1t *= st.opt.beta1
+                st.beta2t *= st.opt.beta2
+            p.value._set_job()
+            p.grad._set_job()
+            for name in ("m", "v", "h"):
+                arr = getattr(st, name, None)
+                if isinstance(arr, Array):
+                    arr._set_job()
+        for a in self.hs + [self.loss, self.L_rows]:
+            a._set_job()
+        return self.hs[-1], self.loss
+
+    def __del__(self):
+        try:
+            if self.graph:
+                from vulkpy_amd import _lib
+                _lib.lib.vkh_graph_destroy(self.graph)
+                self.graph = None
+        except Exception:
+            pass
+
+
+class _DeviceDoubles:
+    """Small FP64 device buffer (Adam beta powers)."""
+
+    def __init__(self, gpu, values: np.ndarray):
+        from vulkpy_amd.vkarray import _DeviceBuffer
+        self._host = np.ascontiguousarray(values, dtype=np.float64)
+        self._buf = _DeviceBuffer(gpu, self._host.size, np.float64)
+        call("vkh_upload", gpu._handle, self._buf.ptr, self._host.ctypes.data, self._host.nbytes)
+        gpu.wait()
+
+    @property
+    def ptr(self):
+        return self._buf.ptr
+
+
+class GraphSequence:
+    """
+    Sequential model whose training step is one captured hipGraph
+
+    Parameters
+    ----------
+    layers : iterable of vulkpy_amd.nn.Module
+        Dense layers, each optionally followed by ReLU or Sigmoid.
+    loss : vulkpy_amd.nn.Loss
+        SoftmaxCrossEntropyLoss, CrossEntropyLoss, MSELoss or HuberLoss.
+    precision : {"bf16", "fp32", "bf16x3"}
+        Product precision of the GEMMs (FP32 accumulation and FP32 master
+        weights in every mode).
+    regularize : bool
+        Add each parameter's regularizer gradient before the update
+        (``Parameter.regular_grad``).  The reference's ``Sequence.train``
+        never does (vulkpy/nn/models.py:55-78).
+
+    Notes
+    -----
+    The graph is captured on the first ``train`` call for a given pair of
+    input / target shapes and replayed afterwards.
+    """
+
+    def __init__(self, layers: Iterable[Module], loss: Loss, *,
+                 precision: str = "bf16", regularize: bool = True):
+        if precision not in GEMM_MODE:
+            raise ValueError(f"unknown precision {precision!r}")
+        self.L: Tuple[Module, ...] = tuple(layers)
+        self.loss = loss
+        self.precision = precision
+        self.regularize = regularize
+        self._plan: Optional[_Plan] = None
+        units = _units(self.L)
+        self.gpu = units[0].dense.w.value._gpu
+
+    def train(self, x: Array, y: Array) -> Tuple[Array, Array]:
+        """
+        One training step; returns (prediction, loss) held in the graph's
+        output buffers.
+        """
+        key = (tuple(x.shape), tuple(y.shape))
+        if self._plan is None or self._plan.key != key:
+            self._plan = None
+            self._plan = _Plan(self, x.shape, y.shape)
+        return self._plan.run(x, y)
+
+    def predict(self, x: Array, y: Optional[Array] = None):
+        """Forward (and loss) through the layers' ordinary path."""
+        for L in self.L:
+            x = L(x)
+        if y is None:
+            return x
+        return x, self.loss(x, y)

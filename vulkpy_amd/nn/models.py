@@ -9,6 +9,7 @@ from typing import Iterable, Optional, Tuple, Union
 
 from vulkpy_amd.vkarray import Array
 from .core import Module, Loss
+from .parameters import Parameter
 
 
 __all__ = ["Sequence"]
@@ -22,9 +23,24 @@ class Sequence:
     """
     def __init__(self,
                  layers: Iterable[Module],
-                 loss: Loss):
+                 loss: Loss, *,
+                 regularize: bool = False):
+        """
+        Parameters
+        ----------
+        layers : iterable of vulkpy_amd.nn.Module
+            Layers to be called sequentially
+        loss : vulkpy_amd.nn.Loss
+            Loss layer
+        regularize : bool, optional
+            If ``True``, ``train`` adds every parameter's regularizer
+            gradient (``Parameter.regular_grad``) between backward and update.
+            The default ``False`` is the reference's behaviour, where
+            regularizers never enter training (vulkpy/nn/models.py:55-78).
+        """
         self.L: Tuple[Module, ...] = tuple(layers)
         self.loss: Loss = loss
+        self.regularize = regularize
 
     def _forward(self, x: Array) -> Array:
         for _L in self.L:
@@ -44,6 +60,12 @@ class Sequence:
         for _L in self.L:
             _L.update()
 
+    def _regularize(self):
+        for _L in self.L:
+            for p in vars(_L).values():
+                if isinstance(p, Parameter):
+                    p.regular_grad()
+
     def train(self, x: Array, y: Array) -> Tuple[Array, Array]:
         """
         Train model: forward, loss, zero_grad, backward, update
@@ -61,6 +83,8 @@ class Sequence:
 
         self._zero_grad()
         self._backward()
+        if self.regularize:
+            self._regularize()
         self._update()
 
         return _y, _loss
