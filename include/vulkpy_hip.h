@@ -230,6 +230,15 @@ int vkh_gemm(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
              float beta, float* C, uint64_t ldc, const float* bias, int act,
              uint16_t* c_bf16);
 
+/* BF16 / BF16X3 GEMMs pack their operands into a BF16 area: a per-context
+ * area that grows on demand for eager calls, and a region of the graph's
+ * arena for calls captured into a graph.  This grows the former for an
+ * M x N x K product ahead of time and records the size so that the next
+ * vkh_graph_begin reserves a large enough region (a capture cannot allocate;
+ * a captured GEMM that does not fit fails and records its size for the next
+ * capture).  Refused during a capture.  No counterpart in the reference. */
+int vkh_gemm_reserve(vkh_gpu* gpu, int mode, uint64_t M, uint64_t N,
+                     uint64_t K);
 /* vkh_gemm whose epilogue finally multiplies by act'(y) with y = aux[m, n]
  * (aux_act: VKH_ACT_RELU -> max(sign(y), 0), VKH_ACT_SIGMOID -> (1-y)*y):
  * the backward of an activation fused into the GEMM that produces its input
@@ -278,10 +287,12 @@ int vkh_huber_rows(vkh_gpu* gpu, const float* x, const float* y, float* L,
 int vkh_loss_grad(vkh_gpu* gpu, int kind, const float* x, const float* y,
                   float* dx, uint64_t n, float scale);
 /* Adam update in the op order of AdamState.grad2diff + Parameter.update
- * (nn/optimizers.py:235-253, nn/parameters.py:94-95), FP32 throughout;
- * b1t / b2t are the already-updated beta powers (FP64, as the Python floats) */
+ * (nn/optimizers.py:235-253, nn/parameters.py:94-95), FP32 arithmetic on
+ * scalars formed like the Python ones: beta, 1-beta, 1-beta^t are computed in
+ * FP64 and then rounded to FP32; b1t / b2t are the already-updated beta
+ * powers */
 int vkh_adam(vkh_gpu* gpu, float* w, const float* g, float* m, float* v,
-             uint64_t n, float lr, float beta1, float beta2, float eps,
+             uint64_t n, float lr, double beta1, double beta2, float eps,
              double b1t, double b2t);
 /* bpow[0] *= beta1, bpow[1] *= beta2 in FP64 on the device (the
  * `beta1t *= beta1` of nn/optimizers.py:241-242), so a captured training
@@ -291,7 +302,7 @@ int vkh_adam_pow_advance(vkh_gpu* gpu, double* bpow, double beta1,
 /* Adam update reading the beta powers from device memory bpow[0..1];
  * w_bf16 (may be NULL) also receives the updated weights as BF16 */
 int vkh_adam_device(vkh_gpu* gpu, float* w, const float* g, float* m, float* v,
-                    uint64_t n, float lr, float beta1, float beta2, float eps,
+                    uint64_t n, float lr, double beta1, double beta2, float eps,
                     const double* bpow, uint16_t* w_bf16);
 /* SGD (nn/optimizers.py:53) and AdaGrad (nn/optimizers.py:134-140) */
 int vkh_sgd(vkh_gpu* gpu, float* w, const float* g, uint64_t n, float lr);

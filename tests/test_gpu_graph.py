@@ -11,7 +11,7 @@ from vulkpy_amd import nn
 pytestmark = pytest.mark.gpu
 
 
-def make_net(gpu, dims, acts, seed, reg=None, opt="adam"):
+def make_net(gpu, dims, acts, seed, reg=None, opt="adam", precision=None):
     layers = []
     for k, (i, o) in enumerate(dims):
         if opt == "adam":
@@ -19,7 +19,8 @@ def make_net(gpu, dims, acts, seed, reg=None, opt="adam"):
         else:
             wo, bo = nn.SGD(0.05), nn.SGD(0.05)
         layers.append(nn.Dense(gpu, i, o, w_init=nn.HeNormal(gpu, i, seed=seed + k),
-                               w_opt=wo, b_opt=bo, w_reg=reg() if reg else None))
+                               w_opt=wo, b_opt=bo, w_reg=reg() if reg else None,
+                               precision=precision))
         if acts[k] == "relu":
             layers.append(nn.ReLU())
         elif acts[k] == "sigmoid":
@@ -84,6 +85,36 @@ def test_graph_bf16_tracks_fp32(gpu):
     l16 = [float(np.asarray(b16.train(X, Y)[1])) for _ in range(60)]
     assert l32[-1] < 0.7 * l32[0]
     assert abs(l16[-1] - l32[-1]) <= 0.05 * l32[-1] + 1e-3, (l16[-1], l32[-1])
+
+
+@pytest.mark.parametrize("precision", ["bf16", "bf16x3"])
+def test_graph_packed_matches_eager_with_interleaved_gemms(gpu, precision):
+    """Captured BF16 GEMMs pack into their graph's own region: eager BF16
+    products between replays (one of them larger than any product of the
+    step, which regrows the eager packing area) leave the replays identical
+    to the eager step with the same kernels."""
+    dims, acts = [(24, 96), (96, 80), (80, 5)], ["relu", "sigmoid", None]
+    B = 160
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal((B, 24)).astype(np.float32)
+    y = np.identity(5, dtype=np.float32)[rng.integers(0, 5, B)]
+    X, Y = vk.Array(gpu, data=x), vk.Array(gpu, data=y)
+    eager_layers = make_net(gpu, dims, acts, 40, lambda: nn.Ridge(1e-3), precision=precision)
+    graph_layers = make_net(gpu, dims, acts, 40, lambda: nn.Ridge(1e-3), precision=precision)
+    eager = nn.Sequence(eager_layers, nn.SoftmaxCrossEntropyLoss(), regularize=True)
+    graph = nn.GraphSequence(graph_layers, nn.SoftmaxCrossEntropyLoss(), precision=precision,
+                             regularize=True)
+    for step in range(4):
+        _, le = eager.train(X, Y)
+        _, lg = graph.train(X, Y)
+        gpu.wait()
+        assert np.asarray(le).item() == np.asarray(lg).item(), step
+        for a, b in zip(weights(eager_layers), weights(graph_layers)):
+            np.testing.assert_array_equal(a, b)
+        n = 300 + 150 * step  # grows beyond every product of the step
+        sq = vk.Array(gpu, data=rng.standard_normal((n, n)).astype(np.float32))
+        sq.matmul(sq, precision=precision)
+        gpu.wait()
 
 
 def test_graph_rejects_unsupported(gpu):

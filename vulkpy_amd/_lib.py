@@ -97,6 +97,7 @@ _SIGNATURES = {
     "vkh_box_muller": (c_int, [P, P, P, U64, c_float, c_float]),
     "vkh_gemm": (c_int, [P, c_int, c_int, c_int, U64, U64, U64, c_float, P, U64, P, U64,
                          c_float, P, U64, P, c_int, P]),
+    "vkh_gemm_reserve": (c_int, [P, c_int, U64, U64, U64]),
     "vkh_gemm_ex": (c_int, [P, c_int, c_int, c_int, U64, U64, U64, c_float, P, U64, P, U64,
                             c_float, P, U64, P, c_int, P, P, U64, c_int]),
     "vkh_activation": (c_int, [P, c_int, P, P, U64]),
@@ -108,9 +109,9 @@ _SIGNATURES = {
     "vkh_mse_rows": (c_int, [P, P, P, P, U64, U64]),
     "vkh_huber_rows": (c_int, [P, P, P, P, U64, U64]),
     "vkh_loss_grad": (c_int, [P, c_int, P, P, P, U64, c_float]),
-    "vkh_adam": (c_int, [P, P, P, P, P, U64, c_float, c_float, c_float, c_float, c_double, c_double]),
+    "vkh_adam": (c_int, [P, P, P, P, P, U64, c_float, c_double, c_double, c_float, c_double, c_double]),
     "vkh_adam_pow_advance": (c_int, [P, P, c_double, c_double]),
-    "vkh_adam_device": (c_int, [P, P, P, P, P, U64, c_float, c_float, c_float, c_float, P, P]),
+    "vkh_adam_device": (c_int, [P, P, P, P, P, U64, c_float, c_double, c_double, c_float, P, P]),
     "vkh_sgd": (c_int, [P, P, P, U64, c_float]),
     "vkh_adagrad": (c_int, [P, P, P, P, U64, c_float, c_float]),
     "vkh_bias_grad": (c_int, [P, P, P, U64, U64, c_float]),

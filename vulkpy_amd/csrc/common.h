@@ -103,6 +103,15 @@ struct vkh_gpu {
   // the stream captures) and owned by the resulting graph.
   char* arena = nullptr;
   size_t arena_size = 0, arena_used = 0, arena_hint = 0;
+  // grow-only BF16 packing area of the matrix-core GEMMs (gemm.hip) for
+  // eager calls
+  void* pack = nullptr;
+  size_t pack_bytes = 0;
+  // packing area of GEMMs captured into a graph: a region of that graph's
+  // arena shared by all of its GEMM nodes (they run in stream order), sized
+  // from graph_pack_hint (vkh_gemm_reserve) when the capture begins
+  void* graph_pack = nullptr;
+  size_t graph_pack_bytes = 0, graph_pack_hint = 0;
 };
 
 struct vkh_event {

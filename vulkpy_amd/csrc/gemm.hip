@@ -21,6 +21,9 @@ namespace vkh {
 
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8_t;
 typedef __attribute__((ext_vector_type(16))) float f32x16_t;
+// register staging of 8 BF16 values; a native vector (unlike HIP's uint4
+// struct) stays in VGPRs when held in arrays across the K loop
+typedef __attribute__((ext_vector_type(4))) uint32_t u32x4_t;
 
 __device__ __forceinline__ uint16_t f2bf(float x) {
   // round to nearest even (NaN kept quiet)
@@ -63,6 +66,7 @@ struct GemmArgs {
   const float* aux;  // activation output whose derivative multiplies the result
   uint64_t ldaux;
   int aux_act;
+  int vec_c;  // C, bias, aux, Cbf rows allow 16-byte (Cbf: 8-byte) accesses
   uint32_t tiles_m, tiles_n;
 };
 
@@ -319,6 +323,391 @@ static void launch_mode(vkh_gpu* g, bool ta, bool tb, GemmArgs& a, int va, int v
   check_launch("gemm");
 }
 
+// ---- packed BF16 path ---------------------------------------------------------------
+// BF16 / BF16X3 GEMMs first pack op(A) and op(B)^T into zero-padded, K-major
+// BF16 buffers ([Mp][Kp], [Np][Kp]; for BF16X3 also the low parts), so the
+// main kernel's inner loop is 16-byte loads, 16-byte LDS stores and MFMAs with
+// no conversion and no bounds checks.
+constexpr int PK_BK = 32;           // k per LDS stage
+constexpr int PK_LDK = PK_BK + 8;   // LDS row of BF16 (80 B: rows stay 16-byte aligned)
+constexpr int PK_TR = 64;           // transpose tile rows (r) in the pack kernel
+
+// dst[r][k] = bf16(op(src)[r][k]) for r < Rp, k < Kp, zero outside [R, K);
+// op(src)[r][k] = src[r*ld + k] (TRANS = false) or src[k*ld + r] (TRANS = true).
+template <bool SPLIT>
+__device__ __forceinline__ void pack8(const float v[8], uint16_t* hi, uint16_t* lo) {
+  uint16_t h[8], l[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    h[q] = f2bf(v[q]);
+    if (SPLIT) l[q] = f2bf(v[q] - bf2f(h[q]));
+  }
+  uint4 hv, lv;
+  hv.x = h[0] | ((uint32_t)h[1] << 16); hv.y = h[2] | ((uint32_t)h[3] << 16);
+  hv.z = h[4] | ((uint32_t)h[5] << 16); hv.w = h[6] | ((uint32_t)h[7] << 16);
+  *reinterpret_cast<uint4*>(hi) = hv;
+  if (SPLIT) {
+    lv.x = l[0] | ((uint32_t)l[1] << 16); lv.y = l[2] | ((uint32_t)l[3] << 16);
+    lv.z = l[4] | ((uint32_t)l[5] << 16); lv.w = l[6] | ((uint32_t)l[7] << 16);
+    *reinterpret_cast<uint4*>(lo) = lv;
+  }
+}
+
+template <bool SPLIT>
+__global__ void __launch_bounds__(256)
+k_pack_rows(const float* src, uint64_t ld, uint64_t R, uint64_t K, uint16_t* hi, uint16_t* lo,
+            uint64_t Rp, uint64_t Kp, int vec) {
+  const uint64_t kc = Kp / 8, total = Rp * kc;
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += stride) {
+    const uint64_t r = t / kc, k = (t - r * kc) * 8;
+    float v[8];
+    if (r < R && vec && k + 7 < K) {
+      const float4 a = *reinterpret_cast<const float4*>(src + r * ld + k);
+      const float4 b = *reinterpret_cast<const float4*>(src + r * ld + k + 4);
+      v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = (r < R && k + q < K) ? src[r * ld + k + q] : 0.f;
+    }
+    pack8<SPLIT>(v, hi + r * Kp + k, SPLIT ? lo + r * Kp + k : nullptr);
+  }
+}
+
+// transposed source: a PK_TR x PK_BK tile goes through LDS so that both the
+// reads (along r) and the writes (along k) are contiguous
+template <bool SPLIT>
+__global__ void __launch_bounds__(256)
+k_pack_cols(const float* src, uint64_t ld, uint64_t R, uint64_t K, uint16_t* hi, uint16_t* lo,
+            uint64_t Rp, uint64_t Kp) {
+  __shared__ float t[PK_BK][PK_TR + 1];
+  const uint64_t r0 = (uint64_t)blockIdx.x * PK_TR, k0 = (uint64_t)blockIdx.y * PK_BK;
+  for (int idx = threadIdx.x; idx < PK_BK * PK_TR; idx += 256) {
+    const int kk = idx / PK_TR, rr = idx % PK_TR;
+    const uint64_t r = r0 + rr, k = k0 + kk;
+    t[kk][rr] = (r < R && k < K) ? src[k * ld + r] : 0.f;
+  }
+  __syncthreads();
+  // PK_TR rows x PK_BK/8 chunks = 256 chunks: one per thread
+  const int rr = threadIdx.x / (PK_BK / 8), kc = threadIdx.x % (PK_BK / 8);
+  const uint64_t r = r0 + rr;
+  if (r < Rp) {
+    float v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = t[kc * 8 + q][rr];
+    const uint64_t off = r * Kp + k0 + kc * 8;
+    pack8<SPLIT>(v, hi + off, SPLIT ? lo + off : nullptr);
+  }
+}
+
+struct PackedArgs {
+  const uint16_t* Ah;
+  const uint16_t* Al;
+  const uint16_t* Bh;
+  const uint16_t* Bl;
+  uint64_t Kp;
+  GemmArgs e;
+};
+
+// Register stage of one packed operand's ROWS x PK_BK slice: 16-byte loads of
+// K-contiguous BF16 rows (both parts in split mode), stored to LDS rows of
+// PK_LDK elements.
+template <int ROWS, bool SPLIT>
+struct PackedStager {
+  static constexpr int NV = ROWS * PK_BK / 8 / GEMM_THREADS;  // 16-byte vectors per thread
+  static constexpr int PARTS = SPLIT ? 2 : 1;
+  u32x4_t r[PARTS][NV];
+
+  __device__ __forceinline__ void load(const uint16_t* hi, const uint16_t* lo, uint64_t row0,
+                                       uint64_t Kp, uint64_t k0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int idx = threadIdx.x + i * GEMM_THREADS;
+      const int row = idx / (PK_BK / 8), c = idx % (PK_BK / 8);
+      const uint64_t off = (row0 + row) * Kp + k0 + c * 8;
+      r[0][i] = *reinterpret_cast<const u32x4_t*>(hi + off);
+      if constexpr (SPLIT) r[1][i] = *reinterpret_cast<const u32x4_t*>(lo + off);
+    }
+  }
+  __device__ __forceinline__ void store(uint16_t* hi, uint16_t* lo) const {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int idx = threadIdx.x + i * GEMM_THREADS;
+      const int row = idx / (PK_BK / 8), c = idx % (PK_BK / 8);
+      *reinterpret_cast<u32x4_t*>(hi + row * PK_LDK + c * 8) = r[0][i];
+      if constexpr (SPLIT) *reinterpret_cast<u32x4_t*>(lo + row * PK_LDK + c * 8) = r[1][i];
+    }
+  }
+};
+
+// Epilogue of 4 consecutive outputs C[m, n..n+3] from their accumulators, in
+// the order alpha*acc (+ beta*C) (+ bias[n]), activation, times the aux
+// activation's derivative; 16-byte accesses when the whole group is in range.
+__device__ __forceinline__ void epilogue4(const GemmArgs& e, uint64_t m, uint64_t n, float4 acc) {
+  const bool full = e.vec_c && n + 3 < e.N;
+  const int cnt = full ? 4 : (int)min<uint64_t>(4, e.N - n);
+  float* cp = e.C + m * e.ldc + n;
+  float v[4] = {acc.x, acc.y, acc.z, acc.w};
+  float c[4] = {0.f, 0.f, 0.f, 0.f}, b[4] = {0.f, 0.f, 0.f, 0.f}, x[4] = {0.f, 0.f, 0.f, 0.f};
+  if (e.beta != 0.f) {
+    if (full) {
+      const float4 t = *reinterpret_cast<const float4*>(cp);
+      c[0] = t.x; c[1] = t.y; c[2] = t.z; c[3] = t.w;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) c[q] = q < cnt ? cp[q] : 0.f;
+    }
+  }
+  if (e.bias) {
+    if (full) {
+      const float4 t = *reinterpret_cast<const float4*>(e.bias + n);
+      b[0] = t.x; b[1] = t.y; b[2] = t.z; b[3] = t.w;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) b[q] = q < cnt ? e.bias[n + q] : 0.f;
+    }
+  }
+  if (e.aux) {
+    const float* ap = e.aux + m * e.ldaux + n;
+    if (full) {
+      const float4 t = *reinterpret_cast<const float4*>(ap);
+      x[0] = t.x; x[1] = t.y; x[2] = t.z; x[3] = t.w;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) x[q] = q < cnt ? ap[q] : 0.f;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    float t = e.alpha * v[q];
+    if (e.beta != 0.f) t += e.beta * c[q];
+    if (e.bias) t += b[q];
+    t = act_apply(e.act, t);
+    if (e.aux) t = act_deriv_mul(e.aux_act, x[q], t);
+    v[q] = t;
+  }
+  if (full) {
+    *reinterpret_cast<float4*>(cp) = make_float4(v[0], v[1], v[2], v[3]);
+    if (e.Cbf) {
+      uint2 h;
+      h.x = f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+      h.y = f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+      *reinterpret_cast<uint2*>(e.Cbf + m * e.ldc + n) = h;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (q < cnt) {
+        cp[q] = v[q];
+        if (e.Cbf) e.Cbf[m * e.ldc + n + q] = f2bf(v[q]);
+      }
+    }
+  }
+}
+
+// BM x BN tile, 4 waves (2 x 2) of (BM/2) x (BN/2), double-buffered LDS and
+// register prefetch of the next K slice; one barrier per slice.  After the
+// K loop the LDS holds each wave's accumulators, 32 rows at a time, so the
+// epilogue reads and writes whole row segments of C.
+template <int BM, int BN, bool SPLIT>
+__global__ void __launch_bounds__(GEMM_THREADS)
+k_gemm_packed(PackedArgs p) {
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int TM = WM / 32, TN = WN / 32;
+  constexpr int PARTS = SPLIT ? 2 : 1;
+  constexpr int A_TILE = BM * PK_LDK, B_TILE = BN * PK_LDK;        // BF16 elements
+  constexpr int MAIN_BYTES = 2 * PARTS * (A_TILE + B_TILE) * 2;
+  constexpr int SROW = WN + 4;                                      // staging row (floats)
+  constexpr int STAGE_BYTES = 4 * 32 * SROW * 4;
+  constexpr int SMEM_BYTES = MAIN_BYTES > STAGE_BYTES ? MAIN_BYTES : STAGE_BYTES;
+  __shared__ __attribute__((aligned(16))) uint16_t smem[SMEM_BYTES / 2];
+  // operand tile (buffer, part): As(buf, q) / Bs(buf, q)
+  auto As = [&](int buf, int q) { return smem + (buf * PARTS + q) * A_TILE; };
+  auto Bs = [&](int buf, int q) { return smem + 2 * PARTS * A_TILE + (buf * PARTS + q) * B_TILE; };
+
+  const GemmArgs& e = p.e;
+  uint32_t tm, tn;
+  tile_coords(e, tm, tn);
+  const uint64_t m0 = (uint64_t)tm * BM, n0 = (uint64_t)tn * BN;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int l32 = lane & 31, lhi = lane >> 5;
+  const uint64_t Kp = p.Kp;
+
+  PackedStager<BM, SPLIT> sa;
+  PackedStager<BN, SPLIT> sb;
+  auto gload = [&](uint64_t k0) {
+    sa.load(p.Ah, p.Al, m0, Kp, k0);
+    sb.load(p.Bh, p.Bl, n0, Kp, k0);
+  };
+  auto lstore = [&](int buf) {
+    sa.store(As(buf, 0), As(buf, PARTS - 1));
+    sb.store(Bs(buf, 0), Bs(buf, PARTS - 1));
+  };
+
+  f32x16_t acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+  const uint64_t nk = Kp / PK_BK;
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  int cur = 0;
+  for (uint64_t kt = 0; kt < nk; ++kt) {
+    const bool more = kt + 1 < nk;
+    if (more) gload((kt + 1) * PK_BK);
+#pragma unroll
+    for (int ks = 0; ks < PK_BK; ks += 16) {
+      bf16x8_t a[TM], b[TN], al[TM], bl[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        const int off = (wm * WM + i * 32 + l32) * PK_LDK + ks + lhi * 8;
+        a[i] = *reinterpret_cast<const bf16x8_t*>(As(cur, 0) + off);
+        if (SPLIT) al[i] = *reinterpret_cast<const bf16x8_t*>(As(cur, PARTS - 1) + off);
+      }
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int off = (wn * WN + j * 32 + l32) * PK_LDK + ks + lhi * 8;
+        b[j] = *reinterpret_cast<const bf16x8_t*>(Bs(cur, 0) + off);
+        if (SPLIT) bl[j] = *reinterpret_cast<const bf16x8_t*>(Bs(cur, PARTS - 1) + off);
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          if (SPLIT) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], b[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], bl[j], acc[i][j], 0, 0, 0);
+          }
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    if (more) lstore(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
+  }
+
+  // Epilogue.  The loop above ended with a barrier, so the operand tiles are
+  // dead.  acc[i][j][r] is C[m, n] with m = wave row + i*32 + (r/4)*8 +
+  // lhi*4 + r%4 and n = wave col + j*32 + l32; each wave writes rows
+  // i*32 .. i*32+31 of its sub-tile to its own 32 x SROW staging slice, then
+  // reads them back as 4-float row segments (WN/4 lanes per row).
+  float* st = reinterpret_cast<float*>(smem) + wave * 32 * SROW;
+  constexpr int F4R = WN / 4, RPI = 64 / F4R;
+  const uint64_t nb = n0 + wn * WN;
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r)
+        st[((r / 4) * 8 + lhi * 4 + (r % 4)) * SROW + j * 32 + l32] = acc[i][j][r];
+    __syncthreads();
+    const uint64_t mb = m0 + wm * WM + i * 32;
+    const int cc = (lane % F4R) * 4;
+    for (int rr = lane / F4R; rr < 32; rr += RPI) {
+      const uint64_t m = mb + rr, n = nb + cc;
+      if (m < e.M && n < e.N)
+        epilogue4(e, m, n, *reinterpret_cast<const float4*>(st + rr * SROW + cc));
+    }
+    __syncthreads();
+  }
+}
+
+// Tile choice of the packed kernel: 256 x 256 when both sides fill at least
+// one such tile (4 waves of 128 x 128), 128 x 128 otherwise and in split mode
+// (whose two BF16 parts double the LDS footprint).
+struct PackedShape {
+  int bm, bn;
+};
+static PackedShape packed_shape(int mode, uint64_t M, uint64_t N) {
+  if (mode == VKH_GEMM_BF16 && M >= 256 && N >= 256) return {256, 256};
+  return {128, 128};
+}
+
+static size_t packed_bytes(int mode, uint64_t M, uint64_t N, uint64_t K) {
+  const PackedShape s = packed_shape(mode, M, N);
+  const uint64_t Mp = ceil_div(M, s.bm) * s.bm, Np = ceil_div(N, s.bn) * s.bn;
+  const uint64_t Kp = ceil_div(std::max<uint64_t>(K, 1), PK_BK) * PK_BK;
+  const uint64_t parts = mode == VKH_GEMM_BF16X3 ? 2 : 1;
+  return (size_t)((Mp + Np) * Kp * 2 * parts);
+}
+
+// Packing area.  Eager calls share a per-context grow-only area (stream order
+// keeps uses apart).  A captured GEMM uses its graph's packing region, which
+// the graph owns for as long as it can be replayed; it is sized when the
+// capture begins from what vkh_gemm_reserve recorded, and a capture that
+// needs more fails and records the size for the next capture.
+static uint16_t* pack_area(vkh_gpu* g, size_t bytes) {
+  if (g->capturing) {
+    if (bytes <= g->graph_pack_bytes) return (uint16_t*)g->graph_pack;
+    g->graph_pack_hint = std::max(g->graph_pack_hint, bytes);
+    fail(VKH_OUT_OF_MEMORY, "graph GEMM packing region too small (" + std::to_string(g->graph_pack_bytes) +
+                                " < " + std::to_string(bytes) +
+                                " bytes); call vkh_gemm_reserve before the capture, or capture again");
+  }
+  if (bytes <= g->pack_bytes) return (uint16_t*)g->pack;
+  VKH_REQUIRE(g->alloc, "GEMM packing needs a context with its own allocator");
+  if (g->pack) g->alloc->release(g->pack);
+  g->pack = nullptr;
+  g->pack_bytes = 0;
+  const size_t want = bytes + bytes / 4;
+  g->pack = g->alloc->allocate(want);
+  g->pack_bytes = want;
+  return (uint16_t*)g->pack;
+}
+
+template <bool SPLIT>
+static void pack_operand(vkh_gpu* g, const float* src, uint64_t ld, bool trans, uint64_t R,
+                         uint64_t K, uint16_t* hi, uint16_t* lo, uint64_t Rp, uint64_t Kp) {
+  if (!trans) {
+    const int vec = ((uintptr_t)src & 15u) == 0 && ld % 4 == 0;
+    unsigned grid = full_grid(Rp * (Kp / 8), 256);
+    hipLaunchKernelGGL((k_pack_rows<SPLIT>), dim3(grid), dim3(256), 0, g->stream, src, ld, R, K, hi,
+                       lo, Rp, Kp, vec);
+  } else {
+    dim3 grid((unsigned)ceil_div(Rp, PK_TR), (unsigned)(Kp / PK_BK));
+    hipLaunchKernelGGL((k_pack_cols<SPLIT>), grid, dim3(256), 0, g->stream, src, ld, R, K, hi, lo,
+                       Rp, Kp);
+  }
+  check_launch("gemm pack");
+}
+
+static void gemm_packed(vkh_gpu* g, int mode, bool ta, bool tb, GemmArgs& a) {
+  const bool split = mode == VKH_GEMM_BF16X3;
+  const PackedShape s = packed_shape(mode, a.M, a.N);
+  const uint64_t Mp = ceil_div(a.M, s.bm) * s.bm, Np = ceil_div(a.N, s.bn) * s.bn;
+  const uint64_t Kp = ceil_div(std::max<uint64_t>(a.K, 1), PK_BK) * PK_BK;
+  uint16_t* base = pack_area(g, packed_bytes(mode, a.M, a.N, a.K));
+  uint16_t* Ah = base;
+  uint16_t* Al = split ? Ah + Mp * Kp : nullptr;
+  uint16_t* Bh = split ? Al + Mp * Kp : Ah + Mp * Kp;
+  uint16_t* Bl = split ? Bh + Np * Kp : nullptr;
+  // op(A)[m][k] and op(B)^T[n][k] = op(B)[k][n]
+  if (split) {
+    pack_operand<true>(g, a.A, a.lda, ta, a.M, a.K, Ah, Al, Mp, Kp);
+    pack_operand<true>(g, a.B, a.ldb, !tb, a.N, a.K, Bh, Bl, Np, Kp);
+  } else {
+    pack_operand<false>(g, a.A, a.lda, ta, a.M, a.K, Ah, nullptr, Mp, Kp);
+    pack_operand<false>(g, a.B, a.ldb, !tb, a.N, a.K, Bh, nullptr, Np, Kp);
+  }
+  a.tiles_m = (uint32_t)(Mp / s.bm);
+  a.tiles_n = (uint32_t)(Np / s.bn);
+  const uint64_t nb = (uint64_t)a.tiles_m * a.tiles_n;
+  VKH_REQUIRE(nb < (uint64_t(1) << 31), "GEMM too large");
+  PackedArgs pa{Ah, Al, Bh, Bl, Kp, a};
+  dim3 grid((unsigned)nb), block(GEMM_THREADS);
+  if (s.bm == 256) hipLaunchKernelGGL((k_gemm_packed<256, 256, false>), grid, block, 0, g->stream, pa);
+  else if (split) hipLaunchKernelGGL((k_gemm_packed<128, 128, true>), grid, block, 0, g->stream, pa);
+  else hipLaunchKernelGGL((k_gemm_packed<128, 128, false>), grid, block, 0, g->stream, pa);
+  check_launch("gemm (packed)");
+}
+
 // ---- helpers for the nn path -------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_to_bf16(const float* src, uint16_t* dst, uint64_t n) {
   const uint64_t stride = (uint64_t)gridDim.x * 256;
@@ -350,14 +739,29 @@ static void gemm_impl(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M
   a.M = M; a.N = N; a.K = K; a.lda = lda; a.ldb = ldb; a.ldc = ldc;
   a.alpha = alpha; a.beta = beta; a.act = act;
   a.aux = aux; a.ldaux = ldaux; a.aux_act = aux_act;
+  a.vec_c = ((uintptr_t)Cp & 15u) == 0 && ldc % 4 == 0 && (!bias || ((uintptr_t)bias & 15u) == 0) &&
+            (!aux || (((uintptr_t)aux & 15u) == 0 && ldaux % 4 == 0)) &&
+            (!c_bf16 || ((uintptr_t)c_bf16 & 7u) == 0);
   const int va = ((uintptr_t)A & 15u) == 0 && lda % 4 == 0;
   const int vb = ((uintptr_t)B & 15u) == 0 && ldb % 4 == 0;
   switch (mode) {
     case VKH_GEMM_FP32: launch_mode<VKH_GEMM_FP32>(g, trans_a, trans_b, a, va, vb); break;
-    case VKH_GEMM_BF16: launch_mode<VKH_GEMM_BF16>(g, trans_a, trans_b, a, va, vb); break;
-    case VKH_GEMM_BF16X3: launch_mode<VKH_GEMM_BF16X3>(g, trans_a, trans_b, a, va, vb); break;
+    case VKH_GEMM_BF16:
+    case VKH_GEMM_BF16X3: gemm_packed(g, mode, trans_a, trans_b, a); break;
     default: fail(VKH_INVALID, "gemm: unknown mode");
   }
+}
+
+int vkh_gemm_reserve(vkh_gpu* g, int mode, uint64_t M, uint64_t N, uint64_t K) {
+  return guarded([&] {
+    use(g);
+    VKH_REQUIRE(mode >= VKH_GEMM_FP32 && mode <= VKH_GEMM_BF16X3, "gemm_reserve: unknown mode");
+    VKH_REQUIRE(!g->capturing, "gemm_reserve: not allowed during graph capture");
+    if (mode == VKH_GEMM_FP32 || M == 0 || N == 0) return;
+    const size_t bytes = packed_bytes(mode, M, N, K);
+    g->graph_pack_hint = std::max(g->graph_pack_hint, bytes);
+    pack_area(g, bytes);
+  });
 }
 
 int vkh_gemm(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M, uint64_t N, uint64_t K,

@@ -421,12 +421,14 @@ int vkh_loss_grad(vkh_gpu* g, int kind, const float* x, const float* y, float* d
 }
 
 int vkh_adam(vkh_gpu* g, float* w, const float* gr, float* m, float* v, uint64_t n, float lr,
-             float beta1, float beta2, float eps, double b1t, double b2t) {
+             double beta1, double beta2, float eps, double b1t, double b2t) {
   return guarded([&] {
     use(g);
     if (n == 0) return;
     VKH_REQUIRE(w && gr && m && v, "adam: null pointer");
-    AdamScalars s{beta1, (float)(1.0 - (double)beta1), beta2, (float)(1.0 - (double)beta2),
+    // (1 - beta) in FP64, then rounded: the Python float expression that the
+    // reference passes to the FP32 scalar kernels
+    AdamScalars s{(float)beta1, (float)(1.0 - beta1), (float)beta2, (float)(1.0 - beta2),
                   eps, -lr, (float)(1.0 - b1t), (float)(1.0 - b2t)};
     hipLaunchKernelGGL(k_adam, dim3(grid_for(g, n)), dim3(256), 0, g->stream, w, gr, m, v, n, s);
     check_launch("adam");
@@ -443,12 +445,12 @@ int vkh_adam_pow_advance(vkh_gpu* g, double* bpow, double beta1, double beta2) {
 }
 
 int vkh_adam_device(vkh_gpu* g, float* w, const float* gr, float* m, float* v, uint64_t n, float lr,
-                    float beta1, float beta2, float eps, const double* bpow, uint16_t* w_bf16) {
+                    double beta1, double beta2, float eps, const double* bpow, uint16_t* w_bf16) {
   return guarded([&] {
     use(g);
     VKH_REQUIRE(w && gr && m && v && bpow, "adam_device: null pointer");
     if (n == 0) return;
-    AdamScalars s{beta1, (float)(1.0 - (double)beta1), beta2, (float)(1.0 - (double)beta2),
+    AdamScalars s{(float)beta1, (float)(1.0 - beta1), (float)beta2, (float)(1.0 - beta2),
                   eps, -lr, 1.0f, 1.0f};
     hipLaunchKernelGGL(k_adam_dev, dim3(grid_for(g, n)), dim3(256), 0, g->stream, w, gr, m, v, n, s,
                        bpow, w_bf16);

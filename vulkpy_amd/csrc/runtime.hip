@@ -394,15 +394,21 @@ int vkh_graph_begin(vkh_gpu* g) {
     use(g);
     VKH_REQUIRE(!g->capturing, "graph capture already active on this context");
     VKH_REQUIRE(g->alloc, "graph capture needs a context with its own allocator");
-    // reserve the workspace arena before the stream starts capturing
-    const size_t want = std::max<size_t>(size_t(64) << 20, g->arena_hint);
-    g->arena = (char*)g->alloc->allocate(want);
-    g->arena_size = Allocator::round_size(want);
+    // reserve the arena (workspaces, then the GEMM packing region) before the
+    // stream starts capturing
+    const size_t ws = ceil_div(std::max<size_t>(size_t(64) << 20, g->arena_hint), 256) * 256;
+    const size_t pk = ceil_div(g->graph_pack_hint, 256) * 256;
+    g->arena = (char*)g->alloc->allocate(ws + pk);
+    g->arena_size = ws;
     g->arena_used = 0;
+    g->graph_pack = pk ? g->arena + ws : nullptr;
+    g->graph_pack_bytes = pk;
     hipError_t r = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal);
     if (r != hipSuccess) {
       g->alloc->release(g->arena);
       g->arena = nullptr;
+      g->graph_pack = nullptr;
+      g->graph_pack_bytes = 0;
       check_hip(r, "hipStreamBeginCapture");
     }
     g->capturing = true;
@@ -423,6 +429,8 @@ int vkh_graph_end(vkh_gpu* g, void** exec) {
     gr->arena = g->arena;
     g->arena = nullptr;
     g->arena_size = g->arena_used = 0;
+    g->graph_pack = nullptr;
+    g->graph_pack_bytes = 0;
     if (r == hipSuccess) {
       r = hipGraphInstantiate(&gr->exec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);

@@ -236,6 +236,14 @@ class _Plan:
             for p in (u.dense.w, u.dense.b):
                 if p.grad is not None and self.seq.regularize and isinstance(p.R, Elastic):
                     self._scratch(p)
+        # size the graph's GEMM packing region for every product of the step
+        B = self.B
+        for i, u in enumerate(self.units):
+            d = u.dense
+            call("vkh_gemm_reserve", self.h, self.mode, B, d.output_dim, d.input_dim)
+            call("vkh_gemm_reserve", self.h, self.mode, d.output_dim, d.input_dim, B)
+            if i > 0:
+                call("vkh_gemm_reserve", self.h, self.mode, B, d.input_dim, d.output_dim)
         self.gpu.wait()
         call("vkh_graph_begin", self.h)
         try:
