@@ -20,7 +20,16 @@ timed per launch with HIP events on the library's stream for the roofline.
 The CPU baseline times the oracle (C + OpenMP, the reference's algorithms) on
 the first 512 rows of the same operands.
 
+Two more sections of the same JSON line cover the rest of the metric:
+  * "mlp": the 256-4096-4096-1 regression MLP (batch 16384, Huber, Ridge,
+    Adam), plain FP32 step against the BF16 step captured as one hipGraph:
+    steps/s of each, the speed-up and the final losses after the same number
+    of steps;
+  * "gemm": 8192^3 and 65536 x 1024 x 1024 products in FP32, BF16 and
+    split-BF16, TFLOP/s and error against an FP64 product.
+
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                [--mlp-steps S] [--no-gemm]
 """
 from __future__ import annotations
 
@@ -126,6 +135,129 @@ def build_suite(vk, gpu, A, B, col, row, P, rng_gen, ubuf, fbuf, timer):
     return ops
 
 
+def mlp_compare(vk, gpu, steps: int, warmup: int = 2):
+    """
+    Regression MLP Dense(256->4096)-ReLU-Dense(4096->4096)-ReLU-Dense(4096->1),
+    HuberLoss(delta=1), Ridge(1e-4) on every weight (applied, see
+    ``regularize``), Adam lr=1e-3, batch 16384, x ~ N(0, 1),
+    y = sin(x @ P) + 0.1 N(0, 1) with a fixed projection P (256 x 1).
+    Both arms start from the same HeNormal weights and train `steps` steps on
+    the same batch: the plain FP32 step (nn.Sequence, one launch per op, FP32
+    matrix cores) and the BF16 step captured as one hipGraph
+    (nn.GraphSequence, BF16 matrix cores, FP32 master weights and Adam).
+    steps/s is timed over the steps after the first `warmup` (the graph arm
+    captures on its first step).
+    """
+    from vulkpy_amd import nn
+    B, D, H = 16384, 256, 4096
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((B, D), dtype=np.float32)
+    proj = (rng.standard_normal((D, 1)) / np.sqrt(D)).astype(np.float32)
+    y = (np.sin(x @ proj) + 0.1 * rng.standard_normal((B, 1))).astype(np.float32)
+    X, Y = vk.Array(gpu, data=x), vk.Array(gpu, data=y)
+    dims = [(D, H), (H, H), (H, 1)]
+
+    def layers(precision):
+        out = []
+        for k, (i, o) in enumerate(dims):
+            out.append(nn.Dense(gpu, i, o, w_init=nn.HeNormal(gpu, i, seed=100 + k),
+                                w_opt=nn.Adam(gpu, lr=1e-3), b_opt=nn.Adam(gpu, lr=1e-3),
+                                w_reg=nn.Ridge(1e-4), precision=precision))
+            if k + 1 < len(dims):
+                out.append(nn.ReLU())
+        return out
+
+    def run(model):
+        losses = []
+        t0 = None
+        for s in range(steps):
+            if s == warmup:
+                gpu.wait()
+                t0 = time.perf_counter()
+            _, L = model.train(X, Y)
+            if s == 0 or s + 1 == steps:
+                losses.append(float(np.asarray(L)))
+        gpu.wait()
+        dt = time.perf_counter() - t0
+        return (steps - warmup) / dt, losses
+
+    res = {}
+    for name, model in (("fp32", lambda: nn.Sequence(layers("fp32"), nn.HuberLoss(), regularize=True)),
+                        ("bf16_graph", lambda: nn.GraphSequence(layers("bf16"), nn.HuberLoss(),
+                                                                precision="bf16", regularize=True))):
+        log(f"bench: MLP {name} {steps} steps")
+        m = model()
+        sps, losses = run(m)
+        res[name] = {"steps_per_s": round(sps, 2), "loss_first": losses[0], "loss_final": losses[-1]}
+        log(f"bench: MLP {name} {res[name]}")
+        del m
+        gpu.wait()
+    f, b = res["fp32"]["loss_final"], res["bf16_graph"]["loss_final"]
+    return {
+        "workload": ("C5 regression MLP 256-4096-4096-1 ReLU, HuberLoss(delta=1), Ridge(1e-4) on "
+                     "weights, Adam lr=1e-3, batch 16384, x~N(0,1), y=sin(x@P)+0.1N(0,1)"),
+        "steps": steps,
+        "timed_steps": steps - warmup,
+        "fp32": res["fp32"],
+        "bf16_graph": res["bf16_graph"],
+        "speedup": round(res["bf16_graph"]["steps_per_s"] / res["fp32"]["steps_per_s"], 3),
+        "final_loss_rel_diff": round(abs(b - f) / abs(f), 5),
+        "gemm_flop_per_step": 2 * (B * H * D + B * H * H + B * H) * 2 + 2 * (B * H * H + B * H),
+    }
+
+
+def gemm_report(vk, gpu, timer, reps: int = 3, sample_every: int = 128):
+    """
+    C3 GEMMs: C = A @ B for 8192^3 and 65536 x 1024 x 1024 in each mode, with
+    N(0, 1) inputs and with U[-1, 1] inputs whose every 64th row of A is
+    scaled by 1e3.  TFLOP/s = 2MNK / mean launch time (HIP events; operand
+    packing included).  Errors against an FP64 product on every
+    `sample_every`-th row: max and mean of |C - C64| / (|A| |B|) elementwise,
+    and ||C - C64||_F / ||C64||_F.
+    """
+    out = []
+    rng = np.random.default_rng(3)
+    for (M, Nn, K) in ((8192, 8192, 8192), (65536, 1024, 1024)):
+        for dist in ("normal", "range"):
+            if dist == "normal":
+                a = rng.standard_normal((M, K), dtype=np.float32)
+                b = rng.standard_normal((K, Nn), dtype=np.float32)
+            else:
+                a = rng.uniform(-1, 1, (M, K)).astype(np.float32)
+                a[::64] *= 1e3
+                b = rng.uniform(-1, 1, (K, Nn)).astype(np.float32)
+            A, Bm = vk.Array(gpu, data=a), vk.Array(gpu, data=b)
+            rows = np.arange(0, M, sample_every)
+            a64 = a[rows].astype(np.float64)
+            b64 = b.astype(np.float64)
+            c64 = a64 @ b64
+            den = np.abs(a64) @ np.abs(b64)
+            for mode in ("fp32", "bf16", "bf16x3"):
+                C = A.matmul(Bm, precision=mode)
+                gpu.wait()
+                for _ in range(reps):
+                    t = timer.start()
+                    C = A.matmul(Bm, precision=mode)
+                    timer.stop(t)
+                ms = timer.mean_ms()
+                c = np.asarray(C)[rows].astype(np.float64)
+                err = np.abs(c - c64)
+                rel = err / np.maximum(den, np.finfo(np.float64).tiny)
+                out.append({
+                    "M": M, "N": Nn, "K": K, "inputs": dist, "mode": mode,
+                    "ms": round(ms, 4),
+                    "tflops": round(2.0 * M * Nn * K / (ms * 1e-3) / 1e12, 1),
+                    "rel_err_max": float(f"{rel.max():.3e}"),
+                    "rel_err_mean": float(f"{rel.mean():.3e}"),
+                    "normwise_err": float(f"{np.linalg.norm(c - c64) / np.linalg.norm(c64):.3e}"),
+                })
+                log(f"bench: GEMM {out[-1]}")
+                del C
+            del A, Bm
+            gpu.wait()
+    return out
+
+
 def cpu_baseline(a, col, row, p, seconds: float = 10.0):
     """Time the oracle's restatement of the reference kernels on a bounded sample."""
     from oracle import oracle as O
@@ -173,6 +305,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--mlp-steps", type=int, default=500,
+                    help="training steps per arm of the MLP comparison (0 skips it)")
+    ap.add_argument("--no-gemm", action="store_true", help="skip the GEMM precision report")
+    ap.add_argument("--no-c2", action="store_true",
+                    help="skip the bandwidth suite (and the CPU baseline that samples it)")
+    ap.add_argument("--pmc-kernels", action="store_true",
+                    help="only launch the copy and dominant kernels (for counter runs) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -183,7 +322,88 @@ def main() -> int:
 
     import vulkpy_amd as vk
     gpu = vk.GPU(0)
+    timer = Timer(gpu)
 
+    c2 = None
+    if not args.no_c2 or args.pmc_kernels:
+        c2 = c2_suite(vk, gpu, args, timer)
+        if c2 is None:  # --pmc-kernels
+            return 0
+    mlp = None
+    if args.mlp_steps > 0:
+        mlp = mlp_compare(vk, gpu, args.mlp_steps)
+    gemm = None
+    if not args.no_gemm:
+        log("bench: GEMM report")
+        gemm = gemm_report(vk, gpu, timer)
+
+    cpu = None
+    if c2 is not None and not args.no_cpu:
+        log("bench: CPU baseline (oracle)")
+        rows = 512
+        h = c2["host"]
+        cpu = cpu_baseline(np.ascontiguousarray(h["a"][:rows]), h["col"][:rows], h["row"],
+                           np.ascontiguousarray(h["p"][:rows]), args.cpu_seconds)
+
+    traffic = pmc_traffic()
+    out = {
+        "metric": METRIC,
+        "value": round(c2["value"], 2) if c2 else None,
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(c2["ms_per_step"], 3) if c2 else None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": ("C2 bandwidth suite: float32 A (8192,16384) N(0,1) seed 0, broadcast "
+                         "operands (8192,1) and (1,16384), same-shape/broadcast/in-place/scalar "
+                         "binary ops, sum/prod/max/min/mean over axis 0, 1, None, Xoshiro128++ "
+                         "2^20 states -> 2^28 uint32/float/normal"),
+            "elements": N,
+            "ops_per_step": c2["n_ops"] if c2 else None,
+            "bytes_per_step": c2["step_bytes"] if c2 else None,
+        },
+        "roofline": {
+            "kernel": "broadcast binary A (8192,16384) + col (8192,1)",
+            "bound": "hbm",
+            "achieved": round(c2["achieved"], 1) if c2 else None,
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": round(c2["achieved"] / PEAK_HBM_GBS, 4) if c2 else None,
+            "traffic": traffic,
+            "algorithmic_bytes": 8 * N + 4 * M_ROWS,
+            "avg_launch_ms": round(c2["dom_ms"], 5) if c2 else None,
+        },
+        "cpu_baseline": cpu,
+        "per_op_GBps": c2["per_op"] if c2 else None,
+        "mlp": mlp,
+        "gemm": gemm,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def pmc_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    summary (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over
+    ``bench.py --pmc-kernels``, each scaled by the copy kernel's reading), or
+    None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "add_col_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["traffic_bytes"]
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def c2_suite(vk, gpu, args, timer):
+    """The C2 bandwidth suite; returns its measurements (None after the
+    --pmc-kernels launches)."""
     log("bench: generating inputs")
     rng = np.random.default_rng(0)
     a_host = rng.standard_normal((M_ROWS, N_COLS), dtype=np.float32)
@@ -198,13 +418,29 @@ def main() -> int:
     row = vk.Array(gpu, data=row_host)
     P = vk.Array(gpu, data=p_host)
     del b_host
+    if args.pmc_kernels:
+        # For rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs: a plain copy of
+        # known traffic (512 MiB read + 512 MiB written per launch) as the
+        # counter scale, then the dominant kernel, 5 launches each.
+        from vulkpy_amd import _lib
+        dst = vk.Array(gpu, shape=A.shape)
+        sh = _lib.shape_array(A.shape)
+        gpu.wait()
+        for _ in range(5):
+            _lib.call("vkh_broadcast_to", gpu._handle, A._ptr, dst._ptr, 2, sh, sh)
+        gpu.wait()
+        for _ in range(5):
+            A + col
+        gpu.wait()
+        print(json.dumps({"pmc_kernels": ["copy (8192,16384) f32 x5", "add A + col x5"],
+                          "copy_bytes": [4 * N, 4 * N], "add_col_bytes": [4 * N + 4 * M_ROWS, 4 * N]}))
+        return None
     log("bench: seeding Xoshiro128++ (2^20 states)")
     gen = vk.random.Xoshiro128pp(gpu, size=PRNG_STATES, seed=1)
     ubuf = vk.U32Array(gpu, shape=(PRNG_N,))
     fbuf = vk.Array(gpu, shape=(PRNG_N,))
     gpu.wait()
 
-    timer = Timer(gpu)
     suite = build_suite(vk, gpu, A, B, col, row, P, gen, ubuf, fbuf, timer)
     step_bytes = sum(nb for _, nb, _ in suite)
 
@@ -218,7 +454,8 @@ def main() -> int:
     gpu.wait()
     timer.mean_ms()  # drop warmup timings
 
-    log(f"bench: timing {args.steps} steps ({len(suite)} ops, {step_bytes / 1e9:.2f} GB per step)")
+    n_ops = len(suite)
+    log(f"bench: timing {args.steps} steps ({n_ops} ops, {step_bytes / 1e9:.2f} GB per step)")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
@@ -239,50 +476,13 @@ def main() -> int:
         gpu.wait()
         per_op[name] = round(nb / (time.perf_counter() - t) / 1e9, 1)
 
-    cpu = None
-    if not args.no_cpu:
-        log("bench: CPU baseline (oracle)")
-        rows = 512
-        cpu = cpu_baseline(np.ascontiguousarray(a_host[:rows]), col_host[:rows], row_host,
-                           np.ascontiguousarray(p_host[:rows]), args.cpu_seconds)
-
-    out = {
-        "metric": METRIC,
-        "value": round(value, 2),
-        "unit": "GB/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(dt / args.steps * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": ("C2 bandwidth suite: float32 A (8192,16384) N(0,1) seed 0, broadcast "
-                         "operands (8192,1) and (1,16384), same-shape/broadcast/in-place/scalar "
-                         "binary ops, sum/prod/max/min/mean over axis 0, 1, None, Xoshiro128++ "
-                         "2^20 states -> 2^28 uint32/float/normal"),
-            "elements": N,
-            "ops_per_step": len(suite),
-            "bytes_per_step": step_bytes,
-        },
-        "roofline": {
-            "kernel": "broadcast binary A (8192,16384) + col (8192,1)",
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": PEAK_HBM_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM_GBS, 4),
-            "traffic": None,
-            "avg_launch_ms": round(dom_ms, 5),
-        },
-        "cpu_baseline": cpu,
-        "per_op_GBps": per_op,
+    del suite, ubuf, fbuf, gen, A, B, P  # free the suite's device buffers
+    gpu.wait()
+    return {
+        "value": value, "ms_per_step": dt / args.steps * 1e3, "n_ops": n_ops,
+        "step_bytes": step_bytes, "achieved": achieved, "dom_ms": dom_ms, "per_op": per_op,
+        "host": {"a": a_host, "col": col_host, "row": row_host, "p": p_host},
     }
-    print(json.dumps(out), flush=True)
-    return 0
 
 
 if __name__ == "__main__":
