@@ -84,8 +84,11 @@ int vkh_device_count(int* out);
  * priority argument is accepted and recorded only. */
 int vkh_gpu_create(int idx, float priority, vkh_gpu** out);
 /* context bound to an existing hipStream_t (e.g. torch's current stream);
- * kernels launched through it go to that stream.  Has no counterpart in the
- * reference (torch interop for the autograd wrappers). */
+ * kernels launched through it go to that stream, and its caching allocator
+ * (workspaces, GEMM packing area, vkh_alloc) reuses memory in that stream's
+ * order, so the caller must not use the context's memory from other streams
+ * without synchronising.  Has no counterpart in the reference (torch interop
+ * for the autograd wrappers). */
 int vkh_gpu_create_on_stream(int idx, void* stream, vkh_gpu** out);
 int vkh_gpu_destroy(vkh_gpu* gpu);
 /* replaces GPU::wait (_vkarray.cc:560-562): waits until the stream is idle */
@@ -259,6 +262,13 @@ int vkh_activation(vkh_gpu* gpu, int act, const float* x, float* y, uint64_t n);
  * Sigmoid / Softmax diagonal dy*y*(1-y) (nn/layers.py:264-267, 320-323) */
 int vkh_activation_backward(vkh_gpu* gpu, int act, const float* y,
                             const float* dy, float* dx, uint64_t n);
+/* exact softmax gradient dx = y * (dy - sum_k dy[k] y[k]) per row of
+ * y[rows, cols] = softmax(x), for the autograd wrapper (torch_autograd.py).
+ * No counterpart in the reference, whose Softmax.backward is the diagonal
+ * form dy * y * (1 - y) (nn/layers.py:320-323; vkh_activation_backward with
+ * VKH_ACT_SIGMOID computes that form). */
+int vkh_softmax_backward(vkh_gpu* gpu, const float* y, const float* dy,
+                         float* dx, uint64_t rows, uint64_t cols);
 /* row softmax of x[rows, cols] (nn/layers.py:297-300) */
 int vkh_softmax(vkh_gpu* gpu, const float* x, float* y, uint64_t rows,
                 uint64_t cols);
@@ -283,7 +293,10 @@ int vkh_huber_rows(vkh_gpu* gpu, const float* x, const float* y, float* L,
 /* loss gradients, each multiplied by scale in FP32 afterwards:
  * kind 0 softmax-CE  dx = sm - y          (nn/losses.py:249)
  * kind 1 MSE         dx = (x - y) * 2     (nn/losses.py:318-320)
- * kind 2 Huber       dx = clamp(x-y,-1,1) (nn/losses.py:391-393) */
+ * kind 2 Huber       dx = clamp(x-y,-1,1) (nn/losses.py:391-393)
+ * kind 3 Huber, exact derivative of the forward 0.5*min(d^2, |d|)
+ *        (nn/losses.py:365-369): dx = d for |d| <= 1, 0.5*sign(d) otherwise;
+ *        used by the autograd wrapper, no counterpart in the reference */
 int vkh_loss_grad(vkh_gpu* gpu, int kind, const float* x, const float* y,
                   float* dx, uint64_t n, float scale);
 /* Adam update in the op order of AdamState.grad2diff + Parameter.update

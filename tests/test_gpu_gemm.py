@@ -83,3 +83,48 @@ def test_split_bf16_accuracy(gpu):
     assert errs["bf16"] <= 1e-2
     assert errs["bf16x3"] <= 1e-5
     assert errs["fp32"] <= 1e-6
+
+
+# Skinny products (min(M, N) <= 16) take the FP32 vector path in every mode:
+# row-dot when the long operand is K-contiguous (vectorised when K % 4 == 0
+# and rows are 16-byte aligned), column walk with k chunks otherwise.
+SKINNY = [(4096, 10, 1024), (16, 300, 777), (2000, 16, 4096), (1, 4096, 16384),
+          (16384, 1, 4096), (3, 500, 5000), (700, 7, 3), (9, 9, 64)]
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("M,N,K", SKINNY)
+def test_skinny_paths(gpu, ta, tb, M, N, K):
+    rng = np.random.default_rng(M + 3 * N + 7 * K + 11 * ta + 13 * tb)
+    A = rng.standard_normal((K, M) if ta else (M, K)).astype(np.float32)
+    B = rng.standard_normal((N, K) if tb else (K, N)).astype(np.float32)
+    ref, bound = ref_gemm(ta, tb, A, B)
+    # classical bound of a length-K FP32 dot product: K * 2^-24 * (|A||B|)
+    tol = max(2.0 ** -20, K * 2.0 ** -24)
+    for mode in ("fp32", "bf16"):
+        got = run_gemm(gpu, mode, ta, tb, A, B)
+        err = (np.abs(got - ref) / bound).max()
+        assert err <= tol, (mode, err)
+
+
+@pytest.mark.parametrize("M,N,K,ta,tb", [(300, 10, 64, 0, 1), (10, 300, 64, 1, 0), (257, 1, 33, 1, 1)])
+def test_skinny_epilogue(gpu, M, N, K, ta, tb):
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((K, M) if ta else (M, K)).astype(np.float32)
+    B = rng.standard_normal((N, K) if tb else (K, N)).astype(np.float32)
+    C0 = rng.standard_normal((M, N)).astype(np.float32)
+    bias = rng.standard_normal(N).astype(np.float32)
+    ref, _ = ref_gemm(ta, tb, A, B)
+    got = run_gemm(gpu, "bf16", ta, tb, A, B, C0=C0, alpha=0.5, beta=2.0, bias=bias)
+    np.testing.assert_allclose(got, 0.5 * ref + 2.0 * C0 + bias, rtol=1e-5, atol=1e-4)
+    got = run_gemm(gpu, "fp32", ta, tb, A, B, bias=bias, act="sigmoid")
+    np.testing.assert_allclose(got, 1 / (1 + np.exp(-(ref + bias))), rtol=1e-5, atol=1e-6)
+    # aux: the activation-derivative epilogue of vkh_gemm_ex (ReLU)
+    aux = np.maximum(rng.standard_normal((M, N)), 0).astype(np.float32)
+    gA, gB, gX = vk.Array(gpu, data=A), vk.Array(gpu, data=B), vk.Array(gpu, data=aux)
+    gC = vk.Array(gpu, shape=(M, N))
+    _lib.call("vkh_gemm_ex", gpu._handle, _lib.GEMM_MODE["fp32"], ta, tb, M, N, K, 1.0,
+              gA._ptr, A.shape[1], gB._ptr, B.shape[1], 0.0, gC._ptr, N, None, 0, None,
+              gX._ptr, N, _lib.ACT["relu"])
+    gC._set_job([gA, gB, gX])
+    np.testing.assert_allclose(np.asarray(gC), ref * (aux > 0), rtol=1e-5, atol=1e-5)

@@ -101,3 +101,42 @@ def test_large_state_seeding_parallel(gpu):
     # oracle's sequential jumps
     r = vk.random.Xoshiro128pp(gpu, size=1 << 16, seed=1)
     np.testing.assert_array_equal(r.state(), O.xoshiro_seed(1 << 16, 1))
+
+
+@pytest.mark.parametrize("size", [1, 2, 3, 64, 1000])
+def test_few_states_chunked_streams(gpu, size):
+    """Few states and long streams take the chunked kernels (each state's
+    chain split into chunks started by GF(2) jump-ahead): outputs, the state
+    afterwards and the continuation must equal the sequential generator."""
+    r = vk.random.Xoshiro128pp(gpu, size=size, seed=17)
+    o = O.Xoshiro128pp(size, 17)
+    n = (1 << 20) + 12345
+    np.testing.assert_array_equal(np.asarray(r.randint(shape=(n,))), o.randint(n))
+    np.testing.assert_array_equal(r.state(), o.state)
+    got = np.asarray(r.random(shape=(n - 777,)))
+    np.testing.assert_array_equal(got.view(np.uint32), o.random(n - 777).view(np.uint32))
+    np.testing.assert_array_equal(r.state(), o.state)
+    got = np.asarray(r.randrange(shape=(n // 2,), low=5, high=1000))
+    np.testing.assert_array_equal(got, o.randrange(n // 2, 5, 1000))
+    for m in (n, n + 1):  # even and odd counts (odd: n + 1 uniforms)
+        got = np.asarray(r.normal(shape=(m,), mean=0.5, stddev=2.0))
+        want = o.normal(m, 0.5, 2.0)
+        scale = np.abs(want - np.float32(0.5)) + np.float32(0.5)
+        assert np.all(np.abs(got - want) <= 4 * np.spacing(scale)), np.abs(got - want).max()
+        np.testing.assert_array_equal(r.state(), o.state)
+    # short calls after the long ones continue the same streams
+    np.testing.assert_array_equal(np.asarray(r.randint(shape=(33,))), o.randint(33))
+    np.testing.assert_array_equal(r.state(), o.state)
+
+
+def test_henormal_shape_chunked(gpu):
+    """The reference initializer's configuration (64 states, one call of
+    in x out normals) at the C5 layer size."""
+    r = vk.random.Xoshiro128pp(gpu, size=64, seed=2)
+    o = O.Xoshiro128pp(64, 2)
+    n = 4096 * 4096
+    got = np.asarray(r.normal(shape=(n,), stddev=0.02))
+    want = o.normal(n, 0.0, 0.02)
+    assert np.all(np.abs(got - want) <= 4 * np.spacing(np.abs(want) + np.float32(0.0))
+                  + 4 * np.spacing(np.float32(0.02))), np.abs(got - want).max()
+    np.testing.assert_array_equal(r.state(), o.state)

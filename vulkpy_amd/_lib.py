@@ -103,6 +103,7 @@ _SIGNATURES = {
     "vkh_activation": (c_int, [P, c_int, P, P, U64]),
     "vkh_activation_backward": (c_int, [P, c_int, P, P, P, U64]),
     "vkh_softmax": (c_int, [P, P, P, U64, U64]),
+    "vkh_softmax_backward": (c_int, [P, P, P, P, U64, U64]),
     "vkh_cross_entropy": (c_int, [P, P, P, P, U64, U64]),
     "vkh_cross_entropy_backward": (c_int, [P, P, P, P, U64, c_float]),
     "vkh_softmax_cross_entropy": (c_int, [P, P, P, P, P, U64, U64]),

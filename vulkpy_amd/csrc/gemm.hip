@@ -328,9 +328,9 @@ static void launch_mode(vkh_gpu* g, bool ta, bool tb, GemmArgs& a, int va, int v
 // BF16 buffers ([Mp][Kp], [Np][Kp]; for BF16X3 also the low parts), so the
 // main kernel's inner loop is 16-byte loads, 16-byte LDS stores and MFMAs with
 // no conversion and no bounds checks.
-constexpr int PK_BK = 32;           // k per LDS stage
-constexpr int PK_LDK = PK_BK + 8;   // LDS row of BF16 (80 B: rows stay 16-byte aligned)
-constexpr int PK_TR = 64;           // transpose tile rows (r) in the pack kernel
+constexpr int PK_KALIGN = 64;  // packed K (Kp) is a multiple of every kernel's k slice
+constexpr int PK_TR = 64;      // transpose tile rows (r) in k_pack_cols
+constexpr int PK_TK = 32;      // transpose tile k in k_pack_cols (divides PK_KALIGN)
 
 // dst[r][k] = bf16(op(src)[r][k]) for r < Rp, k < Kp, zero outside [R, K);
 // op(src)[r][k] = src[r*ld + k] (TRANS = false) or src[k*ld + r] (TRANS = true).
@@ -374,22 +374,22 @@ k_pack_rows(const float* src, uint64_t ld, uint64_t R, uint64_t K, uint16_t* hi,
   }
 }
 
-// transposed source: a PK_TR x PK_BK tile goes through LDS so that both the
+// transposed source: a PK_TR x PK_TK tile goes through LDS so that both the
 // reads (along r) and the writes (along k) are contiguous
 template <bool SPLIT>
 __global__ void __launch_bounds__(256)
 k_pack_cols(const float* src, uint64_t ld, uint64_t R, uint64_t K, uint16_t* hi, uint16_t* lo,
             uint64_t Rp, uint64_t Kp) {
-  __shared__ float t[PK_BK][PK_TR + 1];
-  const uint64_t r0 = (uint64_t)blockIdx.x * PK_TR, k0 = (uint64_t)blockIdx.y * PK_BK;
-  for (int idx = threadIdx.x; idx < PK_BK * PK_TR; idx += 256) {
+  static_assert(PK_TR * (PK_TK / 8) == 256, "one 8-element chunk per thread");
+  __shared__ float t[PK_TK][PK_TR + 1];
+  const uint64_t r0 = (uint64_t)blockIdx.x * PK_TR, k0 = (uint64_t)blockIdx.y * PK_TK;
+  for (int idx = threadIdx.x; idx < PK_TK * PK_TR; idx += 256) {
     const int kk = idx / PK_TR, rr = idx % PK_TR;
     const uint64_t r = r0 + rr, k = k0 + kk;
     t[kk][rr] = (r < R && k < K) ? src[k * ld + r] : 0.f;
   }
   __syncthreads();
-  // PK_TR rows x PK_BK/8 chunks = 256 chunks: one per thread
-  const int rr = threadIdx.x / (PK_BK / 8), kc = threadIdx.x % (PK_BK / 8);
+  const int rr = threadIdx.x / (PK_TK / 8), kc = threadIdx.x % (PK_TK / 8);
   const uint64_t r = r0 + rr;
   if (r < Rp) {
     float v[8];
@@ -409,12 +409,14 @@ struct PackedArgs {
   GemmArgs e;
 };
 
-// Register stage of one packed operand's ROWS x PK_BK slice: 16-byte loads of
+// Register stage of one packed operand's ROWS x BK slice: 16-byte loads of
 // K-contiguous BF16 rows (both parts in split mode), stored to LDS rows of
-// PK_LDK elements.
-template <int ROWS, bool SPLIT>
+// LDK = BK + 8 elements (the pad keeps rows 16-byte aligned and spreads the
+// rows of a 16-lane LDS access over distinct banks).
+template <int ROWS, int BK, bool SPLIT>
 struct PackedStager {
-  static constexpr int NV = ROWS * PK_BK / 8 / GEMM_THREADS;  // 16-byte vectors per thread
+  static constexpr int LDK = BK + 8;
+  static constexpr int NV = ROWS * BK / 8 / GEMM_THREADS;  // 16-byte vectors per thread
   static constexpr int PARTS = SPLIT ? 2 : 1;
   u32x4_t r[PARTS][NV];
 
@@ -423,7 +425,7 @@ struct PackedStager {
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
       const int idx = threadIdx.x + i * GEMM_THREADS;
-      const int row = idx / (PK_BK / 8), c = idx % (PK_BK / 8);
+      const int row = idx / (BK / 8), c = idx % (BK / 8);
       const uint64_t off = (row0 + row) * Kp + k0 + c * 8;
       r[0][i] = *reinterpret_cast<const u32x4_t*>(hi + off);
       if constexpr (SPLIT) r[1][i] = *reinterpret_cast<const u32x4_t*>(lo + off);
@@ -433,9 +435,9 @@ struct PackedStager {
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
       const int idx = threadIdx.x + i * GEMM_THREADS;
-      const int row = idx / (PK_BK / 8), c = idx % (PK_BK / 8);
-      *reinterpret_cast<u32x4_t*>(hi + row * PK_LDK + c * 8) = r[0][i];
-      if constexpr (SPLIT) *reinterpret_cast<u32x4_t*>(lo + row * PK_LDK + c * 8) = r[1][i];
+      const int row = idx / (BK / 8), c = idx % (BK / 8);
+      *reinterpret_cast<u32x4_t*>(hi + row * LDK + c * 8) = r[0][i];
+      if constexpr (SPLIT) *reinterpret_cast<u32x4_t*>(lo + row * LDK + c * 8) = r[1][i];
     }
   }
 };
@@ -506,15 +508,17 @@ __device__ __forceinline__ void epilogue4(const GemmArgs& e, uint64_t m, uint64_
 }
 
 // BM x BN tile, 4 waves (2 x 2) of (BM/2) x (BN/2), double-buffered LDS and
-// register prefetch of the next K slice; one barrier per slice.  After the
-// K loop the LDS holds each wave's accumulators, 32 rows at a time, so the
-// epilogue reads and writes whole row segments of C.
-template <int BM, int BN, bool SPLIT>
+// register prefetch of the next BK-deep K slice; one barrier per slice.
+// After the K loop the LDS holds each wave's accumulators, 32 rows at a time,
+// so the epilogue reads and writes whole row segments of C.
+template <int BM, int BN, int BK, bool SPLIT>
 __global__ void __launch_bounds__(GEMM_THREADS)
 k_gemm_packed(PackedArgs p) {
+  static_assert(PK_KALIGN % BK == 0, "Kp must be a multiple of the k slice");
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int TM = WM / 32, TN = WN / 32;
   constexpr int PARTS = SPLIT ? 2 : 1;
+  constexpr int PK_LDK = BK + 8;
   constexpr int A_TILE = BM * PK_LDK, B_TILE = BN * PK_LDK;        // BF16 elements
   constexpr int MAIN_BYTES = 2 * PARTS * (A_TILE + B_TILE) * 2;
   constexpr int SROW = WN + 4;                                      // staging row (floats)
@@ -534,16 +538,8 @@ k_gemm_packed(PackedArgs p) {
   const int l32 = lane & 31, lhi = lane >> 5;
   const uint64_t Kp = p.Kp;
 
-  PackedStager<BM, SPLIT> sa;
-  PackedStager<BN, SPLIT> sb;
-  auto gload = [&](uint64_t k0) {
-    sa.load(p.Ah, p.Al, m0, Kp, k0);
-    sb.load(p.Bh, p.Bl, n0, Kp, k0);
-  };
-  auto lstore = [&](int buf) {
-    sa.store(As(buf, 0), As(buf, PARTS - 1));
-    sb.store(Bs(buf, 0), Bs(buf, PARTS - 1));
-  };
+  PackedStager<BM, BK, SPLIT> sa;
+  PackedStager<BN, BK, SPLIT> sb;
 
   f32x16_t acc[TM][TN];
 #pragma unroll
@@ -553,44 +549,89 @@ k_gemm_packed(PackedArgs p) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
 
-  const uint64_t nk = Kp / PK_BK;
-  gload(0);
-  lstore(0);
+  // Operand fragments of one 16-deep MFMA k-step, double-buffered in
+  // registers: the fragments of step s+1 are read from LDS while the MFMAs
+  // of step s issue, so LDS latency hides behind matrix-core work.
+  constexpr int KS = BK / 16;
+  static_assert(KS % 2 == 0, "the first k-step of every slice uses fragment buffer 0");
+  constexpr int NM = TM * TN * (SPLIT ? 3 : 1);  // MFMAs per k-step
+  constexpr int NDS = (TM + TN) * PARTS;         // fragment reads per k-step
+  constexpr int NVM = (PackedStager<BM, BK, SPLIT>::NV + PackedStager<BN, BK, SPLIT>::NV) * PARTS;
+  constexpr int VPP = (NVM + NDS - 1) / NDS;     // global loads per MFMA of the first step
+  constexpr int WPM = (NVM + NM - 1) / NM;       // LDS stores per MFMA of the last step
+  bf16x8_t fa[2][TM], fb[2][TN], fal[2][TM], fbl[2][TN];
+  const int arow = wm * WM + l32, brow = wn * WN + l32, kcol = lhi * 8;
+  auto frags = [&](int buf, int fb_idx, int k) {
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int off = (arow + i * 32) * PK_LDK + k + kcol;
+      fa[fb_idx][i] = *reinterpret_cast<const bf16x8_t*>(As(buf, 0) + off);
+      if constexpr (SPLIT) fal[fb_idx][i] = *reinterpret_cast<const bf16x8_t*>(As(buf, 1) + off);
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int off = (brow + j * 32) * PK_LDK + k + kcol;
+      fb[fb_idx][j] = *reinterpret_cast<const bf16x8_t*>(Bs(buf, 0) + off);
+      if constexpr (SPLIT) fbl[fb_idx][j] = *reinterpret_cast<const bf16x8_t*>(Bs(buf, 1) + off);
+    }
+  };
+
+  const uint64_t nk = Kp / BK;
+  sa.load(p.Ah, p.Al, m0, Kp, 0);
+  sb.load(p.Bh, p.Bl, n0, Kp, 0);
+  sa.store(As(0, 0), As(0, PARTS - 1));
+  sb.store(Bs(0, 0), Bs(0, PARTS - 1));
   __syncthreads();
+  frags(0, 0, 0);
   int cur = 0;
   for (uint64_t kt = 0; kt < nk; ++kt) {
-    const bool more = kt + 1 < nk;
-    if (more) gload((kt + 1) * PK_BK);
+    // Prefetch of the next slice into registers.  The final iteration loads
+    // the last slice again (in bounds, never read): the body has no branch,
+    // so the scheduler sees one region.
+    const uint64_t kn = (kt + 1 < nk ? kt + 1 : kt) * BK;
+    sa.load(p.Ah, p.Al, m0, Kp, kn);
+    sb.load(p.Bh, p.Bl, n0, Kp, kn);
 #pragma unroll
-    for (int ks = 0; ks < PK_BK; ks += 16) {
-      bf16x8_t a[TM], b[TN], al[TM], bl[TN];
-#pragma unroll
-      for (int i = 0; i < TM; ++i) {
-        const int off = (wm * WM + i * 32 + l32) * PK_LDK + ks + lhi * 8;
-        a[i] = *reinterpret_cast<const bf16x8_t*>(As(cur, 0) + off);
-        if (SPLIT) al[i] = *reinterpret_cast<const bf16x8_t*>(As(cur, PARTS - 1) + off);
-      }
-#pragma unroll
-      for (int j = 0; j < TN; ++j) {
-        const int off = (wn * WN + j * 32 + l32) * PK_LDK + ks + lhi * 8;
-        b[j] = *reinterpret_cast<const bf16x8_t*>(Bs(cur, 0) + off);
-        if (SPLIT) bl[j] = *reinterpret_cast<const bf16x8_t*>(Bs(cur, PARTS - 1) + off);
-      }
+    for (int s = 0; s < KS; ++s) {
+      const int c = s & 1;
+      if (s + 1 < KS) frags(cur, c ^ 1, (s + 1) * 16);
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
-          if (SPLIT) {
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], b[j], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], bl[j], acc[i][j], 0, 0, 0);
+          if constexpr (SPLIT) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fal[c][i], fb[c][j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[c][i], fbl[c][j], acc[i][j], 0, 0, 0);
           }
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[c][i], fb[c][j], acc[i][j], 0, 0, 0);
         }
+      // Issue order within the slice: every k-step but the last alternates
+      // one MFMA with one fragment read for the next step (the first step
+      // also issues the next slice's global loads, VPP per pair, so that
+      // they have the whole slice to arrive); the last step alternates its
+      // MFMAs with the LDS stores of the prefetched slice.
+      if (s + 1 < KS) {
+#pragma unroll
+        for (int q = 0; q < NDS; ++q) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
+          if (s == 0) __builtin_amdgcn_sched_group_barrier(0x020, VPP, 0);  // VMEM read
+        }
+        if constexpr (NM > NDS) __builtin_amdgcn_sched_group_barrier(0x008, NM - NDS, 0);
+      }
     }
-    if (more) lstore(cur ^ 1);
+    sa.store(As(cur ^ 1, 0), As(cur ^ 1, PARTS - 1));
+    sb.store(Bs(cur ^ 1, 0), Bs(cur ^ 1, PARTS - 1));
+#pragma unroll
+    for (int q = 0; q < NM; ++q) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);    // MFMA
+      __builtin_amdgcn_sched_group_barrier(0x200, WPM, 0);  // DS write
+    }
     __syncthreads();
     cur ^= 1;
+    if (kt + 1 < nk) frags(cur, 0, 0);
   }
+  __syncthreads();  // every wave is done with the operand tiles before staging
 
   // Epilogue.  The loop above ended with a barrier, so the operand tiles are
   // dead.  acc[i][j][r] is C[m, n] with m = wave row + i*32 + (r/4)*8 +
@@ -633,7 +674,7 @@ static PackedShape packed_shape(int mode, uint64_t M, uint64_t N) {
 static size_t packed_bytes(int mode, uint64_t M, uint64_t N, uint64_t K) {
   const PackedShape s = packed_shape(mode, M, N);
   const uint64_t Mp = ceil_div(M, s.bm) * s.bm, Np = ceil_div(N, s.bn) * s.bn;
-  const uint64_t Kp = ceil_div(std::max<uint64_t>(K, 1), PK_BK) * PK_BK;
+  const uint64_t Kp = ceil_div(std::max<uint64_t>(K, 1), PK_KALIGN) * PK_KALIGN;
   const uint64_t parts = mode == VKH_GEMM_BF16X3 ? 2 : 1;
   return (size_t)((Mp + Np) * Kp * 2 * parts);
 }
@@ -671,7 +712,7 @@ static void pack_operand(vkh_gpu* g, const float* src, uint64_t ld, bool trans, 
     hipLaunchKernelGGL((k_pack_rows<SPLIT>), dim3(grid), dim3(256), 0, g->stream, src, ld, R, K, hi,
                        lo, Rp, Kp, vec);
   } else {
-    dim3 grid((unsigned)ceil_div(Rp, PK_TR), (unsigned)(Kp / PK_BK));
+    dim3 grid((unsigned)ceil_div(Rp, PK_TR), (unsigned)(Kp / PK_TK));
     hipLaunchKernelGGL((k_pack_cols<SPLIT>), grid, dim3(256), 0, g->stream, src, ld, R, K, hi, lo,
                        Rp, Kp);
   }
@@ -682,7 +723,7 @@ static void gemm_packed(vkh_gpu* g, int mode, bool ta, bool tb, GemmArgs& a) {
   const bool split = mode == VKH_GEMM_BF16X3;
   const PackedShape s = packed_shape(mode, a.M, a.N);
   const uint64_t Mp = ceil_div(a.M, s.bm) * s.bm, Np = ceil_div(a.N, s.bn) * s.bn;
-  const uint64_t Kp = ceil_div(std::max<uint64_t>(a.K, 1), PK_BK) * PK_BK;
+  const uint64_t Kp = ceil_div(std::max<uint64_t>(a.K, 1), PK_KALIGN) * PK_KALIGN;
   uint16_t* base = pack_area(g, packed_bytes(mode, a.M, a.N, a.K));
   uint16_t* Ah = base;
   uint16_t* Al = split ? Ah + Mp * Kp : nullptr;
@@ -702,10 +743,192 @@ static void gemm_packed(vkh_gpu* g, int mode, bool ta, bool tb, GemmArgs& a) {
   VKH_REQUIRE(nb < (uint64_t(1) << 31), "GEMM too large");
   PackedArgs pa{Ah, Al, Bh, Bl, Kp, a};
   dim3 grid((unsigned)nb), block(GEMM_THREADS);
-  if (s.bm == 256) hipLaunchKernelGGL((k_gemm_packed<256, 256, false>), grid, block, 0, g->stream, pa);
-  else if (split) hipLaunchKernelGGL((k_gemm_packed<128, 128, true>), grid, block, 0, g->stream, pa);
-  else hipLaunchKernelGGL((k_gemm_packed<128, 128, false>), grid, block, 0, g->stream, pa);
+  if (s.bm == 256) hipLaunchKernelGGL((k_gemm_packed<256, 256, 64, false>), grid, block, 0, g->stream, pa);
+  else if (split) hipLaunchKernelGGL((k_gemm_packed<128, 128, 32, true>), grid, block, 0, g->stream, pa);
+  else hipLaunchKernelGGL((k_gemm_packed<128, 128, 64, false>), grid, block, 0, g->stream, pa);
   check_launch("gemm (packed)");
+}
+
+// ---- skinny products: min(M, N) <= 16 ------------------------------------------------
+// C = op(A) op(B) with one small output dimension Q is a pass over the large
+// operand (the Dense layer with one or ten outputs, its weight gradient):
+// memory-bound, so it runs in FP32 on the vector units in every mode.  The
+// problem is restated as out[p, q] = sum_k X[p, k] Y[k, q] with P the long
+// output dimension; X is read once, Y (Q x K) stays in the caches.
+//   row-dot      X K-contiguous: one wave per p, lanes along k, wave sums
+//   column-walk  X p-contiguous: one thread per p, k chunks, partial sums
+//                combined in chunk order by a second kernel
+struct SkinnyArgs {
+  const float* X;
+  const float* Y;
+  uint64_t sxp, sxk, syk, syq, P, K;
+  int Q;
+  int swap;  // 1: p indexes columns n of C and q rows m
+  GemmArgs e;
+};
+
+__device__ __forceinline__ void skinny_store(const SkinnyArgs& a, uint64_t p, int q, float acc) {
+  const GemmArgs& e = a.e;
+  const uint64_t m = a.swap ? (uint64_t)q : p, n = a.swap ? p : (uint64_t)q;
+  float* cp = e.C + m * e.ldc + n;
+  float v = e.alpha * acc;
+  if (e.beta != 0.f) v += e.beta * *cp;
+  if (e.bias) v += e.bias[n];
+  v = act_apply(e.act, v);
+  if (e.aux) v = act_deriv_mul(e.aux_act, e.aux[m * e.ldaux + n], v);
+  *cp = v;
+  if (e.Cbf) e.Cbf[m * e.ldc + n] = f2bf(v);
+}
+
+__device__ __forceinline__ float wave_sum64(float v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <int QB>
+__global__ void __launch_bounds__(256) k_skinny_rowdot(SkinnyArgs a, int vec) {
+  const uint64_t p = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (p >= a.P) return;  // uniform per wave
+  float acc[QB];
+#pragma unroll
+  for (int q = 0; q < QB; ++q) acc[q] = 0.f;
+  const float* xr = a.X + p * a.sxp;
+  if (vec) {  // X rows and Y columns 16-byte aligned, K % 4 == 0, Y K-contiguous
+    for (uint64_t k = (uint64_t)lane * 4; k < a.K; k += 256) {
+      const float4 x = *reinterpret_cast<const float4*>(xr + k);
+#pragma unroll
+      for (int q = 0; q < QB; ++q) {
+        if (q < a.Q) {
+          const float4 y = *reinterpret_cast<const float4*>(a.Y + q * a.syq + k);
+          acc[q] += x.x * y.x;
+          acc[q] += x.y * y.y;
+          acc[q] += x.z * y.z;
+          acc[q] += x.w * y.w;
+        }
+      }
+    }
+  } else {
+    for (uint64_t k = lane; k < a.K; k += 64) {
+      const float x = xr[k];
+#pragma unroll
+      for (int q = 0; q < QB; ++q)
+        if (q < a.Q) acc[q] += x * a.Y[k * a.syk + q * a.syq];
+    }
+  }
+  float mine = 0.f;
+#pragma unroll
+  for (int q = 0; q < QB; ++q) {
+    const float s = wave_sum64(acc[q]);
+    if (lane == q) mine = s;
+  }
+  if (lane < a.Q) skinny_store(a, p, lane, mine);
+}
+
+// partial sums part[chunk][q][p] over k in [chunk*kchunk, (chunk+1)*kchunk)
+template <int QB>
+__global__ void __launch_bounds__(256)
+k_skinny_colwalk(SkinnyArgs a, uint64_t kchunk, float* part) {
+  const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  const uint64_t chunk = blockIdx.y;
+  if (p >= a.P) return;
+  const uint64_t k0 = chunk * kchunk, k1 = min(a.K, k0 + kchunk);
+  float acc[QB];
+#pragma unroll
+  for (int q = 0; q < QB; ++q) acc[q] = 0.f;
+  for (uint64_t k = k0; k < k1; ++k) {
+    const float x = a.X[k * a.sxk + p];
+#pragma unroll
+    for (int q = 0; q < QB; ++q)
+      if (q < a.Q) acc[q] += x * a.Y[k * a.syk + q * a.syq];
+  }
+#pragma unroll
+  for (int q = 0; q < QB; ++q)
+    if (q < a.Q) part[(chunk * a.Q + q) * a.P + p] = acc[q];
+}
+
+__global__ void __launch_bounds__(256)
+k_skinny_combine(SkinnyArgs a, const float* part, uint64_t nchunk) {
+  const uint64_t total = a.P * (uint64_t)a.Q;
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += stride) {
+    const int q = (int)(t / a.P);
+    const uint64_t p = t - (uint64_t)q * a.P;
+    float s = 0.f;
+    for (uint64_t c = 0; c < nchunk; ++c) s += part[(c * a.Q + q) * a.P + p];
+    skinny_store(a, p, q, s);
+  }
+}
+
+constexpr uint64_t SKINNY_MAX = 16;  // largest small dimension of the skinny path
+
+static void gemm_skinny(vkh_gpu* g, bool ta, bool tb, const GemmArgs& e) {
+  SkinnyArgs a{};
+  a.e = e;
+  if (e.N <= e.M) {  // q = n, p = m: X = op(A), Y = op(B)
+    a.swap = 0;
+    a.P = e.M;
+    a.Q = (int)e.N;
+    a.X = e.A;
+    a.sxp = ta ? 1 : e.lda;
+    a.sxk = ta ? e.lda : 1;
+    a.Y = e.B;
+    a.syk = tb ? 1 : e.ldb;
+    a.syq = tb ? e.ldb : 1;
+  } else {  // q = m, p = n: X = op(B)^T, Y = op(A)^T
+    a.swap = 1;
+    a.P = e.N;
+    a.Q = (int)e.M;
+    a.X = e.B;
+    a.sxp = tb ? e.ldb : 1;
+    a.sxk = tb ? 1 : e.ldb;
+    a.Y = e.A;
+    a.syk = ta ? e.lda : 1;
+    a.syq = ta ? 1 : e.lda;
+  }
+  a.K = e.K;
+  const int qb = a.Q <= 1 ? 1 : a.Q <= 2 ? 2 : a.Q <= 4 ? 4 : a.Q <= 8 ? 8 : 16;
+  if (a.sxk == 1) {
+    const int vec = ((uintptr_t)a.X & 15u) == 0 && a.sxp % 4 == 0 && a.K % 4 == 0 && a.syk == 1 &&
+                    ((uintptr_t)a.Y & 15u) == 0 && a.syq % 4 == 0;
+    const uint64_t blocks = ceil_div(a.P, 4);
+    VKH_REQUIRE(blocks < (uint64_t(1) << 31), "GEMM too large");
+    dim3 grid((unsigned)blocks), block(256);
+    switch (qb) {
+      case 1: hipLaunchKernelGGL(k_skinny_rowdot<1>, grid, block, 0, g->stream, a, vec); break;
+      case 2: hipLaunchKernelGGL(k_skinny_rowdot<2>, grid, block, 0, g->stream, a, vec); break;
+      case 4: hipLaunchKernelGGL(k_skinny_rowdot<4>, grid, block, 0, g->stream, a, vec); break;
+      case 8: hipLaunchKernelGGL(k_skinny_rowdot<8>, grid, block, 0, g->stream, a, vec); break;
+      default: hipLaunchKernelGGL(k_skinny_rowdot<16>, grid, block, 0, g->stream, a, vec); break;
+    }
+    check_launch("gemm (skinny row-dot)");
+    return;
+  }
+  // column walk with k chunks: about 1024 threads per compute unit, at
+  // least 64 k per chunk
+  const uint64_t target = (uint64_t)(g->num_cus > 0 ? g->num_cus : 256) * 1024;
+  uint64_t nchunk = std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(target, a.P), ceil_div(a.K, 64)));
+  nchunk = std::min<uint64_t>(nchunk, 65535);
+  const uint64_t kchunk = ceil_div(std::max<uint64_t>(a.K, 1), nchunk);
+  nchunk = ceil_div(std::max<uint64_t>(a.K, 1), kchunk);
+  Workspace ws(g, sizeof(float) * nchunk * (uint64_t)a.Q * a.P);
+  float* part = (float*)ws.get();
+  const uint64_t bx = ceil_div(a.P, 256);
+  VKH_REQUIRE(bx < (uint64_t(1) << 31), "GEMM too large");
+  dim3 grid((unsigned)bx, (unsigned)nchunk), block(256);
+  switch (qb) {
+    case 1: hipLaunchKernelGGL(k_skinny_colwalk<1>, grid, block, 0, g->stream, a, kchunk, part); break;
+    case 2: hipLaunchKernelGGL(k_skinny_colwalk<2>, grid, block, 0, g->stream, a, kchunk, part); break;
+    case 4: hipLaunchKernelGGL(k_skinny_colwalk<4>, grid, block, 0, g->stream, a, kchunk, part); break;
+    case 8: hipLaunchKernelGGL(k_skinny_colwalk<8>, grid, block, 0, g->stream, a, kchunk, part); break;
+    default: hipLaunchKernelGGL(k_skinny_colwalk<16>, grid, block, 0, g->stream, a, kchunk, part); break;
+  }
+  check_launch("gemm (skinny column walk)");
+  const uint64_t total = a.P * (uint64_t)a.Q;
+  hipLaunchKernelGGL(k_skinny_combine, dim3(full_grid(total, 256)), dim3(256), 0, g->stream, a,
+                     (const float*)part, nchunk);
+  check_launch("gemm (skinny combine)");
 }
 
 // ---- helpers for the nn path -------------------------------------------------------------
@@ -742,6 +965,11 @@ static void gemm_impl(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M
   a.vec_c = ((uintptr_t)Cp & 15u) == 0 && ldc % 4 == 0 && (!bias || ((uintptr_t)bias & 15u) == 0) &&
             (!aux || (((uintptr_t)aux & 15u) == 0 && ldaux % 4 == 0)) &&
             (!c_bf16 || ((uintptr_t)c_bf16 & 7u) == 0);
+  VKH_REQUIRE(mode >= VKH_GEMM_FP32 && mode <= VKH_GEMM_BF16X3, "gemm: unknown mode");
+  if (std::min(M, N) <= SKINNY_MAX && K > 0) {
+    gemm_skinny(g, trans_a, trans_b, a);
+    return;
+  }
   const int va = ((uintptr_t)A & 15u) == 0 && lda % 4 == 0;
   const int vb = ((uintptr_t)B & 15u) == 0 && ldb % 4 == 0;
   switch (mode) {

@@ -128,6 +128,34 @@ k_softmax_rows_block(const float* x, float* y, uint64_t rows, uint64_t cols) {
   }
 }
 
+// Softmax Jacobian-vector product: dx = y * (dy - sum_k dy[k] y[k]) per row
+// (the exact gradient, used by the autograd wrapper; the reference layer's
+// own backward is the diagonal form dy * y * (1 - y), k_act_bwd's job).
+__global__ void __launch_bounds__(256)
+k_softmax_bwd_thread(const float* y, const float* dy, float* dx, uint64_t rows, uint64_t cols) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x; r < rows; r += stride) {
+    const float* yr = y + r * cols;
+    const float* gr = dy + r * cols;
+    float s = 0.0f;
+    for (uint64_t j = 0; j < cols; ++j) s += gr[j] * yr[j];
+    for (uint64_t j = 0; j < cols; ++j) dx[r * cols + j] = yr[j] * (gr[j] - s);
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_softmax_bwd_block(const float* y, const float* dy, float* dx, uint64_t rows, uint64_t cols) {
+  __shared__ float lds[4];
+  for (uint64_t r = blockIdx.x; r < rows; r += gridDim.x) {
+    const float* yr = y + r * cols;
+    const float* gr = dy + r * cols;
+    float s = 0.0f;
+    for (uint64_t j = threadIdx.x; j < cols; j += 256) s += gr[j] * yr[j];
+    s = block_sum(s, lds);
+    for (uint64_t j = threadIdx.x; j < cols; j += 256) dx[r * cols + j] = yr[j] * (gr[j] - s);
+  }
+}
+
 // nn_cross_entropy.comp:26 + sum(axis=1): L[r] = sum_j (-y) * log(x + 1e-8)
 __device__ __forceinline__ float xent_term(float x, float y) { return (-y) * dlog(x + 1e-8f); }
 
@@ -198,6 +226,7 @@ k_loss_grad(int kind, const float* x, const float* y, float* dx, uint64_t n, flo
     float d = x[i] - y[i];
     if (kind == 1) d = d * 2.0f;
     else if (kind == 2) d = np_min(np_max(d, -1.0f), 1.0f);
+    else if (kind == 3) d = (fabsf(d) <= 1.0f || d != d) ? d : (d > 0.0f ? 0.5f : -0.5f);
     if (scale != 1.0f) d = d * scale;
     dx[i] = d;
   }
@@ -347,6 +376,23 @@ int vkh_softmax(vkh_gpu* g, const float* x, float* y, uint64_t rows, uint64_t co
   });
 }
 
+int vkh_softmax_backward(vkh_gpu* g, const float* y, const float* dy, float* dx, uint64_t rows,
+                         uint64_t cols) {
+  return guarded([&] {
+    use(g);
+    if (rows == 0 || cols == 0) return;
+    VKH_REQUIRE(y && dy && dx, "softmax_backward: null pointer");
+    if (cols <= 64) {
+      unsigned grid = (unsigned)std::min<uint64_t>(ceil_div(rows, 256), (uint64_t)g->num_cus * 8);
+      hipLaunchKernelGGL(k_softmax_bwd_thread, dim3(grid), dim3(256), 0, g->stream, y, dy, dx, rows, cols);
+    } else {
+      unsigned grid = (unsigned)std::min<uint64_t>(rows, (uint64_t)g->num_cus * 16);
+      hipLaunchKernelGGL(k_softmax_bwd_block, dim3(grid), dim3(256), 0, g->stream, y, dy, dx, rows, cols);
+    }
+    check_launch("softmax_backward");
+  });
+}
+
 int vkh_cross_entropy(vkh_gpu* g, const float* x, const float* y, float* L, uint64_t rows, uint64_t cols) {
   return guarded([&] {
     use(g);
@@ -414,7 +460,7 @@ int vkh_loss_grad(vkh_gpu* g, int kind, const float* x, const float* y, float* d
     use(g);
     if (n == 0) return;
     VKH_REQUIRE(x && y && dx, "loss_grad: null pointer");
-    VKH_REQUIRE(kind >= 0 && kind <= 2, "loss_grad: unknown kind");
+    VKH_REQUIRE(kind >= 0 && kind <= 3, "loss_grad: unknown kind");
     hipLaunchKernelGGL(k_loss_grad, dim3(grid_for(g, n)), dim3(256), 0, g->stream, kind, x, y, dx, n, scale);
     check_launch("loss_grad");
   });

@@ -102,6 +102,45 @@ static Mat mat_pow(Mat base, uint64_t p) {
   return r;
 }
 
+// one step of next() as a matrix (the state update is linear; the ++ output
+// scrambler is not part of it)
+static Mat step_matrix() {
+  Mat M;
+  for (int k = 0; k < 128; ++k) {
+    St e{{0, 0, 0, 0}};
+    e.w[k >> 5] = 1u << (k & 31);
+    next_host(e);
+    M[k] = e;
+  }
+  return M;
+}
+
+// P_j = M^(Lc * 2^j), j < nbits, as [nbits][128 columns][4 words].  Kept
+// per chunk length for the life of the process (generators made by the
+// same initializer shapes share chunk lengths).
+static void chunk_jumps(uint64_t Lc, int nbits, std::vector<uint32_t>& out) {
+  static std::mutex mu;
+  static std::unordered_map<uint64_t, std::vector<uint32_t>> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  std::vector<uint32_t>& v = cache[Lc];
+  int have = (int)(v.size() / 512);
+  if (have < nbits) {
+    Mat P;
+    if (have == 0) {
+      P = mat_pow(step_matrix(), Lc);
+    } else {
+      for (int k = 0; k < 128; ++k) std::memcpy(P[k].w, &v[((size_t)(have - 1) * 128 + k) * 4], 16);
+      P = compose(P, P);
+    }
+    v.resize((size_t)nbits * 512);
+    for (int j = have; j < nbits; ++j) {
+      if (j > have) P = compose(P, P);
+      for (int k = 0; k < 128; ++k) std::memcpy(&v[((size_t)j * 128 + k) * 4], P[k].w, 16);
+    }
+  }
+  out.assign(v.begin(), v.begin() + (size_t)nbits * 512);
+}
+
 static void seed_states(uint32_t S, uint64_t seed, std::vector<uint32_t>& out) {
   out.resize((size_t)4 * S);
   St s0;
@@ -175,6 +214,67 @@ k_xoshiro(uint4* state, void* out, uint32_t S, uint64_t n, uint32_t low, uint32_
   state[i] = s;
 }
 
+// ---- chunked generation for few states ---------------------------------------------
+// With S states and n outputs each state takes up to ceil(n / S) sequential
+// steps; when S is small (the reference's HeNormal uses 64 states,
+// nn/initializers.py:75-89) that serial chain is the whole kernel.  Here
+// thread (i, c) produces steps [c*Lc, (c+1)*Lc) of state i: it starts from
+// M^(c*Lc) applied to the state (M = one next() step over GF(2)), built
+// from the precomputed P_j = M^(Lc * 2^j) for the set bits j of c.  Outputs
+// keep the mapping k = t*S + i.  Start states are read from a copy of the
+// state buffer; the chunk holding a state's last step writes it back.
+
+// y = P x for a 128x128 bit matrix given by its columns (4 words each)
+__device__ __forceinline__ uint4 gf2_apply(const uint32_t* __restrict__ P, uint4 x) {
+  uint4 y = make_uint4(0u, 0u, 0u, 0u);
+  const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    for (int b = 0; b < 32; ++b) {
+      if ((w[q] >> b) & 1u) {
+        const uint32_t* col = P + (q * 32 + b) * 4;
+        y.x ^= col[0]; y.y ^= col[1]; y.z ^= col[2]; y.w ^= col[3];
+      }
+    }
+  }
+  return y;
+}
+
+__device__ __forceinline__ uint4 chunk_start(const uint4* init, const uint32_t* jumps, uint64_t i,
+                                             uint64_t c) {
+  uint4 s = init[i];
+  for (int j = 0; (c >> j) != 0; ++j)
+    if ((c >> j) & 1u) s = gf2_apply(jumps + (size_t)j * 512, s);
+  return s;
+}
+
+// steps of state i when n outputs are taken round-robin from S states
+__device__ __forceinline__ uint64_t steps_of(uint64_t i, uint64_t S, uint64_t n) {
+  return n > i ? (n - 1 - i) / S + 1 : 0;
+}
+
+template <int OUT>
+__global__ void __launch_bounds__(PRNG_BLOCK)
+k_xoshiro_chunked(const uint4* init, uint4* state, const uint32_t* jumps, void* out, uint32_t S,
+                  uint64_t n, uint64_t Lc, uint64_t nchunks, uint32_t low, uint32_t range) {
+  const uint64_t tid = (uint64_t)blockIdx.x * PRNG_BLOCK + threadIdx.x;
+  if (tid >= (uint64_t)S * nchunks) return;
+  const uint64_t i = tid % S, c = tid / S;
+  const uint64_t L = steps_of(i, S, n);
+  const uint64_t t0 = c * Lc;
+  if (t0 >= L) return;
+  const uint64_t t1 = min(t0 + Lc, L);
+  uint4 s = chunk_start(init, jumps, i, c);
+  for (uint64_t t = t0; t < t1; ++t) {
+    const uint64_t k = t * S + i;
+    const uint32_t r = dnext(s);
+    if (OUT == 0) st_stream1(static_cast<uint32_t*>(out) + k, r);
+    else if (OUT == 1) st_stream1(static_cast<float*>(out) + k, to_unit(r));
+    else st_stream1(static_cast<uint32_t*>(out) + k, low + __float2uint_rz((float)range * to_unit(r)));
+  }
+  if (t1 == L) state[i] = s;
+}
+
 // Box-Muller of one pair of uniforms (prng_box_muller.comp:25-31) in the
 // reference's FP32 operation order.  log, sin and cos are the device's FP32
 // functions (within 2 ulp, like the GLSL.std.450 ones the reference relies
@@ -208,6 +308,37 @@ k_xoshiro_normal(uint4* state, float* out, uint32_t S, uint64_t nu, uint64_t n, 
   }
   state[i] = s1;
   state[i + 1] = s2;
+}
+
+// chunked form of k_xoshiro_normal: thread (p, c) advances states 2p and 2p+1
+// over steps [c*Lc, (c+1)*Lc); both take the same number of steps because S
+// and nu are even
+__global__ void __launch_bounds__(PRNG_BLOCK)
+k_xoshiro_normal_chunked(const uint4* init, uint4* state, const uint32_t* jumps, float* out,
+                         uint32_t S, uint64_t nu, uint64_t n, uint64_t Lc, uint64_t nchunks,
+                         float mean, float sd) {
+  const uint64_t tid = (uint64_t)blockIdx.x * PRNG_BLOCK + threadIdx.x;
+  const uint64_t half = S / 2;
+  if (tid >= half * nchunks) return;
+  const uint64_t p = tid % half, c = tid / half, i = 2 * p;
+  const uint64_t L = steps_of(i, S, nu);
+  const uint64_t t0 = c * Lc;
+  if (t0 >= L) return;
+  const uint64_t t1 = min(t0 + Lc, L);
+  uint4 s1 = chunk_start(init, jumps, i, c), s2 = chunk_start(init, jumps, i + 1, c);
+  for (uint64_t t = t0; t < t1; ++t) {
+    const uint64_t j = t * S + i;
+    const float u1 = to_unit(dnext(s1));
+    const float u2 = to_unit(dnext(s2));
+    float o1, o2;
+    box_muller(u1, u2, mean, sd, o1, o2);
+    if (j < n) st_stream1(out + j, o1);
+    if (j + 1 < n) st_stream1(out + j + 1, o2);
+  }
+  if (t1 == L) {
+    state[i] = s1;
+    state[i + 1] = s2;
+  }
 }
 
 __global__ void __launch_bounds__(PRNG_BLOCK)
@@ -257,11 +388,92 @@ struct vkh_xoshiro {
   uint32_t size = 0;
   uint32_t* state = nullptr;  // [size][4]
   bool from_alloc = false;
+  // jump matrices of the chunked kernels, valid for chunk length jumps_lc
+  uint32_t* jumps = nullptr;
+  size_t jumps_bytes = 0;
+  uint64_t jumps_lc = 0;
+  int jumps_nbits = 0;
 };
 
 using namespace vkh;
 
 static unsigned prng_grid(uint64_t threads) { return (unsigned)ceil_div(threads, PRNG_BLOCK); }
+
+namespace {
+
+struct ChunkPlan {
+  bool use = false;
+  uint64_t Lc = 0, C = 1;
+  int nbits = 0;
+};
+
+// Chunk the per-state chains when `units` parallel chains (states, or state
+// pairs) are too few to fill the GPU and each is long: about 1024 threads
+// per compute unit, and at least 256 steps per chunk so the start-state
+// matrix products stay small next to the generation.  Not during graph
+// capture (the jump matrices are uploaded synchronously).
+ChunkPlan plan_chunks(const vkh_gpu* g, uint64_t units, uint64_t steps) {
+  ChunkPlan p;
+  if (g->capturing) return p;
+  const uint64_t target = (uint64_t)(g->num_cus > 0 ? g->num_cus : 256) * 1024;
+  if (units * 4 >= target) return p;
+  const uint64_t C = std::min<uint64_t>(ceil_div(target, units), steps / 256);
+  if (C < 2) return p;
+  p.Lc = ceil_div(steps, C);
+  p.C = ceil_div(steps, p.Lc);
+  if (p.C < 2) return p;
+  p.use = true;
+  while ((uint64_t(1) << p.nbits) < p.C) ++p.nbits;  // bits of c in [0, C)
+  return p;
+}
+
+const uint32_t* prepare_jumps(vkh_xoshiro* r, const ChunkPlan& p) {
+  if (r->jumps && r->jumps_lc == p.Lc && r->jumps_nbits >= p.nbits) return r->jumps;
+  std::vector<uint32_t> host;
+  chunk_jumps(p.Lc, p.nbits, host);
+  const size_t nbytes = host.size() * sizeof(uint32_t);
+  vkh_gpu* g = r->gpu;
+  if (nbytes > r->jumps_bytes) {
+    if (r->jumps) {
+      VKH_HIP(hipStreamSynchronize(g->stream));
+      VKH_HIP(hipFree(r->jumps));
+      r->jumps = nullptr;
+      r->jumps_bytes = 0;
+    }
+    VKH_HIP(hipMalloc(&r->jumps, nbytes));
+    r->jumps_bytes = nbytes;
+  }
+  // synchronous: earlier chunked launches may still read the old matrices,
+  // and the host vector dies at the end of this call
+  VKH_HIP(hipStreamSynchronize(g->stream));
+  VKH_HIP(hipMemcpy(r->jumps, host.data(), nbytes, hipMemcpyHostToDevice));
+  r->jumps_lc = p.Lc;
+  r->jumps_nbits = p.nbits;
+  return r->jumps;
+}
+
+template <int OUT>
+void launch_uniform(vkh_xoshiro* r, void* out, uint64_t n, uint32_t low, uint32_t range) {
+  vkh_gpu* g = r->gpu;
+  const ChunkPlan cp = plan_chunks(g, r->size, ceil_div(n, r->size));
+  if (cp.use) {
+    const uint32_t* jumps = prepare_jumps(r, cp);
+    const size_t sb = (size_t)16 * r->size;
+    Workspace init(g, sb);
+    VKH_HIP(hipMemcpyAsync(init.get(), r->state, sb, hipMemcpyDeviceToDevice, g->stream));
+    const uint64_t th = (uint64_t)r->size * cp.C;
+    hipLaunchKernelGGL(k_xoshiro_chunked<OUT>, dim3(prng_grid(th)), dim3(PRNG_BLOCK), 0, g->stream,
+                       (const uint4*)init.get(), (uint4*)r->state, jumps, out, r->size, n, cp.Lc, cp.C,
+                       low, range);
+  } else {
+    const uint64_t th = std::min<uint64_t>(r->size, n);
+    hipLaunchKernelGGL(k_xoshiro<OUT>, dim3(prng_grid(th)), dim3(PRNG_BLOCK), 0, g->stream,
+                       (uint4*)r->state, out, r->size, n, low, range);
+  }
+  check_launch("xoshiro");
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -301,6 +513,10 @@ int vkh_xoshiro_destroy(vkh_xoshiro* r) {
     use(r->gpu);
     if (r->from_alloc) r->gpu->alloc->release(r->state);
     else { VKH_HIP(hipStreamSynchronize(r->gpu->stream)); VKH_HIP(hipFree(r->state)); }
+    if (r->jumps) {
+      VKH_HIP(hipStreamSynchronize(r->gpu->stream));
+      VKH_HIP(hipFree(r->jumps));
+    }
     delete r;
   });
 }
@@ -319,10 +535,7 @@ int vkh_xoshiro_uint32(vkh_xoshiro* r, uint32_t* out, uint64_t n) {
     use(r->gpu);
     if (n == 0) return;
     VKH_REQUIRE(out, "null output");
-    uint64_t th = std::min<uint64_t>(r->size, n);
-    hipLaunchKernelGGL(k_xoshiro<0>, dim3(prng_grid(th)), dim3(PRNG_BLOCK), 0, r->gpu->stream,
-                       (uint4*)r->state, (void*)out, r->size, n, 0u, 0u);
-    check_launch("xoshiro uint32");
+    launch_uniform<0>(r, out, n, 0u, 0u);
   });
 }
 
@@ -332,10 +545,7 @@ int vkh_xoshiro_float(vkh_xoshiro* r, float* out, uint64_t n) {
     use(r->gpu);
     if (n == 0) return;
     VKH_REQUIRE(out, "null output");
-    uint64_t th = std::min<uint64_t>(r->size, n);
-    hipLaunchKernelGGL(k_xoshiro<1>, dim3(prng_grid(th)), dim3(PRNG_BLOCK), 0, r->gpu->stream,
-                       (uint4*)r->state, (void*)out, r->size, n, 0u, 0u);
-    check_launch("xoshiro float");
+    launch_uniform<1>(r, out, n, 0u, 0u);
   });
 }
 
@@ -346,10 +556,7 @@ int vkh_xoshiro_randrange(vkh_xoshiro* r, uint32_t* out, uint64_t n, uint32_t lo
     if (n == 0) return;
     VKH_REQUIRE(out, "null output");
     uint32_t range = high_incl - low + 1u;  // uint32 arithmetic as in the shader
-    uint64_t th = std::min<uint64_t>(r->size, n);
-    hipLaunchKernelGGL(k_xoshiro<2>, dim3(prng_grid(th)), dim3(PRNG_BLOCK), 0, r->gpu->stream,
-                       (uint4*)r->state, (void*)out, r->size, n, low, range);
-    check_launch("xoshiro randrange");
+    launch_uniform<2>(r, out, n, low, range);
   });
 }
 
@@ -374,19 +581,28 @@ int vkh_xoshiro_normal(vkh_xoshiro* r, float* out, uint64_t n, float mean, float
     VKH_REQUIRE(out, "null output");
     const uint64_t nu = n + (n & 1);  // random.py:105-123: n even, or n + 1 uniforms
     if (r->size % 2 == 0) {
-      uint64_t th = std::min<uint64_t>(r->size, nu) / 2;
-      hipLaunchKernelGGL(k_xoshiro_normal, dim3(prng_grid(th)), dim3(PRNG_BLOCK), 0, g->stream,
-                         (uint4*)r->state, out, r->size, nu, n, mean, sd);
+      const ChunkPlan cp = plan_chunks(g, r->size / 2, ceil_div(nu, r->size));
+      if (cp.use) {
+        const uint32_t* jumps = prepare_jumps(r, cp);
+        const size_t sb = (size_t)16 * r->size;
+        Workspace init(g, sb);
+        VKH_HIP(hipMemcpyAsync(init.get(), r->state, sb, hipMemcpyDeviceToDevice, g->stream));
+        const uint64_t th = (uint64_t)(r->size / 2) * cp.C;
+        hipLaunchKernelGGL(k_xoshiro_normal_chunked, dim3(prng_grid(th)), dim3(PRNG_BLOCK), 0, g->stream,
+                           (const uint4*)init.get(), (uint4*)r->state, jumps, out, r->size, nu, n,
+                           cp.Lc, cp.C, mean, sd);
+      } else {
+        uint64_t th = std::min<uint64_t>(r->size, nu) / 2;
+        hipLaunchKernelGGL(k_xoshiro_normal, dim3(prng_grid(th)), dim3(PRNG_BLOCK), 0, g->stream,
+                           (uint4*)r->state, out, r->size, nu, n, mean, sd);
+      }
       check_launch("xoshiro normal");
       return;
     }
     // odd number of states: uniforms first, then the transform
     Workspace ws(g, sizeof(float) * nu);
     float* u = (float*)ws.get();
-    uint64_t th = std::min<uint64_t>(r->size, nu);
-    hipLaunchKernelGGL(k_xoshiro<1>, dim3(prng_grid(th)), dim3(PRNG_BLOCK), 0, g->stream,
-                       (uint4*)r->state, (void*)u, r->size, nu, 0u, 0u);
-    check_launch("xoshiro float");
+    launch_uniform<1>(r, u, nu, 0u, 0u);
     uint64_t npairs = nu / 2;
     unsigned grid = stream_grid(g, npairs, PRNG_BLOCK, 8);
     hipLaunchKernelGGL(k_box_muller, dim3(grid), dim3(PRNG_BLOCK), 0, g->stream, u, out, npairs, n, mean, sd);

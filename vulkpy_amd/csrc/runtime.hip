@@ -203,7 +203,10 @@ int vkh_gpu_create_on_stream(int idx, void* stream, vkh_gpu** out) {
     try {
       VKH_HIP(hipSetDevice(idx));
       init_props(g);
-    } catch (...) { delete g; throw; }
+      // workspaces and packing areas of this context are reused in the order
+      // of the caller's stream, which is the only stream the context uses
+      g->alloc = new Allocator(idx);
+    } catch (...) { delete g->alloc; delete g; throw; }
     *out = g;
   });
 }
