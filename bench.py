@@ -141,12 +141,14 @@ def mlp_compare(vk, gpu, steps: int, warmup: int = 2):
     HuberLoss(delta=1), Ridge(1e-4) on every weight (applied, see
     ``regularize``), Adam lr=1e-3, batch 16384, x ~ N(0, 1),
     y = sin(x @ P) + 0.1 N(0, 1) with a fixed projection P (256 x 1).
-    Both arms start from the same HeNormal weights and train `steps` steps on
-    the same batch: the plain FP32 step (nn.Sequence, one launch per op, FP32
-    matrix cores) and the BF16 step captured as one hipGraph
-    (nn.GraphSequence, BF16 matrix cores, FP32 master weights and Adam).
-    steps/s is timed over the steps after the first `warmup` (the graph arm
-    captures on its first step).
+    Every arm starts from the same HeNormal weights and trains `steps` steps
+    on the same batch: the plain FP32 step (nn.Sequence, one launch per op,
+    FP32 matrix cores) and the step captured as one hipGraph
+    (nn.GraphSequence) with BF16 and with split-BF16 (3-product) matrix-core
+    GEMMs, FP32 master weights and Adam in both.  steps/s is timed over the
+    steps after the first `warmup` (the graph arms capture on their first
+    step).  Besides the final training loss, each arm's trained weights are
+    evaluated in FP32 on a held-out batch of the same distribution.
     """
     from vulkpy_amd import nn
     B, D, H = 16384, 256, 4096
@@ -155,6 +157,9 @@ def mlp_compare(vk, gpu, steps: int, warmup: int = 2):
     proj = (rng.standard_normal((D, 1)) / np.sqrt(D)).astype(np.float32)
     y = (np.sin(x @ proj) + 0.1 * rng.standard_normal((B, 1))).astype(np.float32)
     X, Y = vk.Array(gpu, data=x), vk.Array(gpu, data=y)
+    xt = rng.standard_normal((B, D), dtype=np.float32)
+    yt = (np.sin(xt @ proj) + 0.1 * rng.standard_normal((B, 1))).astype(np.float32)
+    Xt, Yt = vk.Array(gpu, data=xt), vk.Array(gpu, data=yt)
     dims = [(D, H), (H, H), (H, 1)]
 
     def layers(precision):
@@ -167,43 +172,67 @@ def mlp_compare(vk, gpu, steps: int, warmup: int = 2):
                 out.append(nn.ReLU())
         return out
 
+    checkpoints = sorted({1, 10, 50, 100, 200, 300, 400, steps} & set(range(1, steps + 1)))
+
     def run(model):
-        losses = []
+        losses = {}
         t0 = None
         for s in range(steps):
             if s == warmup:
                 gpu.wait()
                 t0 = time.perf_counter()
             _, L = model.train(X, Y)
-            if s == 0 or s + 1 == steps:
-                losses.append(float(np.asarray(L)))
+            if s + 1 in checkpoints:  # a handful of scalar reads
+                losses[s + 1] = float(np.asarray(L))
         gpu.wait()
         dt = time.perf_counter() - t0
         return (steps - warmup) / dt, losses
 
+    def heldout(model):
+        """loss of the trained weights on (Xt, Yt), FP32 products"""
+        for L in model.L:
+            if isinstance(L, nn.Dense):
+                L.precision = "fp32"
+        h = Xt
+        for L in model.L:
+            h = L(h)
+        return float(np.asarray(nn.HuberLoss()(h, Yt)))
+
+    arms = (("fp32", lambda: nn.Sequence(layers("fp32"), nn.HuberLoss(), regularize=True)),
+            ("bf16_graph", lambda: nn.GraphSequence(layers("bf16"), nn.HuberLoss(),
+                                                    precision="bf16", regularize=True)),
+            ("bf16x3_graph", lambda: nn.GraphSequence(layers("bf16x3"), nn.HuberLoss(),
+                                                      precision="bf16x3", regularize=True)))
     res = {}
-    for name, model in (("fp32", lambda: nn.Sequence(layers("fp32"), nn.HuberLoss(), regularize=True)),
-                        ("bf16_graph", lambda: nn.GraphSequence(layers("bf16"), nn.HuberLoss(),
-                                                                precision="bf16", regularize=True))):
+    for name, model in arms:
         log(f"bench: MLP {name} {steps} steps")
         m = model()
         sps, losses = run(m)
-        res[name] = {"steps_per_s": round(sps, 2), "loss_first": losses[0], "loss_final": losses[-1]}
+        res[name] = {"steps_per_s": round(sps, 2), "loss_first": losses[1], "loss_final": losses[steps],
+                     "loss_at_step": {str(k): v for k, v in losses.items()},
+                     "heldout_loss": heldout(m)}
         log(f"bench: MLP {name} {res[name]}")
         del m
         gpu.wait()
-    f, b = res["fp32"]["loss_final"], res["bf16_graph"]["loss_final"]
-    return {
+    out = {
         "workload": ("C5 regression MLP 256-4096-4096-1 ReLU, HuberLoss(delta=1), Ridge(1e-4) on "
                      "weights, Adam lr=1e-3, batch 16384, x~N(0,1), y=sin(x@P)+0.1N(0,1)"),
         "steps": steps,
         "timed_steps": steps - warmup,
-        "fp32": res["fp32"],
-        "bf16_graph": res["bf16_graph"],
-        "speedup": round(res["bf16_graph"]["steps_per_s"] / res["fp32"]["steps_per_s"], 3),
-        "final_loss_rel_diff": round(abs(b - f) / abs(f), 5),
         "gemm_flop_per_step": 2 * (B * H * D + B * H * H + B * H) * 2 + 2 * (B * H * H + B * H),
     }
+    ref = res["fp32"]
+    for name, r in res.items():
+        out[name] = r
+        if name == "fp32":
+            continue
+        out[name]["speedup"] = round(r["steps_per_s"] / ref["steps_per_s"], 3)
+        out[name]["final_loss_rel_diff"] = round(abs(r["loss_final"] - ref["loss_final"]) / abs(ref["loss_final"]), 5)
+        out[name]["heldout_rel_diff"] = round(abs(r["heldout_loss"] - ref["heldout_loss"]) / abs(ref["heldout_loss"]), 5)
+        out[name]["rel_diff_at_step"] = {
+            k: round(abs(v - ref["loss_at_step"][k]) / abs(ref["loss_at_step"][k]), 5)
+            for k, v in r["loss_at_step"].items()}
+    return out
 
 
 def gemm_report(vk, gpu, timer, reps: int = 3, sample_every: int = 128):

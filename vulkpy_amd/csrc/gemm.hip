@@ -329,8 +329,8 @@ static void launch_mode(vkh_gpu* g, bool ta, bool tb, GemmArgs& a, int va, int v
 // main kernel's inner loop is 16-byte loads, 16-byte LDS stores and MFMAs with
 // no conversion and no bounds checks.
 constexpr int PK_KALIGN = 64;  // packed K (Kp) is a multiple of every kernel's k slice
-constexpr int PK_TR = 64;      // transpose tile rows (r) in k_pack_cols
-constexpr int PK_TK = 32;      // transpose tile k in k_pack_cols (divides PK_KALIGN)
+constexpr int PK_TR = 128;     // transpose tile rows (r) in k_pack_cols (divides every Rp)
+constexpr int PK_TK = 128;     // transpose tile k in k_pack_cols
 
 // dst[r][k] = bf16(op(src)[r][k]) for r < Rp, k < Kp, zero outside [R, K);
 // op(src)[r][k] = src[r*ld + k] (TRANS = false) or src[k*ld + r] (TRANS = true).
@@ -374,28 +374,49 @@ k_pack_rows(const float* src, uint64_t ld, uint64_t R, uint64_t K, uint16_t* hi,
   }
 }
 
-// transposed source: a PK_TR x PK_TK tile goes through LDS so that both the
-// reads (along r) and the writes (along k) are contiguous
+// Transposed source: a PK_TK (k) x PK_TR (r) tile goes through LDS (rows of
+// PK_TR + 1 floats: the column reads of the second phase fall in distinct
+// banks), so the reads are 16-byte loads along r (512-byte row segments) and
+// the writes 16-byte stores along k (256-byte row segments of the packed
+// buffer).  Rp is a multiple of PK_TR; the last k tile may pass Kp.
 template <bool SPLIT>
 __global__ void __launch_bounds__(256)
 k_pack_cols(const float* src, uint64_t ld, uint64_t R, uint64_t K, uint16_t* hi, uint16_t* lo,
-            uint64_t Rp, uint64_t Kp) {
-  static_assert(PK_TR * (PK_TK / 8) == 256, "one 8-element chunk per thread");
+            uint64_t Rp, uint64_t Kp, int vec) {
   __shared__ float t[PK_TK][PK_TR + 1];
   const uint64_t r0 = (uint64_t)blockIdx.x * PK_TR, k0 = (uint64_t)blockIdx.y * PK_TK;
-  for (int idx = threadIdx.x; idx < PK_TK * PK_TR; idx += 256) {
-    const int kk = idx / PK_TR, rr = idx % PK_TR;
+  constexpr int R4 = PK_TR / 4;
+  constexpr int NL = PK_TK * R4 / 256;  // 16-byte loads per thread, all issued before the LDS writes
+  vkh_f4v v[NL];
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    const int f = threadIdx.x + i * 256;
+    const int kk = f / R4, rr = (f % R4) * 4;
     const uint64_t r = r0 + rr, k = k0 + kk;
-    t[kk][rr] = (r < R && k < K) ? src[k * ld + r] : 0.f;
+    if (vec && k < K && r + 3 < R) {
+      v[i] = *reinterpret_cast<const vkh_f4v*>(src + k * ld + r);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[i][q] = (k < K && r + q < R) ? src[k * ld + r + q] : 0.f;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    const int f = threadIdx.x + i * 256;
+    const int kk = f / R4, rr = (f % R4) * 4;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) t[kk][rr + q] = v[i][q];
   }
   __syncthreads();
-  const int rr = threadIdx.x / (PK_TK / 8), kc = threadIdx.x % (PK_TK / 8);
-  const uint64_t r = r0 + rr;
-  if (r < Rp) {
+  constexpr int KC = PK_TK / 8;  // 8-element chunks per packed row in the tile
+  for (int f = threadIdx.x; f < PK_TR * KC; f += 256) {
+    const int rr = f / KC, kc = f % KC;
+    const uint64_t k = k0 + kc * 8;
+    if (k >= Kp) continue;
     float v[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) v[q] = t[kc * 8 + q][rr];
-    const uint64_t off = r * Kp + k0 + kc * 8;
+    const uint64_t off = (r0 + rr) * Kp + k;
     pack8<SPLIT>(v, hi + off, SPLIT ? lo + off : nullptr);
   }
 }
@@ -712,9 +733,11 @@ static void pack_operand(vkh_gpu* g, const float* src, uint64_t ld, bool trans, 
     hipLaunchKernelGGL((k_pack_rows<SPLIT>), dim3(grid), dim3(256), 0, g->stream, src, ld, R, K, hi,
                        lo, Rp, Kp, vec);
   } else {
-    dim3 grid((unsigned)ceil_div(Rp, PK_TR), (unsigned)(Kp / PK_TK));
+    const int vec = ((uintptr_t)src & 15u) == 0 && ld % 4 == 0;
+    VKH_REQUIRE(Rp % PK_TR == 0, "gemm pack: padded rows not a multiple of the tile");
+    dim3 grid((unsigned)(Rp / PK_TR), (unsigned)ceil_div(Kp, PK_TK));
     hipLaunchKernelGGL((k_pack_cols<SPLIT>), grid, dim3(256), 0, g->stream, src, ld, R, K, hi, lo,
-                       Rp, Kp);
+                       Rp, Kp, vec);
   }
   check_launch("gemm pack");
 }
@@ -826,26 +849,43 @@ __global__ void __launch_bounds__(256) k_skinny_rowdot(SkinnyArgs a, int vec) {
   if (lane < a.Q) skinny_store(a, p, lane, mine);
 }
 
-// partial sums part[chunk][q][p] over k in [chunk*kchunk, (chunk+1)*kchunk)
-template <int QB>
+// partial sums part[chunk][q][p] over k in [chunk*kchunk, (chunk+1)*kchunk);
+// VW consecutive p per thread (VW = 4: 16-byte loads, P % 4 == 0 and X rows
+// 16-byte aligned)
+template <int QB, int VW>
 __global__ void __launch_bounds__(256)
 k_skinny_colwalk(SkinnyArgs a, uint64_t kchunk, float* part) {
-  const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  const uint64_t p = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * VW;
   const uint64_t chunk = blockIdx.y;
   if (p >= a.P) return;
   const uint64_t k0 = chunk * kchunk, k1 = min(a.K, k0 + kchunk);
-  float acc[QB];
+  float acc[QB][VW];
 #pragma unroll
-  for (int q = 0; q < QB; ++q) acc[q] = 0.f;
+  for (int q = 0; q < QB; ++q)
+#pragma unroll
+    for (int w = 0; w < VW; ++w) acc[q][w] = 0.f;
   for (uint64_t k = k0; k < k1; ++k) {
-    const float x = a.X[k * a.sxk + p];
+    float x[VW];
+    if constexpr (VW == 4) {
+      const float4 t = *reinterpret_cast<const float4*>(a.X + k * a.sxk + p);
+      x[0] = t.x; x[1] = t.y; x[2] = t.z; x[3] = t.w;
+    } else {
+      x[0] = a.X[k * a.sxk + p];
+    }
 #pragma unroll
-    for (int q = 0; q < QB; ++q)
-      if (q < a.Q) acc[q] += x * a.Y[k * a.syk + q * a.syq];
+    for (int q = 0; q < QB; ++q) {
+      if (q < a.Q) {
+        const float y = a.Y[k * a.syk + q * a.syq];
+#pragma unroll
+        for (int w = 0; w < VW; ++w) acc[q][w] += x[w] * y;
+      }
+    }
   }
 #pragma unroll
   for (int q = 0; q < QB; ++q)
-    if (q < a.Q) part[(chunk * a.Q + q) * a.P + p] = acc[q];
+    if (q < a.Q)
+#pragma unroll
+      for (int w = 0; w < VW; ++w) part[(chunk * a.Q + q) * a.P + p + w] = acc[q][w];
 }
 
 __global__ void __launch_bounds__(256)
@@ -855,8 +895,18 @@ k_skinny_combine(SkinnyArgs a, const float* part, uint64_t nchunk) {
   for (uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += stride) {
     const int q = (int)(t / a.P);
     const uint64_t p = t - (uint64_t)q * a.P;
+    const uint64_t step = (uint64_t)a.Q * a.P;
+    const float* pp = part + (uint64_t)q * a.P + p;
     float s = 0.f;
-    for (uint64_t c = 0; c < nchunk; ++c) s += part[(c * a.Q + q) * a.P + p];
+    uint64_t c = 0;
+    for (; c + 8 <= nchunk; c += 8) {  // eight loads in flight, summed in chunk order
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = pp[(c + u) * step];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; c < nchunk; ++c) s += pp[c * step];
     skinny_store(a, p, q, s);
   }
 }
@@ -907,23 +957,31 @@ static void gemm_skinny(vkh_gpu* g, bool ta, bool tb, const GemmArgs& e) {
   }
   // column walk with k chunks: about 1024 threads per compute unit, at
   // least 64 k per chunk
+  const bool v4 = a.P % 4 == 0 && ((uintptr_t)a.X & 15u) == 0 && a.sxk % 4 == 0;
+  const uint64_t pthreads = v4 ? a.P / 4 : a.P;
   const uint64_t target = (uint64_t)(g->num_cus > 0 ? g->num_cus : 256) * 1024;
-  uint64_t nchunk = std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(target, a.P), ceil_div(a.K, 64)));
+  uint64_t nchunk = std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(target, pthreads), ceil_div(a.K, 64)));
   nchunk = std::min<uint64_t>(nchunk, 65535);
   const uint64_t kchunk = ceil_div(std::max<uint64_t>(a.K, 1), nchunk);
   nchunk = ceil_div(std::max<uint64_t>(a.K, 1), kchunk);
   Workspace ws(g, sizeof(float) * nchunk * (uint64_t)a.Q * a.P);
   float* part = (float*)ws.get();
-  const uint64_t bx = ceil_div(a.P, 256);
+  const uint64_t bx = ceil_div(pthreads, 256);
   VKH_REQUIRE(bx < (uint64_t(1) << 31), "GEMM too large");
   dim3 grid((unsigned)bx, (unsigned)nchunk), block(256);
+#define VKH_COLWALK(QB)                                                                                 \
+  do {                                                                                                  \
+    if (v4) hipLaunchKernelGGL((k_skinny_colwalk<QB, 4>), grid, block, 0, g->stream, a, kchunk, part); \
+    else hipLaunchKernelGGL((k_skinny_colwalk<QB, 1>), grid, block, 0, g->stream, a, kchunk, part);    \
+  } while (0)
   switch (qb) {
-    case 1: hipLaunchKernelGGL(k_skinny_colwalk<1>, grid, block, 0, g->stream, a, kchunk, part); break;
-    case 2: hipLaunchKernelGGL(k_skinny_colwalk<2>, grid, block, 0, g->stream, a, kchunk, part); break;
-    case 4: hipLaunchKernelGGL(k_skinny_colwalk<4>, grid, block, 0, g->stream, a, kchunk, part); break;
-    case 8: hipLaunchKernelGGL(k_skinny_colwalk<8>, grid, block, 0, g->stream, a, kchunk, part); break;
-    default: hipLaunchKernelGGL(k_skinny_colwalk<16>, grid, block, 0, g->stream, a, kchunk, part); break;
+    case 1: VKH_COLWALK(1); break;
+    case 2: VKH_COLWALK(2); break;
+    case 4: VKH_COLWALK(4); break;
+    case 8: VKH_COLWALK(8); break;
+    default: VKH_COLWALK(16); break;
   }
+#undef VKH_COLWALK
   check_launch("gemm (skinny column walk)");
   const uint64_t total = a.P * (uint64_t)a.Q;
   hipLaunchKernelGGL(k_skinny_combine, dim3(full_grid(total, 256)), dim3(256), 0, g->stream, a,

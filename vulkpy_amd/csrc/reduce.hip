@@ -208,7 +208,15 @@ k_combine_chunks(const float* part, float* out, uint64_t m, uint64_t nchunk, flo
   const uint64_t stride = (uint64_t)gridDim.x * 256;
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += stride) {
     float acc = part[i];
-    for (uint64_t c = 1; c < nchunk; ++c) acc = Red<OP, float>::f(acc, part[c * m + i]);
+    uint64_t c = 1;
+    for (; c + 8 <= nchunk; c += 8) {  // eight loads in flight, combined in chunk order
+      float v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = part[(c + q) * m + i];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) acc = Red<OP, float>::f(acc, v[q]);
+    }
+    for (; c < nchunk; ++c) acc = Red<OP, float>::f(acc, part[c * m + i]);
     out[i] = epilogue(acc, mul, dv);
   }
 }
