@@ -145,10 +145,12 @@ def mlp_compare(vk, gpu, steps: int, warmup: int = 2):
     on the same batch: the plain FP32 step (nn.Sequence, one launch per op,
     FP32 matrix cores) and the step captured as one hipGraph
     (nn.GraphSequence) with BF16 and with split-BF16 (3-product) matrix-core
-    GEMMs, FP32 master weights and Adam in both.  steps/s is timed over the
-    steps after the first `warmup` (the graph arms capture on their first
-    step).  Besides the final training loss, each arm's trained weights are
-    evaluated in FP32 on a held-out batch of the same distribution.
+    GEMMs, FP32 master weights and Adam in both, and kernel configurations
+    autotuned per product shape before the capture.  steps/s is timed over
+    the steps after the first `warmup` (the graph arms tune and capture on
+    their first step).  Besides the final training loss, each arm's trained
+    weights are evaluated in FP32 on a held-out batch of the same
+    distribution.
     """
     from vulkpy_amd import nn
     B, D, H = 16384, 256, 4096
@@ -200,9 +202,11 @@ def mlp_compare(vk, gpu, steps: int, warmup: int = 2):
 
     arms = (("fp32", lambda: nn.Sequence(layers("fp32"), nn.HuberLoss(), regularize=True)),
             ("bf16_graph", lambda: nn.GraphSequence(layers("bf16"), nn.HuberLoss(),
-                                                    precision="bf16", regularize=True)),
+                                                    precision="bf16", regularize=True,
+                                                    autotune=True)),
             ("bf16x3_graph", lambda: nn.GraphSequence(layers("bf16x3"), nn.HuberLoss(),
-                                                      precision="bf16x3", regularize=True)))
+                                                      precision="bf16x3", regularize=True,
+                                                      autotune=True)))
     res = {}
     for name, model in arms:
         log(f"bench: MLP {name} {steps} steps")
@@ -262,6 +266,9 @@ def gemm_report(vk, gpu, timer, reps: int = 3, sample_every: int = 128):
             c64 = a64 @ b64
             den = np.abs(a64) @ np.abs(b64)
             for mode in ("fp32", "bf16", "bf16x3"):
+                tuned = None
+                if mode != "fp32":
+                    tuned = vk.autotune_gemm(gpu, M, Nn, K, precision=mode)
                 C = A.matmul(Bm, precision=mode)
                 gpu.wait()
                 for _ in range(reps):
@@ -279,6 +286,8 @@ def gemm_report(vk, gpu, timer, reps: int = 3, sample_every: int = 128):
                     "rel_err_max": float(f"{rel.max():.3e}"),
                     "rel_err_mean": float(f"{rel.mean():.3e}"),
                     "normwise_err": float(f"{np.linalg.norm(c - c64) / np.linalg.norm(c64):.3e}"),
+                    "config": (vk.gemm_configs(mode)[tuned["best"]] if tuned else None),
+                    "autotune_ms": tuned["ms"] if tuned else None,
                 })
                 log(f"bench: GEMM {out[-1]}")
                 del C

@@ -242,6 +242,31 @@ int vkh_gemm(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
  * capture).  Refused during a capture.  No counterpart in the reference. */
 int vkh_gemm_reserve(vkh_gpu* gpu, int mode, uint64_t M, uint64_t N,
                      uint64_t K);
+
+/* ---- GEMM autotuning (no counterpart in the reference) -------------------
+ * The BF16 and split-BF16 products have several kernel configurations (tile
+ * BM x BN, k slice BK, waves per block); FP32 has one.  A context keeps one
+ * chosen configuration per (mode, trans_a, trans_b, M, N, K); without one a
+ * shape uses the built-in default.  Products with min(M, N) <= 16 use the
+ * skinny kernels and no configuration. */
+int vkh_gemm_configs(int mode, int* count);
+int vkh_gemm_config_info(int mode, int index, int* bm, int* bn, int* bk,
+                         int* waves);
+/* the configuration a shape uses; *tuned = 1 if chosen, 0 for the default */
+int vkh_gemm_get_config(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
+                        uint64_t M, uint64_t N, uint64_t K, int* index,
+                        int* tuned);
+/* choose a configuration for a shape (index < 0 restores the default) */
+int vkh_gemm_set_config(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
+                        uint64_t M, uint64_t N, uint64_t K, int index);
+/* times every configuration on the shape (scratch operands from the
+ * context's allocator, one warm-up and `reps` timed runs each, operand
+ * packing included), keeps the fastest for the shape and returns its index;
+ * ms_per_config (optional, vkh_gemm_configs entries) receives each mean time
+ * in milliseconds.  Synchronises the stream; refused during graph capture. */
+int vkh_gemm_autotune(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
+                      uint64_t M, uint64_t N, uint64_t K, int reps, int* best,
+                      float* ms_per_config);
 /* vkh_gemm whose epilogue finally multiplies by act'(y) with y = aux[m, n]
  * (aux_act: VKH_ACT_RELU -> max(sign(y), 0), VKH_ACT_SIGMOID -> (1-y)*y):
  * the backward of an activation fused into the GEMM that produces its input

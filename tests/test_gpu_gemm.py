@@ -128,3 +128,40 @@ def test_skinny_epilogue(gpu, M, N, K, ta, tb):
               gX._ptr, N, _lib.ACT["relu"])
     gC._set_job([gA, gB, gX])
     np.testing.assert_allclose(np.asarray(gC), ref * (aux > 0), rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("mode", ["bf16", "bf16x3"])
+@pytest.mark.parametrize("ta,tb", [(0, 1), (1, 0)])
+def test_configurations_bit_identical(gpu, mode, ta, tb):
+    """Every kernel configuration accumulates each output over k in the
+    same order, so tuning never changes a result."""
+    M, N, K = 300, 260, 200
+    rng = np.random.default_rng(9)
+    A = rng.standard_normal((K, M) if ta else (M, K)).astype(np.float32)
+    B = rng.standard_normal((N, K) if tb else (K, N)).astype(np.float32)
+    cfgs = vk.gemm_configs(mode)
+    assert len(cfgs) >= 2
+    outs = []
+    for i in range(len(cfgs)):
+        _lib.call("vkh_gemm_set_config", gpu._handle, _lib.GEMM_MODE[mode], ta, tb, M, N, K, i)
+        outs.append(run_gemm(gpu, mode, ta, tb, A, B))
+    _lib.call("vkh_gemm_set_config", gpu._handle, _lib.GEMM_MODE[mode], ta, tb, M, N, K, -1)
+    for i, o in enumerate(outs[1:], 1):
+        np.testing.assert_array_equal(o, outs[0], err_msg=f"configuration {i} {cfgs[i]}")
+    ref, bound = ref_gemm(ta, tb, A, B)
+    assert (np.abs(outs[0] - ref) / bound).max() <= TOL[mode]
+
+
+def test_autotune_api(gpu):
+    import ctypes
+    r = vk.autotune_gemm(gpu, 512, 384, 256, precision="bf16", trans_b=True, reps=2)
+    assert 0 <= r["best"] < len(vk.gemm_configs("bf16"))
+    assert all(t > 0 for t in r["ms"]) and r["ms"][r["best"]] == min(r["ms"])
+    idx, tuned = ctypes.c_int(), ctypes.c_int()
+    _lib.call("vkh_gemm_get_config", gpu._handle, _lib.GEMM_MODE["bf16"], 0, 1, 512, 384, 256,
+              ctypes.byref(idx), ctypes.byref(tuned))
+    assert (idx.value, tuned.value) == (r["best"], 1)
+    # skinny and FP32 shapes have a single kernel choice
+    assert vk.autotune_gemm(gpu, 4096, 1, 512, precision="bf16")["ms"][0] == -1
+    with pytest.raises(ValueError):
+        _lib.call("vkh_gemm_set_config", gpu._handle, _lib.GEMM_MODE["bf16"], 0, 0, 8, 8, 8, 99)

@@ -123,3 +123,26 @@ def test_graph_rejects_unsupported(gpu):
     net = nn.GraphSequence(make_net(gpu, [(4, 3)], [None], 1), nn.MSELoss(), precision="fp32")
     with pytest.raises(ValueError):
         net.train(vk.Array(gpu, shape=(8, 5)), vk.Array(gpu, shape=(8, 3)))
+
+
+def test_graph_autotuned_matches_eager(gpu):
+    """autotune=True picks kernel configurations per product shape; results
+    stay identical to the eager step (configurations never change results)."""
+    dims, acts = [(64, 256), (256, 256), (256, 4)], ["relu", "relu", None]
+    B = 512
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal((B, 64)).astype(np.float32)
+    y = rng.standard_normal((B, 4)).astype(np.float32)
+    X, Y = vk.Array(gpu, data=x), vk.Array(gpu, data=y)
+    eager_layers = make_net(gpu, dims, acts, 60, lambda: nn.Ridge(1e-4), precision="bf16")
+    graph_layers = make_net(gpu, dims, acts, 60, lambda: nn.Ridge(1e-4), precision="bf16")
+    eager = nn.Sequence(eager_layers, nn.HuberLoss(), regularize=True)
+    graph = nn.GraphSequence(graph_layers, nn.HuberLoss(), precision="bf16", regularize=True,
+                             autotune=True)
+    for step in range(3):
+        _, le = eager.train(X, Y)
+        _, lg = graph.train(X, Y)
+        gpu.wait()
+        assert np.asarray(le).item() == np.asarray(lg).item(), step
+        for a, b in zip(weights(eager_layers), weights(graph_layers)):
+            np.testing.assert_array_equal(a, b)

@@ -17,7 +17,8 @@ Examples
 [4. 5. 6.]
 """
 from .vkarray import (GPU, U32Array, Shape, Array, zeros,
-                      set_matmul_precision, get_matmul_precision)
+                      set_matmul_precision, get_matmul_precision,
+                      gemm_configs, autotune_gemm)
 from . import random
 from . import nn
 from . import util

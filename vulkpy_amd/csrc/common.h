@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdint>
+#include <map>
 #include <cstring>
 #include <mutex>
 #include <stdexcept>
@@ -112,6 +114,9 @@ struct vkh_gpu {
   // from graph_pack_hint (vkh_gemm_reserve) when the capture begins
   void* graph_pack = nullptr;
   size_t graph_pack_bytes = 0, graph_pack_hint = 0;
+  // GEMM kernel configuration per (mode, trans_a, trans_b, M, N, K), chosen
+  // by vkh_gemm_autotune or vkh_gemm_set_config (gemm.hip)
+  std::map<std::array<uint64_t, 6>, int> gemm_cfg;
 };
 
 struct vkh_event {

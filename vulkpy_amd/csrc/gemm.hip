@@ -434,10 +434,11 @@ struct PackedArgs {
 // K-contiguous BF16 rows (both parts in split mode), stored to LDS rows of
 // LDK = BK + 8 elements (the pad keeps rows 16-byte aligned and spreads the
 // rows of a 16-lane LDS access over distinct banks).
-template <int ROWS, int BK, bool SPLIT>
+template <int ROWS, int BK, bool SPLIT, int THREADS>
 struct PackedStager {
   static constexpr int LDK = BK + 8;
-  static constexpr int NV = ROWS * BK / 8 / GEMM_THREADS;  // 16-byte vectors per thread
+  static constexpr int NV = ROWS * BK / 8 / THREADS;  // 16-byte vectors per thread
+  static_assert(ROWS * BK / 8 % THREADS == 0, "whole 16-byte vectors per thread");
   static constexpr int PARTS = SPLIT ? 2 : 1;
   u32x4_t r[PARTS][NV];
 
@@ -445,7 +446,7 @@ struct PackedStager {
                                        uint64_t Kp, uint64_t k0) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
-      const int idx = threadIdx.x + i * GEMM_THREADS;
+      const int idx = threadIdx.x + i * THREADS;
       const int row = idx / (BK / 8), c = idx % (BK / 8);
       const uint64_t off = (row0 + row) * Kp + k0 + c * 8;
       r[0][i] = *reinterpret_cast<const u32x4_t*>(hi + off);
@@ -455,7 +456,7 @@ struct PackedStager {
   __device__ __forceinline__ void store(uint16_t* hi, uint16_t* lo) const {
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
-      const int idx = threadIdx.x + i * GEMM_THREADS;
+      const int idx = threadIdx.x + i * THREADS;
       const int row = idx / (BK / 8), c = idx % (BK / 8);
       *reinterpret_cast<u32x4_t*>(hi + row * LDK + c * 8) = r[0][i];
       if constexpr (SPLIT) *reinterpret_cast<u32x4_t*>(lo + row * LDK + c * 8) = r[1][i];
@@ -528,22 +529,24 @@ __device__ __forceinline__ void epilogue4(const GemmArgs& e, uint64_t m, uint64_
   }
 }
 
-// BM x BN tile, 4 waves (2 x 2) of (BM/2) x (BN/2), double-buffered LDS and
-// register prefetch of the next BK-deep K slice; one barrier per slice.
+// BM x BN tile, WGM x WGN waves of (BM/WGM) x (BN/WGN), double-buffered LDS
+// and register prefetch of the next BK-deep K slice; one barrier per slice.
 // After the K loop the LDS holds each wave's accumulators, 32 rows at a time,
 // so the epilogue reads and writes whole row segments of C.
-template <int BM, int BN, int BK, bool SPLIT>
-__global__ void __launch_bounds__(GEMM_THREADS)
+template <int BM, int BN, int BK, bool SPLIT, int WGM, int WGN>
+__global__ void __launch_bounds__(64 * WGM * WGN)
 k_gemm_packed(PackedArgs p) {
   static_assert(PK_KALIGN % BK == 0, "Kp must be a multiple of the k slice");
-  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int THREADS = 64 * WGM * WGN, WAVES = WGM * WGN;
+  constexpr int WM = BM / WGM, WN = BN / WGN;
+  static_assert(WM % 32 == 0 && WN % 32 == 0 && WN <= 256, "32 x 32 MFMA blocks per wave");
   constexpr int TM = WM / 32, TN = WN / 32;
   constexpr int PARTS = SPLIT ? 2 : 1;
   constexpr int PK_LDK = BK + 8;
   constexpr int A_TILE = BM * PK_LDK, B_TILE = BN * PK_LDK;        // BF16 elements
   constexpr int MAIN_BYTES = 2 * PARTS * (A_TILE + B_TILE) * 2;
   constexpr int SROW = WN + 4;                                      // staging row (floats)
-  constexpr int STAGE_BYTES = 4 * 32 * SROW * 4;
+  constexpr int STAGE_BYTES = WAVES * 32 * SROW * 4;
   constexpr int SMEM_BYTES = MAIN_BYTES > STAGE_BYTES ? MAIN_BYTES : STAGE_BYTES;
   __shared__ __attribute__((aligned(16))) uint16_t smem[SMEM_BYTES / 2];
   // operand tile (buffer, part): As(buf, q) / Bs(buf, q)
@@ -555,12 +558,12 @@ k_gemm_packed(PackedArgs p) {
   tile_coords(e, tm, tn);
   const uint64_t m0 = (uint64_t)tm * BM, n0 = (uint64_t)tn * BN;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / WGN, wn = wave % WGN;
   const int l32 = lane & 31, lhi = lane >> 5;
   const uint64_t Kp = p.Kp;
 
-  PackedStager<BM, BK, SPLIT> sa;
-  PackedStager<BN, BK, SPLIT> sb;
+  PackedStager<BM, BK, SPLIT, THREADS> sa;
+  PackedStager<BN, BK, SPLIT, THREADS> sb;
 
   f32x16_t acc[TM][TN];
 #pragma unroll
@@ -577,7 +580,7 @@ k_gemm_packed(PackedArgs p) {
   static_assert(KS % 2 == 0, "the first k-step of every slice uses fragment buffer 0");
   constexpr int NM = TM * TN * (SPLIT ? 3 : 1);  // MFMAs per k-step
   constexpr int NDS = (TM + TN) * PARTS;         // fragment reads per k-step
-  constexpr int NVM = (PackedStager<BM, BK, SPLIT>::NV + PackedStager<BN, BK, SPLIT>::NV) * PARTS;
+  constexpr int NVM = (PackedStager<BM, BK, SPLIT, THREADS>::NV + PackedStager<BN, BK, SPLIT, THREADS>::NV) * PARTS;
   constexpr int VPP = (NVM + NDS - 1) / NDS;     // global loads per MFMA of the first step
   constexpr int WPM = (NVM + NM - 1) / NM;       // LDS stores per MFMA of the last step
   bf16x8_t fa[2][TM], fb[2][TN], fal[2][TM], fbl[2][TN];
@@ -684,17 +687,68 @@ k_gemm_packed(PackedArgs p) {
 // Tile choice of the packed kernel: 256 x 256 when both sides fill at least
 // one such tile (4 waves of 128 x 128), 128 x 128 otherwise and in split mode
 // (whose two BF16 parts double the LDS footprint).
-struct PackedShape {
-  int bm, bn;
+// Kernel configurations of the packed path: tile, k slice and wave grid.
+// BF16 keeps one part per operand; split mode (two parts) needs twice the
+// LDS per slice, hence its smaller k slice.
+typedef void (*PackedLaunch)(dim3, hipStream_t, const PackedArgs&);
+template <int BM, int BN, int BK, bool SPLIT, int WGM, int WGN>
+static void launch_packed(dim3 grid, hipStream_t s, const PackedArgs& pa) {
+  hipLaunchKernelGGL((k_gemm_packed<BM, BN, BK, SPLIT, WGM, WGN>), grid, dim3(64 * WGM * WGN), 0, s, pa);
+}
+struct PackedCfg {
+  int bm, bn, bk, waves;
+  PackedLaunch launch;
 };
-static PackedShape packed_shape(int mode, uint64_t M, uint64_t N) {
-  if (mode == VKH_GEMM_BF16 && M >= 256 && N >= 256) return {256, 256};
-  return {128, 128};
+static const PackedCfg BF16_CFGS[] = {
+    {256, 256, 64, 4, launch_packed<256, 256, 64, false, 2, 2>},
+    {256, 256, 64, 8, launch_packed<256, 256, 64, false, 2, 4>},
+    {128, 128, 64, 4, launch_packed<128, 128, 64, false, 2, 2>},
+    {256, 128, 64, 4, launch_packed<256, 128, 64, false, 2, 2>},
+    {128, 256, 64, 4, launch_packed<128, 256, 64, false, 2, 2>},
+};
+static const PackedCfg SPLIT_CFGS[] = {
+    {128, 128, 32, 4, launch_packed<128, 128, 32, true, 2, 2>},
+    {256, 128, 32, 8, launch_packed<256, 128, 32, true, 4, 2>},
+    {128, 128, 32, 8, launch_packed<128, 128, 32, true, 2, 4>},
+};
+constexpr int PK_MAXTILE = 256;  // largest bm / bn of any configuration
+
+static const PackedCfg* packed_cfgs(int mode, int& count) {
+  if (mode == VKH_GEMM_BF16) { count = (int)(sizeof(BF16_CFGS) / sizeof(BF16_CFGS[0])); return BF16_CFGS; }
+  count = (int)(sizeof(SPLIT_CFGS) / sizeof(SPLIT_CFGS[0]));
+  return SPLIT_CFGS;
 }
 
+// Configuration without tuning, from measurements on MI355X: BF16 uses the
+// 8-wave 256 x 256 tile when both sides fill it (8192^3: 1.24 ms against
+// 1.43 ms for 4 waves; 65536 x 1024 x 1024: 0.294 against 0.348 ms), split
+// mode the 8-wave 256 x 128 tile for long K (8192^3: 3.55 against 4.18 ms)
+// and 128 x 128 otherwise (K = 1024: 0.59 against 0.74 ms).
+static int default_cfg(int mode, uint64_t M, uint64_t N, uint64_t K) {
+  if (mode == VKH_GEMM_BF16) return (M >= 256 && N >= 256) ? 1 : 2;
+  return (M >= 256 && N >= 128 && K >= 4096) ? 1 : 0;
+}
+
+static std::array<uint64_t, 6> cfg_key(int mode, int ta, int tb, uint64_t M, uint64_t N, uint64_t K) {
+  return {(uint64_t)mode, (uint64_t)(ta != 0), (uint64_t)(tb != 0), M, N, K};
+}
+
+// Fills p[0..n) with values in [-0.5, 0.5) from a hash of the index (operands
+// of the autotuning runs: data-dependent power draw makes zero operands an
+// unrepresentative benchmark).
+__global__ void __launch_bounds__(256) k_fill_noise(float* p, uint64_t n, uint32_t seed) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+    uint32_t h = (uint32_t)i * 0x9e3779b1u ^ (uint32_t)(i >> 32) ^ seed;
+    h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
+    p[i] = (float)(h >> 8) * (1.0f / 16777216.0f) - 0.5f;
+  }
+}
+
+// Packing area of an M x N x K product: rows padded to PK_MAXTILE, which
+// covers every configuration (so a reservation never depends on tuning).
 static size_t packed_bytes(int mode, uint64_t M, uint64_t N, uint64_t K) {
-  const PackedShape s = packed_shape(mode, M, N);
-  const uint64_t Mp = ceil_div(M, s.bm) * s.bm, Np = ceil_div(N, s.bn) * s.bn;
+  const uint64_t Mp = ceil_div(M, PK_MAXTILE) * PK_MAXTILE, Np = ceil_div(N, PK_MAXTILE) * PK_MAXTILE;
   const uint64_t Kp = ceil_div(std::max<uint64_t>(K, 1), PK_KALIGN) * PK_KALIGN;
   const uint64_t parts = mode == VKH_GEMM_BF16X3 ? 2 : 1;
   return (size_t)((Mp + Np) * Kp * 2 * parts);
@@ -742,9 +796,12 @@ static void pack_operand(vkh_gpu* g, const float* src, uint64_t ld, bool trans, 
   check_launch("gemm pack");
 }
 
-static void gemm_packed(vkh_gpu* g, int mode, bool ta, bool tb, GemmArgs& a) {
+static void gemm_packed(vkh_gpu* g, int mode, bool ta, bool tb, GemmArgs& a, int cfg) {
   const bool split = mode == VKH_GEMM_BF16X3;
-  const PackedShape s = packed_shape(mode, a.M, a.N);
+  int ncfg = 0;
+  const PackedCfg* cfgs = packed_cfgs(mode, ncfg);
+  VKH_REQUIRE(cfg >= 0 && cfg < ncfg, "gemm: configuration index out of range");
+  const PackedCfg& s = cfgs[cfg];
   const uint64_t Mp = ceil_div(a.M, s.bm) * s.bm, Np = ceil_div(a.N, s.bn) * s.bn;
   const uint64_t Kp = ceil_div(std::max<uint64_t>(a.K, 1), PK_KALIGN) * PK_KALIGN;
   uint16_t* base = pack_area(g, packed_bytes(mode, a.M, a.N, a.K));
@@ -765,10 +822,7 @@ static void gemm_packed(vkh_gpu* g, int mode, bool ta, bool tb, GemmArgs& a) {
   const uint64_t nb = (uint64_t)a.tiles_m * a.tiles_n;
   VKH_REQUIRE(nb < (uint64_t(1) << 31), "GEMM too large");
   PackedArgs pa{Ah, Al, Bh, Bl, Kp, a};
-  dim3 grid((unsigned)nb), block(GEMM_THREADS);
-  if (s.bm == 256) hipLaunchKernelGGL((k_gemm_packed<256, 256, 64, false>), grid, block, 0, g->stream, pa);
-  else if (split) hipLaunchKernelGGL((k_gemm_packed<128, 128, 32, true>), grid, block, 0, g->stream, pa);
-  else hipLaunchKernelGGL((k_gemm_packed<128, 128, 64, false>), grid, block, 0, g->stream, pa);
+  s.launch(dim3((unsigned)nb), g->stream, pa);
   check_launch("gemm (packed)");
 }
 
@@ -1033,9 +1087,131 @@ static void gemm_impl(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M
   switch (mode) {
     case VKH_GEMM_FP32: launch_mode<VKH_GEMM_FP32>(g, trans_a, trans_b, a, va, vb); break;
     case VKH_GEMM_BF16:
-    case VKH_GEMM_BF16X3: gemm_packed(g, mode, trans_a, trans_b, a); break;
+    case VKH_GEMM_BF16X3: {
+      auto it = g->gemm_cfg.find(cfg_key(mode, trans_a, trans_b, M, N, K));
+      gemm_packed(g, mode, trans_a, trans_b, a, it != g->gemm_cfg.end() ? it->second : default_cfg(mode, M, N, K));
+      break;
+    }
     default: fail(VKH_INVALID, "gemm: unknown mode");
   }
+}
+
+int vkh_gemm_configs(int mode, int* count) {
+  return guarded([&] {
+    VKH_REQUIRE(count, "null out");
+    VKH_REQUIRE(mode >= VKH_GEMM_FP32 && mode <= VKH_GEMM_BF16X3, "unknown mode");
+    if (mode == VKH_GEMM_FP32) { *count = 1; return; }
+    packed_cfgs(mode, *count);
+  });
+}
+
+int vkh_gemm_config_info(int mode, int index, int* bm, int* bn, int* bk, int* waves) {
+  return guarded([&] {
+    VKH_REQUIRE(bm && bn && bk && waves, "null out");
+    VKH_REQUIRE(mode >= VKH_GEMM_FP32 && mode <= VKH_GEMM_BF16X3, "unknown mode");
+    if (mode == VKH_GEMM_FP32) {
+      VKH_REQUIRE(index == 0, "configuration index out of range");
+      *bm = Cfg<VKH_GEMM_FP32>::BM; *bn = Cfg<VKH_GEMM_FP32>::BN; *bk = Cfg<VKH_GEMM_FP32>::BK; *waves = 4;
+      return;
+    }
+    int n = 0;
+    const PackedCfg* c = packed_cfgs(mode, n);
+    VKH_REQUIRE(index >= 0 && index < n, "configuration index out of range");
+    *bm = c[index].bm; *bn = c[index].bn; *bk = c[index].bk; *waves = c[index].waves;
+  });
+}
+
+int vkh_gemm_get_config(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M, uint64_t N,
+                        uint64_t K, int* index, int* tuned) {
+  return guarded([&] {
+    VKH_REQUIRE(g && index, "null argument");
+    VKH_REQUIRE(mode >= VKH_GEMM_FP32 && mode <= VKH_GEMM_BF16X3, "unknown mode");
+    auto it = g->gemm_cfg.find(cfg_key(mode, trans_a, trans_b, M, N, K));
+    const bool have = it != g->gemm_cfg.end();
+    *index = have ? it->second : (mode == VKH_GEMM_FP32 ? 0 : default_cfg(mode, M, N, K));
+    if (tuned) *tuned = have ? 1 : 0;
+  });
+}
+
+int vkh_gemm_set_config(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M, uint64_t N,
+                        uint64_t K, int index) {
+  return guarded([&] {
+    VKH_REQUIRE(g, "null GPU handle");
+    VKH_REQUIRE(mode >= VKH_GEMM_FP32 && mode <= VKH_GEMM_BF16X3, "unknown mode");
+    int n = 1;
+    if (mode != VKH_GEMM_FP32) packed_cfgs(mode, n);
+    const auto key = cfg_key(mode, trans_a, trans_b, M, N, K);
+    if (index < 0) { g->gemm_cfg.erase(key); return; }
+    VKH_REQUIRE(index < n, "configuration index out of range");
+    g->gemm_cfg[key] = index;
+  });
+}
+
+int vkh_gemm_autotune(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M, uint64_t N,
+                      uint64_t K, int reps, int* best, float* ms_per_config) {
+  return guarded([&] {
+    use(g);
+    VKH_REQUIRE(mode >= VKH_GEMM_FP32 && mode <= VKH_GEMM_BF16X3, "unknown mode");
+    VKH_REQUIRE(!g->capturing, "gemm_autotune: not allowed during graph capture");
+    VKH_REQUIRE(M > 0 && N > 0 && K > 0, "gemm_autotune: empty product");
+    VKH_REQUIRE(g->alloc, "gemm_autotune needs a context with its own allocator");
+    if (reps < 1) reps = 3;
+    int n = 1;
+    if (mode != VKH_GEMM_FP32) packed_cfgs(mode, n);
+    const bool skinny = std::min(M, N) <= SKINNY_MAX;
+    if (mode == VKH_GEMM_FP32 || skinny) {  // a single kernel choice
+      if (ms_per_config)
+        for (int i = 0; i < n; ++i) ms_per_config[i] = -1.0f;
+      if (best) *best = mode == VKH_GEMM_FP32 ? 0 : default_cfg(mode, M, N, K);
+      return;
+    }
+    // scratch operands in the product's own layout
+    const uint64_t na = M * K, nbv = K * N, nc = M * N;
+    Workspace wa(g, sizeof(float) * na), wb(g, sizeof(float) * nbv), wc(g, sizeof(float) * nc);
+    float* A = (float*)wa.get();
+    float* B = (float*)wb.get();
+    float* C = (float*)wc.get();
+    hipLaunchKernelGGL(k_fill_noise, dim3(full_grid(na, 256)), dim3(256), 0, g->stream, A, na, 1u);
+    hipLaunchKernelGGL(k_fill_noise, dim3(full_grid(nbv, 256)), dim3(256), 0, g->stream, B, nbv, 2u);
+    check_launch("gemm_autotune fill");
+    GemmArgs a{};
+    a.A = A; a.B = B; a.C = C;
+    a.M = M; a.N = N; a.K = K;
+    a.lda = trans_a ? M : K; a.ldb = trans_b ? K : N; a.ldc = N;
+    a.alpha = 1.0f; a.beta = 0.0f; a.act = VKH_ACT_NONE; a.aux_act = VKH_ACT_NONE;
+    a.vec_c = 1;
+    hipEvent_t e0, e1;
+    VKH_HIP(hipEventCreate(&e0));
+    VKH_HIP(hipEventCreate(&e1));
+    int bi = -1;
+    float bt = 0.0f;
+    try {
+      for (int i = 0; i < n; ++i) {
+        GemmArgs w = a;
+        gemm_packed(g, mode, trans_a, trans_b, w, i);  // warm-up (and packing-area growth)
+        VKH_HIP(hipEventRecord(e0, g->stream));
+        for (int r = 0; r < reps; ++r) {
+          GemmArgs t = a;
+          gemm_packed(g, mode, trans_a, trans_b, t, i);
+        }
+        VKH_HIP(hipEventRecord(e1, g->stream));
+        VKH_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        VKH_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms /= (float)reps;
+        if (ms_per_config) ms_per_config[i] = ms;
+        if (bi < 0 || ms < bt) { bi = i; bt = ms; }
+      }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g->gemm_cfg[cfg_key(mode, trans_a, trans_b, M, N, K)] = bi;
+    if (best) *best = bi;
+  });
 }
 
 int vkh_gemm_reserve(vkh_gpu* g, int mode, uint64_t M, uint64_t N, uint64_t K) {

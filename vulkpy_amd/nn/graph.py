@@ -206,6 +206,16 @@ class _Plan:
                 call("vkh_adagrad", h, p.value._ptr, p.grad._ptr, st.h._ptr, n,
                      float(st.opt.lr), float(st.opt.eps))
 
+    def _gemm_shapes(self):
+        """(trans_a, trans_b, M, N, K) of every product _enqueue issues"""
+        B = self.B
+        for i, u in enumerate(self.units):
+            d = u.dense
+            yield 0, 1, B, d.output_dim, d.input_dim      # forward x W^T
+            yield 1, 0, d.output_dim, d.input_dim, B      # dW = dY^T X
+            if i > 0:
+                yield 0, 0, B, d.input_dim, d.output_dim  # dX = dY W
+
     def _enqueue_reg_grad(self, p: Parameter):
         """grad = R.grad(value) (regularizers.py:23-192), before dW / db are added"""
         h, n, R = self.h, p.value.size, p.R
@@ -236,14 +246,12 @@ class _Plan:
             for p in (u.dense.w, u.dense.b):
                 if p.grad is not None and self.seq.regularize and isinstance(p.R, Elastic):
                     self._scratch(p)
-        # size the graph's GEMM packing region for every product of the step
-        B = self.B
-        for i, u in enumerate(self.units):
-            d = u.dense
-            call("vkh_gemm_reserve", self.h, self.mode, B, d.output_dim, d.input_dim)
-            call("vkh_gemm_reserve", self.h, self.mode, d.output_dim, d.input_dim, B)
-            if i > 0:
-                call("vkh_gemm_reserve", self.h, self.mode, B, d.input_dim, d.output_dim)
+        # kernel configuration (when tuning) and the graph's GEMM packing
+        # region for every product of the step
+        for ta, tb, M, N, K in self._gemm_shapes():
+            if self.seq.autotune and self.mode != GEMM_MODE["fp32"]:
+                call("vkh_gemm_autotune", self.h, self.mode, ta, tb, M, N, K, 3, None, None)
+            call("vkh_gemm_reserve", self.h, self.mode, M, N, K)
         self.gpu.wait()
         call("vkh_graph_begin", self.h)
         try:
@@ -321,6 +329,10 @@ class GraphSequence:
         Add each parameter's regularizer gradient before the update
         (``Parameter.regular_grad``).  The reference's ``Sequence.train``
         never does (vulkpy/nn/models.py:55-78).
+    autotune : bool
+        Before capturing, time every GEMM kernel configuration on each
+        product shape of the step and use the fastest
+        (``vulkpy_amd.autotune_gemm``).  BF16 modes only.
 
     Notes
     -----
@@ -329,13 +341,14 @@ class GraphSequence:
     """
 
     def __init__(self, layers: Iterable[Module], loss: Loss, *,
-                 precision: str = "bf16", regularize: bool = True):
+                 precision: str = "bf16", regularize: bool = True, autotune: bool = False):
         if precision not in GEMM_MODE:
             raise ValueError(f"unknown precision {precision!r}")
         self.L: Tuple[Module, ...] = tuple(layers)
         self.loss = loss
         self.precision = precision
         self.regularize = regularize
+        self.autotune = autotune
         self._plan: Optional[_Plan] = None
         units = _units(self.L)
         self.gpu = units[0].dense.w.value._gpu

@@ -1071,3 +1071,40 @@ def set_matmul_precision(precision: str):
 
 def get_matmul_precision() -> str:
     return _matmul_precision
+
+
+def gemm_configs(precision: str) -> List[dict]:
+    """Kernel configurations of the matrix products in a precision mode:
+    tile ``bm`` x ``bn``, k slice ``bk`` and waves per block."""
+    mode = GEMM_MODE[precision]
+    n = ctypes.c_int()
+    call("vkh_gemm_configs", mode, ctypes.byref(n))
+    out = []
+    for i in range(n.value):
+        v = [ctypes.c_int() for _ in range(4)]
+        call("vkh_gemm_config_info", mode, i, *[ctypes.byref(x) for x in v])
+        out.append({"bm": v[0].value, "bn": v[1].value, "bk": v[2].value, "waves": v[3].value})
+    return out
+
+
+def autotune_gemm(gpu: GPU, M: int, N: int, K: int, *, precision: str = "bf16",
+                  trans_a: bool = False, trans_b: bool = False, reps: int = 3) -> dict:
+    """
+    Time every kernel configuration of an ``op(A)[M,K] @ op(B)[K,N]`` product
+    on ``gpu`` and keep the fastest for that shape (later products of the
+    same shape, layout and precision use it).
+
+    Returns
+    -------
+    dict
+        ``best`` (index into :func:`gemm_configs`) and ``ms`` (mean time of
+        each configuration, operand packing included; -1 where the shape
+        has a single kernel choice).
+    """
+    mode = GEMM_MODE[precision]
+    n = len(gemm_configs(precision))
+    ms = (ctypes.c_float * n)()
+    best = ctypes.c_int()
+    call("vkh_gemm_autotune", gpu._handle, mode, int(trans_a), int(trans_b), int(M), int(N), int(K),
+         int(reps), ctypes.byref(best), ms)
+    return {"best": best.value, "ms": [round(float(x), 5) for x in ms]}
