@@ -96,6 +96,64 @@ k_ew2d_vec(F f, Opnd a, Opnd b, Opnd c, float* out, uint32_t total4, FastDiv col
   }
 }
 
+// The same traversal with the operand kinds fixed at compile time.  With the
+// kinds known, the loads of all operands of a unit issue back to back; the
+// run-time switch of k_ew2d_vec splits the code into branches whose merge
+// points wait for each load before the next is issued.
+template <int KIND>
+__device__ __forceinline__ float4 load4k(const Opnd& o, uint32_t t, uint32_t row, uint32_t col4) {
+  if constexpr (KIND == K_FULL) return ld_stream(reinterpret_cast<const float4*>(o.p) + t);
+  else if constexpr (KIND == K_ROW) return reinterpret_cast<const float4*>(o.p)[col4];
+  else if constexpr (KIND == K_COL) { const float x = o.p[row]; return make_float4(x, x, x, x); }
+  else if constexpr (KIND == K_FIRST) { const float x = o.p[0]; return make_float4(x, x, x, x); }
+  else return make_float4(o.s, o.s, o.s, o.s);
+}
+
+template <class F, int KA, int KB, int KC>
+__global__ void __launch_bounds__(EW_BLOCK)
+k_ew2d_kind(F f, Opnd a, Opnd b, Opnd c, float* out, uint32_t total4, FastDiv cols4) {
+  constexpr bool RC = KA == K_ROW || KA == K_COL || KB == K_ROW || KB == K_COL || KC == K_ROW || KC == K_COL;
+  const uint32_t stride = gridDim.x * EW_BLOCK;
+  for (uint32_t t = blockIdx.x * EW_BLOCK + threadIdx.x; t < total4; t += stride) {
+    uint32_t row = 0, col4 = t;
+    if constexpr (RC) { row = cols4.div(t); col4 = t - row * cols4.d; }
+    const float4 va = load4k<KA>(a, t, row, col4);
+    const float4 vb = F::nin > 1 ? load4k<KB>(b, t, row, col4) : va;
+    const float4 vc = F::nin > 2 ? load4k<KC>(c, t, row, col4) : va;
+    st_stream(reinterpret_cast<float4*>(out) + t, apply4(f, va, vb, vc));
+  }
+}
+
+// Launches k_ew2d_kind when the operand kinds are one of the streaming
+// combinations (at least one full-size operand); returns false otherwise.
+template <class F>
+static bool launch_kind(dim3 grid, hipStream_t s, const F& f, const Opnd* op, float* out, uint32_t total4,
+                        FastDiv fd) {
+  const int ka = op[0].kind, kb = op[1].kind, kc = op[2].kind;
+#define VKH_KIND(A, B, C)                                                                       \
+  if (ka == A && kb == B && kc == C) {                                                          \
+    hipLaunchKernelGGL((k_ew2d_kind<F, A, B, C>), grid, dim3(EW_BLOCK), 0, s, f, op[0], op[1], \
+                       op[2], out, total4, fd);                                                 \
+    return true;                                                                                \
+  }
+  if constexpr (F::nin == 1) {
+    VKH_KIND(K_FULL, K_SCALAR, K_SCALAR)
+  } else if constexpr (F::nin == 2) {
+    VKH_KIND(K_FULL, K_FULL, K_SCALAR)
+    VKH_KIND(K_FULL, K_COL, K_SCALAR)
+    VKH_KIND(K_FULL, K_ROW, K_SCALAR)
+    VKH_KIND(K_FULL, K_SCALAR, K_SCALAR)
+    VKH_KIND(K_COL, K_FULL, K_SCALAR)
+    VKH_KIND(K_ROW, K_FULL, K_SCALAR)
+    VKH_KIND(K_SCALAR, K_FULL, K_SCALAR)
+  } else {
+    VKH_KIND(K_FULL, K_SCALAR, K_SCALAR)
+    VKH_KIND(K_FULL, K_FULL, K_FULL)
+  }
+#undef VKH_KIND
+  return false;
+}
+
 // Same traversal with 64-bit indices for 2^31 float4 units or more.
 template <class F>
 __global__ void __launch_bounds__(EW_BLOCK)
@@ -253,7 +311,9 @@ static void run_ew(vkh_gpu* g, const F& f, const std::vector<InSpec>& ins,
       if (total4 < (uint64_t(1) << 31)) {
         unsigned grid = full_grid(total4, EW_BLOCK);
         FastDiv fd((uint32_t)cols4);
-        if (need_rc)
+        if (launch_kind(dim3(grid), g->stream, f, op, out, (uint32_t)total4, fd))
+          ;  // specialised kernel
+        else if (need_rc)
           hipLaunchKernelGGL((k_ew2d_vec<F, true>), dim3(grid), dim3(EW_BLOCK), 0, g->stream,
                              f, op[0], op[1], op[2], out, (uint32_t)total4, fd);
         else
