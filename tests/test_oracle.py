@@ -374,3 +374,16 @@ def test_fastdiv_header_on_host(tmp_path):
                     "-o", str(exe)], check=True, capture_output=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
     assert out[0] == "0" and int(out[1]) > 10 ** 8
+
+
+def test_sincos_header_on_host(tmp_path):
+    # vulkpy_amd/csrc/bmath.h gives the fused normal kernel its sin / cos; it
+    # is checked at every angle the generator can produce (2^23 of them)
+    # against FP64 sin / cos: within 2 ulp of the correctly rounded value
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = tmp_path / "sincos_check"
+    subprocess.run(["hipcc", "-O2", "-std=c++17", "-ffp-contract=off",
+                    os.path.join(root, "tests", "sincos_check.cpp"), "-o", str(exe)],
+                   check=True, capture_output=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    assert float(out[0]) <= 2.0 and float(out[1]) <= 2.0 and int(out[2]) == 1 << 23, out
