@@ -249,6 +249,7 @@ def gemm_report(vk, gpu, timer, reps: int = 3, sample_every: int = 128):
     and ||C - C64||_F / ||C64||_F.
     """
     out = []
+    timer.mean_ms()  # start from an empty list of event pairs
     rng = np.random.default_rng(3)
     for (M, Nn, K) in ((8192, 8192, 8192), (65536, 1024, 1024)):
         for dist in ("normal", "range"):
@@ -293,6 +294,59 @@ def gemm_report(vk, gpu, timer, reps: int = 3, sample_every: int = 128):
                 del C
             del A, Bm
             gpu.wait()
+    return out
+
+
+def transfer_report(vk, gpu, nbytes: int = 1 << 29, reps: int = 3) -> dict:
+    """
+    Host <-> device rates (GB/s, PCIe included) for a 512 MiB float32 array:
+    DMA from / to its page-locked host mirror, the same copies from / to
+    ordinary (pageable) numpy memory for comparison, and the API paths end to
+    end: ``Array(gpu, data=x)`` (copy into the mirror + upload, mirror block
+    reused from the pinned pool) and ``np.asarray`` of a freshly computed
+    array (download).
+    """
+    from vulkpy_amd import _lib
+    n = nbytes // 4
+    host = np.random.default_rng(4).standard_normal(n, dtype=np.float32)
+    pageable = np.empty(n, dtype=np.float32)
+    a = vk.Array(gpu, data=host)
+    gpu.wait()
+    live = ctypes.c_size_t()
+    _lib.call("vkh_host_stats", ctypes.byref(live), None)
+    pinned = live.value >= nbytes
+
+    def rate(fn):
+        fn()
+        gpu.wait()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        gpu.wait()
+        return round(nbytes * reps / (time.perf_counter() - t) / 1e9, 2)
+
+    h = gpu._handle
+    out = {
+        "bytes": nbytes,
+        "mirror_pinned": pinned,
+        "h2d_pinned": rate(lambda: _lib.call("vkh_upload", h, a._ptr, a._host.ctypes.data, nbytes)),
+        "h2d_pageable": rate(lambda: _lib.call("vkh_upload", h, a._ptr, host.ctypes.data, nbytes)),
+        "d2h_pinned": rate(lambda: _lib.call("vkh_download", h, a._host.ctypes.data, a._ptr, nbytes)),
+        "d2h_pageable": rate(lambda: _lib.call("vkh_download", h, pageable.ctypes.data, a._ptr, nbytes)),
+    }
+
+    def create():
+        b = vk.Array(gpu, data=host)
+        b.wait()
+
+    def readback():
+        c = a * 1.0
+        np.asarray(c)
+
+    out["array_from_numpy"] = rate(create)
+    out["numpy_from_array"] = rate(readback)
+    del a
+    gpu.wait()
     return out
 
 
@@ -374,6 +428,9 @@ def main() -> int:
     if not args.no_gemm:
         log("bench: GEMM report")
         gemm = gemm_report(vk, gpu, timer)
+    log("bench: host <-> device transfers")
+    transfer = transfer_report(vk, gpu)
+    log(f"bench: transfer {transfer}")
 
     cpu = None
     if c2 is not None and not args.no_cpu:
@@ -419,8 +476,10 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "per_op_GBps": c2["per_op"] if c2 else None,
+        "reduce_tuning": c2["reduce_tuning"] if c2 else None,
         "mlp": mlp,
         "gemm": gemm,
+        "transfer": transfer,
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -479,6 +538,16 @@ def c2_suite(vk, gpu, args, timer):
     fbuf = vk.Array(gpu, shape=(PRNG_N,))
     gpu.wait()
 
+    # reduction kernel configurations for the suite's three reduction shapes
+    # (outside the timed region, like the GEMM tuning)
+    log("bench: tuning reductions")
+    reduce_tuning = {}
+    for name, axis in (("axis0", 0), ("axis1", 1), ("all", None)):
+        r = vk.autotune_reduce(gpu, A.shape, axis)[0]
+        reduce_tuning[name] = {"family": r["family"], "config": r["config"],
+                               "ms_default": r["ms"][0], "ms_best": r["ms"][r["best"]]}
+        log(f"bench: reduce {name} {r}")
+
     suite = build_suite(vk, gpu, A, B, col, row, P, gen, ubuf, fbuf, timer)
     step_bytes = sum(nb for _, nb, _ in suite)
 
@@ -505,20 +574,26 @@ def c2_suite(vk, gpu, args, timer):
     dom_bytes = 8 * N + 4 * M_ROWS
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
 
-    # per-op GB/s from one extra, individually synchronised pass (reported only)
+    # per-op GB/s (reported only): HIP events around each op on the library's
+    # stream, mean of 3 launches, so host latency is not counted
     per_op = {}
+    op_timer = Timer(gpu)
     for name, nb, f in suite:
         gpu.wait()
-        t = time.perf_counter()
-        f()
-        gpu.wait()
-        per_op[name] = round(nb / (time.perf_counter() - t) / 1e9, 1)
+        for _ in range(3):
+            t = op_timer.start()
+            f()
+            op_timer.stop(t)
+        per_op[name] = round(nb / (op_timer.mean_ms() * 1e-3) / 1e9, 1)
+    gpu.wait()
+    timer.mean_ms()  # drop the dominant op's pairs from this pass
 
     del suite, ubuf, fbuf, gen, A, B, P  # free the suite's device buffers
     gpu.wait()
     return {
         "value": value, "ms_per_step": dt / args.steps * 1e3, "n_ops": n_ops,
         "step_bytes": step_bytes, "achieved": achieved, "dom_ms": dom_ms, "per_op": per_op,
+        "reduce_tuning": reduce_tuning,
         "host": {"a": a_host, "col": col_host, "row": row_host, "p": p_host},
     }
 

@@ -56,6 +56,14 @@ enum vkh_unop {
 /* reductions: vulkpy/shader/{sum,prod,maximum,minimum}*.comp */
 enum vkh_redop { VKH_RSUM = 0, VKH_RPROD = 1, VKH_RMAX = 2, VKH_RMIN = 3 };
 
+/* reduction kernel families that have configurations (vkh_reduce_autotune) */
+enum vkh_red_family {
+  VKH_RFAM_NONE = -1, /* a single kernel choice (rows shorter than 2048)  */
+  VKH_RFAM_ALL = 0,   /* full reduction                                   */
+  VKH_RFAM_ROWS = 1,  /* axis with post == 1 and rows of >= 2048 elements */
+  VKH_RFAM_COLS = 2   /* strided axis, post > 1                           */
+};
+
 /* GEMM arithmetic modes (precision of the products; accumulation is FP32) */
 enum vkh_gemm_mode {
   VKH_GEMM_FP32 = 0,   /* FP32 MFMA                                   */
@@ -116,6 +124,17 @@ int vkh_upload(vkh_gpu* gpu, void* dst, const void* src, size_t nbytes);
 /* device -> host copy ordered after all work on the stream; returns once the
  * data is in host memory (replaces reading the mapped pointer, _vkarray.cc:107) */
 int vkh_download(vkh_gpu* gpu, void* dst, const void* src, size_t nbytes);
+/* Page-locked host blocks for the host mirrors of arrays, so that vkh_upload /
+ * vkh_download move them by DMA without the runtime's staging copies.  The
+ * reference's buffers are host-visible mappings instead (_vkarray.cc:61-63),
+ * read and written in place.  Freed blocks are cached by size class (up to
+ * 4 GiB); live pinned memory is capped at 32 GiB, beyond which vkh_host_alloc
+ * fails with VKH_OUT_OF_MEMORY and callers use ordinary host memory.
+ * Not tied to a device. */
+int vkh_host_alloc(size_t nbytes, void** out);
+int vkh_host_free(void* p);
+int vkh_host_empty_cache(void);
+int vkh_host_stats(size_t* live_bytes, size_t* cached_bytes);
 /* device -> device copy on the stream */
 int vkh_copy(vkh_gpu* gpu, void* dst, const void* src, size_t nbytes);
 /* dst[0..n) = value (vkarray.py:1541 `z[:] = 0.0`, initializers.py:48) */
@@ -181,6 +200,39 @@ int vkh_reduce_all(vkh_gpu* gpu, int op, const float* a, uint64_t n,
 int vkh_reduce_axis(vkh_gpu* gpu, int op, const float* a, float* out,
                     uint64_t prev, uint64_t axis, uint64_t post,
                     int rebroadcast, float scale_mul, float scale_div);
+
+/* Reduction kernel configurations.  No counterpart in the reference, whose
+ * shaders have fixed 64-thread workgroups (sum_axis.comp:3).  kind 0 is a
+ * full reduction of prev * axis * post elements, kind 1 an axis reduction of
+ * [prev, axis, post].  A configuration is (unroll, par): unroll = 128-bit
+ * loads in flight per thread (4 or 8), par = blocks per CU (families ALL and
+ * ROWS) or target threads per CU (COLS: sets how many chunks the axis is
+ * split into).  Configurations of the ROWS family and the unroll of every
+ * family never change results.  The COLS chunk count fixes the summation
+ * order of sums and products, and the ALL blocks per CU the partition of the
+ * FP64 partial sums; every configuration keeps the stated error bounds, and
+ * the chosen one is used by every later call of the shape on the context,
+ * eager or captured.  Index 0 is each family's default. */
+int vkh_reduce_family(int kind, uint64_t prev, uint64_t axis, uint64_t post,
+                      int* family);
+int vkh_reduce_configs(int family, int* count);
+int vkh_reduce_config_info(int family, int index, int* unroll, int* par);
+/* the configuration a shape uses; *tuned = 1 if chosen, 0 for the default */
+int vkh_reduce_get_config(vkh_gpu* gpu, int kind, uint64_t prev,
+                          uint64_t axis, uint64_t post, int* index,
+                          int* tuned);
+/* choose a configuration for a shape (index < 0 restores the default) */
+int vkh_reduce_set_config(vkh_gpu* gpu, int kind, uint64_t prev,
+                          uint64_t axis, uint64_t post, int index);
+/* times every configuration of the shape's family on scratch data (a sum;
+ * one warm-up and `reps` timed runs each), keeps the fastest for the shape
+ * and returns its index; ms_per_config (optional, vkh_reduce_configs
+ * entries) receives each mean time in milliseconds (-1 for a family with a
+ * single kernel choice).  Synchronises the stream; refused during graph
+ * capture. */
+int vkh_reduce_autotune(vkh_gpu* gpu, int kind, uint64_t prev, uint64_t axis,
+                        uint64_t post, int reps, int* best,
+                        float* ms_per_config);
 
 /* ---- gather (F17) -------------------------------------------------------- */
 /* c[i] = a[min(idx[i], n_a - 1)]: gather.comp; caller vkarray.py:1493-1501 */

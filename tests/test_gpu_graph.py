@@ -2,11 +2,13 @@
 Sequence.train path: identical weights in FP32 after every replay (with eager
 work and synchronisations between replays), and a BF16 run that tracks the
 FP32 loss."""
+import ctypes
+
 import numpy as np
 import pytest
 
 import vulkpy_amd as vk
-from vulkpy_amd import nn
+from vulkpy_amd import _lib, nn
 
 pytestmark = pytest.mark.gpu
 
@@ -126,8 +128,11 @@ def test_graph_rejects_unsupported(gpu):
 
 
 def test_graph_autotuned_matches_eager(gpu):
-    """autotune=True picks kernel configurations per product shape; results
-    stay identical to the eager step (configurations never change results)."""
+    """autotune=True picks GEMM configurations per product shape (they never
+    change results) and reduction configurations per bias-gradient shape
+    (kept on the context, so the eager step uses them too): results stay
+    identical to the eager step.  The graph's first step, which tunes, runs
+    before the eager one."""
     dims, acts = [(64, 256), (256, 256), (256, 4)], ["relu", "relu", None]
     B = 512
     rng = np.random.default_rng(12)
@@ -139,10 +144,21 @@ def test_graph_autotuned_matches_eager(gpu):
     eager = nn.Sequence(eager_layers, nn.HuberLoss(), regularize=True)
     graph = nn.GraphSequence(graph_layers, nn.HuberLoss(), precision="bf16", regularize=True,
                              autotune=True)
-    for step in range(3):
-        _, le = eager.train(X, Y)
-        _, lg = graph.train(X, Y)
-        gpu.wait()
-        assert np.asarray(le).item() == np.asarray(lg).item(), step
-        for a, b in zip(weights(eager_layers), weights(graph_layers)):
-            np.testing.assert_array_equal(a, b)
+    try:
+        for step in range(3):
+            _, lg = graph.train(X, Y)
+            _, le = eager.train(X, Y)
+            gpu.wait()
+            assert np.asarray(le).item() == np.asarray(lg).item(), step
+            for a, b in zip(weights(eager_layers), weights(graph_layers)):
+                np.testing.assert_array_equal(a, b)
+        # the step's reductions were tuned too (bias gradients: [B, 256], [B, 4])
+        tuned = ctypes.c_int()
+        idx = ctypes.c_int()
+        _lib.call("vkh_reduce_get_config", gpu._handle, 1, 1, B, 256, ctypes.byref(idx),
+                  ctypes.byref(tuned))
+        assert tuned.value == 1
+    finally:
+        # reduction choices stay on the context: restore the defaults
+        for d in (1, 4, 256):
+            _lib.call("vkh_reduce_set_config", gpu._handle, 1, 1, B, d, -1)

@@ -18,7 +18,7 @@ Examples
 """
 from .vkarray import (GPU, U32Array, Shape, Array, zeros,
                       set_matmul_precision, get_matmul_precision,
-                      gemm_configs, autotune_gemm)
+                      gemm_configs, autotune_gemm, reduce_configs, autotune_reduce)
 from . import random
 from . import nn
 from . import util

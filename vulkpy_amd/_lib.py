@@ -67,6 +67,10 @@ _SIGNATURES = {
     "vkh_memory_stats": (c_int, [P, POINTER(c_size_t), POINTER(c_size_t)]),
     "vkh_upload": (c_int, [P, P, P, c_size_t]),
     "vkh_download": (c_int, [P, P, P, c_size_t]),
+    "vkh_host_alloc": (c_int, [c_size_t, POINTER(P)]),
+    "vkh_host_free": (c_int, [P]),
+    "vkh_host_empty_cache": (c_int, []),
+    "vkh_host_stats": (c_int, [POINTER(c_size_t), POINTER(c_size_t)]),
     "vkh_copy": (c_int, [P, P, P, c_size_t]),
     "vkh_fill_f32": (c_int, [P, P, c_float, c_size_t]),
     "vkh_fill_u32": (c_int, [P, P, c_uint32, c_size_t]),
@@ -84,6 +88,12 @@ _SIGNATURES = {
     "vkh_broadcast_to": (c_int, [P, P, P, c_int, _u64p, _u64p]),
     "vkh_reduce_all": (c_int, [P, c_int, P, U64, P, c_int]),
     "vkh_reduce_axis": (c_int, [P, c_int, P, P, U64, U64, U64, c_int, c_float, c_float]),
+    "vkh_reduce_family": (c_int, [c_int, U64, U64, U64, POINTER(c_int)]),
+    "vkh_reduce_configs": (c_int, [c_int, POINTER(c_int)]),
+    "vkh_reduce_config_info": (c_int, [c_int, c_int, POINTER(c_int), POINTER(c_int)]),
+    "vkh_reduce_get_config": (c_int, [P, c_int, U64, U64, U64, POINTER(c_int), POINTER(c_int)]),
+    "vkh_reduce_set_config": (c_int, [P, c_int, U64, U64, U64, c_int]),
+    "vkh_reduce_autotune": (c_int, [P, c_int, U64, U64, U64, c_int, POINTER(c_int), POINTER(c_float)]),
     "vkh_gather": (c_int, [P, P, U64, P, P, U64]),
     "vkh_gather_axis": (c_int, [P, P, P, P, U64, U64, U64, U64]),
     "vkh_xoshiro_create": (c_int, [P, c_uint32, U64, POINTER(P)]),

@@ -117,6 +117,9 @@ struct vkh_gpu {
   // GEMM kernel configuration per (mode, trans_a, trans_b, M, N, K), chosen
   // by vkh_gemm_autotune or vkh_gemm_set_config (gemm.hip)
   std::map<std::array<uint64_t, 6>, int> gemm_cfg;
+  // reduction kernel configuration per (family, prev, axis, post), chosen by
+  // vkh_reduce_autotune or vkh_reduce_set_config (reduce.hip)
+  std::map<std::array<uint64_t, 4>, int> red_cfg;
 };
 
 struct vkh_event {
@@ -192,6 +195,10 @@ template <typename T>
 __device__ __forceinline__ void st_stream1(T* p, T v) {
   __builtin_nontemporal_store(v, p);
 }
+
+// Fills p[0..n) with hashed values in [-0.5, 0.5): the scratch operands of
+// the autotuning runs (gemm.hip, block of 256 threads).
+__global__ void __launch_bounds__(256) k_fill_noise(float* p, uint64_t n, uint32_t seed);
 
 // Grid for a one-work-item-per-thread kernel (with a grid-stride loop as a
 // guard for very large arrays).

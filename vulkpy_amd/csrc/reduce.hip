@@ -71,7 +71,9 @@ __device__ __forceinline__ float epilogue(float v, float mul, float dv) {
 constexpr int ROW_BLOCK = 256;
 
 // one block per row (long rows)
-template <int OP, bool VEC>
+// U: 128-bit loads in flight per thread (4 or 8); each thread still combines
+// its elements in index order, so U never changes the result
+template <int OP, bool VEC, int U>
 __global__ void __launch_bounds__(ROW_BLOCK)
 k_rows_block(const float* a, float* out, uint64_t rows, uint64_t L, int rebroadcast, float mul, float dv) {
   __shared__ float lds[ROW_BLOCK / 64];
@@ -83,14 +85,13 @@ k_rows_block(const float* a, float* out, uint64_t rows, uint64_t L, int rebroadc
       const float4* row4 = reinterpret_cast<const float4*>(row);
       const uint64_t L4 = L / 4;
       uint64_t i = threadIdx.x;
-      // four loads in flight, combined in the same order as the plain loop
-      for (; i + 3 * ROW_BLOCK < L4; i += 4 * ROW_BLOCK) {
-        const float4 v0 = ld_stream(row4 + i), v1 = ld_stream(row4 + i + ROW_BLOCK);
-        const float4 v2 = ld_stream(row4 + i + 2 * ROW_BLOCK), v3 = ld_stream(row4 + i + 3 * ROW_BLOCK);
-        acc = R::f(acc, R::f(R::f(v0.x, v0.y), R::f(v0.z, v0.w)));
-        acc = R::f(acc, R::f(R::f(v1.x, v1.y), R::f(v1.z, v1.w)));
-        acc = R::f(acc, R::f(R::f(v2.x, v2.y), R::f(v2.z, v2.w)));
-        acc = R::f(acc, R::f(R::f(v3.x, v3.y), R::f(v3.z, v3.w)));
+      // U loads in flight, combined in the same order as the plain loop
+      for (; i + (U - 1) * ROW_BLOCK < L4; i += U * ROW_BLOCK) {
+        float4 v[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) v[q] = ld_stream(row4 + i + q * ROW_BLOCK);
+#pragma unroll
+        for (int q = 0; q < U; ++q) acc = R::f(acc, R::f(R::f(v[q].x, v[q].y), R::f(v[q].z, v[q].w)));
       }
       for (; i < L4; i += ROW_BLOCK) {
         const float4 v = ld_stream(row4 + i);
@@ -151,8 +152,9 @@ k_rows_thread(const float* a, float* out, uint64_t rows, uint64_t L, int rebroad
 // ---- strided axes: post > 1 ------------------------------------------------------
 // a viewed as [prev, K, post]; thread owns VW consecutive columns j of one p
 // and reduces k in [k0, k1) sequentially.  Result goes to dst[(c * prev + p) *
-// post + j] where c is the chunk index (blockIdx.y).
-template <int OP, int VW>
+// post + j] where c is the chunk index (blockIdx.y).  U rows are loaded at a
+// time (VW == 4) and still combined in row order, so U never changes the result.
+template <int OP, int VW, int U>
 __global__ void __launch_bounds__(256)
 k_cols(const float* a, float* dst, uint64_t prev, uint64_t K, uint64_t post, uint64_t kchunk,
        int final_pass, float mul, float dv) {
@@ -175,16 +177,13 @@ k_cols(const float* a, float* dst, uint64_t prev, uint64_t K, uint64_t post, uin
       };
       float4 acc = ld_stream(reinterpret_cast<const float4*>(base + k0 * post));
       uint64_t k = k0 + 1;
-      // four rows in flight, still combined in row order
-      for (; k + 3 < k1; k += 4) {
-        const float4 v0 = ld_stream(reinterpret_cast<const float4*>(base + k * post));
-        const float4 v1 = ld_stream(reinterpret_cast<const float4*>(base + (k + 1) * post));
-        const float4 v2 = ld_stream(reinterpret_cast<const float4*>(base + (k + 2) * post));
-        const float4 v3 = ld_stream(reinterpret_cast<const float4*>(base + (k + 3) * post));
-        comb(acc, v0);
-        comb(acc, v1);
-        comb(acc, v2);
-        comb(acc, v3);
+      // U rows in flight, still combined in row order
+      for (; k + (U - 1) < k1; k += U) {
+        float4 v[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) v[q] = ld_stream(reinterpret_cast<const float4*>(base + (k + q) * post));
+#pragma unroll
+        for (int q = 0; q < U; ++q) comb(acc, v[q]);
       }
       for (; k < k1; ++k) comb(acc, ld_stream(reinterpret_cast<const float4*>(base + k * post)));
       if (final_pass) {
@@ -235,7 +234,10 @@ k_rebroadcast_cols(const float* red, float* out, uint64_t prev, uint64_t K, uint
 // ---- full reduction ----------------------------------------------------------------
 constexpr int ALL_BLOCK = 256;
 
-template <int OP>
+// U (4 or 8) 128-bit loads in flight per thread; a group of eight is combined
+// as two consecutive groups of four, so for a given grid U never changes the
+// result
+template <int OP, int U>
 __global__ void __launch_bounds__(ALL_BLOCK)
 k_all_partial(const float* a, uint64_t n, double* part) {
   __shared__ double lds[ALL_BLOCK / 64];
@@ -248,6 +250,15 @@ k_all_partial(const float* a, uint64_t n, double* part) {
     return R::f(R::f((double)v.x, (double)v.y), R::f((double)v.z, (double)v.w));
   };
   uint64_t i = (uint64_t)blockIdx.x * ALL_BLOCK + threadIdx.x;
+  if constexpr (U == 8) {
+    for (; i + 7 * stride < n4; i += 8 * stride) {
+      float4 v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = ld_stream(a4 + i + q * stride);
+      acc = R::f(acc, R::f(R::f(quad(v[0]), quad(v[1])), R::f(quad(v[2]), quad(v[3]))));
+      acc = R::f(acc, R::f(R::f(quad(v[4]), quad(v[5])), R::f(quad(v[6]), quad(v[7]))));
+    }
+  }
   for (; i + 3 * stride < n4; i += 4 * stride) {  // four loads in flight
     const float4 v0 = ld_stream(a4 + i), v1 = ld_stream(a4 + i + stride);
     const float4 v2 = ld_stream(a4 + i + 2 * stride), v3 = ld_stream(a4 + i + 3 * stride);
@@ -270,12 +281,64 @@ k_all_final(const double* part, uint64_t nb, float* out, uint64_t n, int mean) {
   if (threadIdx.x == 0) out[0] = (float)(mean ? r / (double)n : r);
 }
 
+// ---- kernel configurations (vkh_reduce_autotune) ------------------------------
+// unroll: 128-bit loads in flight per thread.  par: blocks per CU (full and row
+// reductions: the grid cap) or target threads per CU (strided axes: decides
+// how many chunks k is split into).  Index 0 of each family is the default.
+struct RedCfg {
+  int unroll, par;
+};
+static const RedCfg ALL_CFGS[] = {{4, 4}, {8, 4}, {4, 2}, {8, 2}, {4, 8}, {8, 8}, {4, 16}, {8, 16}};
+static const RedCfg ROWS_CFGS[] = {{4, 16}, {8, 16}, {4, 8}, {8, 8}, {4, 32}, {8, 32}};
+static const RedCfg COLS_CFGS[] = {{4, 512}, {8, 512}, {4, 256}, {8, 256},
+                                   {4, 1024}, {8, 1024}, {4, 2048}, {8, 2048}};
+constexpr int ROWS_BLOCK_MIN_L = 2048;  // shorter rows use the wave / thread kernels
+
+static const RedCfg* red_cfgs(int fam, int& n) {
+  switch (fam) {
+    case VKH_RFAM_ALL: n = (int)(sizeof(ALL_CFGS) / sizeof(RedCfg)); return ALL_CFGS;
+    case VKH_RFAM_ROWS: n = (int)(sizeof(ROWS_CFGS) / sizeof(RedCfg)); return ROWS_CFGS;
+    case VKH_RFAM_COLS: n = (int)(sizeof(COLS_CFGS) / sizeof(RedCfg)); return COLS_CFGS;
+  }
+  n = 1;  // a single kernel choice
+  return nullptr;
+}
+
+// family of a reduction: kind 0 = full (n = prev * K * post), 1 = axis
+static int red_family(int kind, uint64_t K, uint64_t post) {
+  if (kind == 0) return VKH_RFAM_ALL;
+  if (post > 1) return VKH_RFAM_COLS;
+  return K >= ROWS_BLOCK_MIN_L ? VKH_RFAM_ROWS : VKH_RFAM_NONE;
+}
+
+static std::array<uint64_t, 4> red_key(int fam, uint64_t prev, uint64_t K, uint64_t post) {
+  if (fam == VKH_RFAM_ALL) return {(uint64_t)fam, 1, prev * K * post, 1};
+  return {(uint64_t)fam, prev, K, post};
+}
+
+// configuration of a call: the explicit index when >= 0, else the one chosen
+// for the shape on this context, else the default
+static RedCfg red_cfg(vkh_gpu* g, int fam, uint64_t prev, uint64_t K, uint64_t post, int index) {
+  int n = 0;
+  const RedCfg* c = red_cfgs(fam, n);
+  if (!c) return RedCfg{4, 0};
+  if (index < 0) {
+    auto it = g->red_cfg.find(red_key(fam, prev, K, post));
+    index = it == g->red_cfg.end() ? 0 : it->second;
+  }
+  return c[index];
+}
+
 template <int OP>
-static void reduce_all(vkh_gpu* g, const float* a, uint64_t n, float* out, int mean) {
-  unsigned nb = stream_grid(g, ceil_div(n, 4 * 4), ALL_BLOCK, 4);
+static void reduce_all(vkh_gpu* g, const float* a, uint64_t n, float* out, int mean, int index = -1) {
+  const RedCfg c = red_cfg(g, VKH_RFAM_ALL, 1, n, 1, index);
+  unsigned nb = stream_grid(g, ceil_div(n, 4 * 4), ALL_BLOCK, (unsigned)c.par);
   Workspace ws(g, sizeof(double) * nb);
   double* part = (double*)ws.get();
-  hipLaunchKernelGGL(k_all_partial<OP>, dim3(nb), dim3(ALL_BLOCK), 0, g->stream, a, n, part);
+  if (c.unroll == 8)
+    hipLaunchKernelGGL((k_all_partial<OP, 8>), dim3(nb), dim3(ALL_BLOCK), 0, g->stream, a, n, part);
+  else
+    hipLaunchKernelGGL((k_all_partial<OP, 4>), dim3(nb), dim3(ALL_BLOCK), 0, g->stream, a, n, part);
   check_launch("reduce_all partial");
   hipLaunchKernelGGL(k_all_final<OP>, dim3(1), dim3(ALL_BLOCK), 0, g->stream, part, (uint64_t)nb, out, n, mean);
   check_launch("reduce_all final");
@@ -283,17 +346,20 @@ static void reduce_all(vkh_gpu* g, const float* a, uint64_t n, float* out, int m
 
 template <int OP>
 static void reduce_axis(vkh_gpu* g, const float* a, float* out, uint64_t prev, uint64_t K,
-                        uint64_t post, int rebroadcast, float mul, float dv) {
+                        uint64_t post, int rebroadcast, float mul, float dv, int index = -1) {
   const unsigned cus = g->num_cus > 0 ? g->num_cus : 256;
+  const RedCfg cfg = red_cfg(g, red_family(1, K, post), prev, K, post, index);
   if (post == 1) {
     const uint64_t rows = prev, L = K;
     const bool vec = (L % 4 == 0) && (((uintptr_t)a & 15u) == 0) && (((uintptr_t)out & 15u) == 0);
-    if (L >= 2048) {
-      unsigned grid = (unsigned)std::min<uint64_t>(rows, (uint64_t)cus * 16);
-      if (vec)
-        hipLaunchKernelGGL((k_rows_block<OP, true>), dim3(grid), dim3(ROW_BLOCK), 0, g->stream, a, out, rows, L, rebroadcast, mul, dv);
+    if (L >= ROWS_BLOCK_MIN_L) {
+      unsigned grid = (unsigned)std::min<uint64_t>(rows, (uint64_t)cus * cfg.par);
+      if (vec && cfg.unroll == 8)
+        hipLaunchKernelGGL((k_rows_block<OP, true, 8>), dim3(grid), dim3(ROW_BLOCK), 0, g->stream, a, out, rows, L, rebroadcast, mul, dv);
+      else if (vec)
+        hipLaunchKernelGGL((k_rows_block<OP, true, 4>), dim3(grid), dim3(ROW_BLOCK), 0, g->stream, a, out, rows, L, rebroadcast, mul, dv);
       else
-        hipLaunchKernelGGL((k_rows_block<OP, false>), dim3(grid), dim3(ROW_BLOCK), 0, g->stream, a, out, rows, L, rebroadcast, mul, dv);
+        hipLaunchKernelGGL((k_rows_block<OP, false, 4>), dim3(grid), dim3(ROW_BLOCK), 0, g->stream, a, out, rows, L, rebroadcast, mul, dv);
     } else if (L >= 64) {
       unsigned grid = (unsigned)std::min<uint64_t>(ceil_div(rows, 4), (uint64_t)cus * 16);
       hipLaunchKernelGGL(k_rows_wave<OP>, dim3(grid), dim3(256), 0, g->stream, a, out, rows, L, rebroadcast, mul, dv);
@@ -307,10 +373,12 @@ static void reduce_axis(vkh_gpu* g, const float* a, float* out, uint64_t prev, u
   const bool vec = (post % 4 == 0) && (((uintptr_t)a & 15u) == 0) && (((uintptr_t)out & 15u) == 0);
   const int VW = vec ? 4 : 1;
   const uint64_t threads = prev * (post / VW);
-  // Split k when there are fewer than ~2 waves of columns per CU and the axis
-  // is long enough to give every chunk a reasonable amount of work (measured
-  // on 8192 x 16384: 32 chunks of 256 rows reach 6.4 TB/s, 64 chunks 6.1).
-  const uint64_t want = (uint64_t)cus * 512;
+  // Split k when there are fewer than cfg.par threads of columns per CU and
+  // the axis is long enough to give every chunk a reasonable amount of work
+  // (default 512: on 8192 x 16384, 32 chunks of 256 rows reach 6.4 TB/s, 64
+  // chunks 6.1).  The chunk count fixes the summation order of sums and
+  // products, so it is part of the shape's configuration.
+  const uint64_t want = (uint64_t)cus * (uint64_t)cfg.par;
   uint64_t nchunk = 1;
   if (threads < want && K >= 64) {
     nchunk = std::min<uint64_t>(ceil_div(want, threads), K / 32);
@@ -327,10 +395,12 @@ static void reduce_axis(vkh_gpu* g, const float* a, float* out, uint64_t prev, u
   if (gx < 1) gx = 1;
   dim3 grid(gx, (unsigned)nchunk);
   const int final_pass = (nchunk == 1) ? 1 : 0;
-  if (VW == 4)
-    hipLaunchKernelGGL((k_cols<OP, 4>), grid, dim3(256), 0, g->stream, a, dst, prev, K, post, kchunk, final_pass, mul, dv);
+  if (VW == 4 && cfg.unroll == 8)
+    hipLaunchKernelGGL((k_cols<OP, 4, 8>), grid, dim3(256), 0, g->stream, a, dst, prev, K, post, kchunk, final_pass, mul, dv);
+  else if (VW == 4)
+    hipLaunchKernelGGL((k_cols<OP, 4, 4>), grid, dim3(256), 0, g->stream, a, dst, prev, K, post, kchunk, final_pass, mul, dv);
   else
-    hipLaunchKernelGGL((k_cols<OP, 1>), grid, dim3(256), 0, g->stream, a, dst, prev, K, post, kchunk, final_pass, mul, dv);
+    hipLaunchKernelGGL((k_cols<OP, 1, 4>), grid, dim3(256), 0, g->stream, a, dst, prev, K, post, kchunk, final_pass, mul, dv);
   check_launch("reduce cols");
   float* red = dst;
   if (nchunk > 1) {
@@ -411,6 +481,123 @@ int vkh_reduce_axis(vkh_gpu* g, int op, const float* a, float* out, uint64_t pre
       case VKH_RMIN: reduce_axis<VKH_RMIN>(g, a, out, prev, axis, post, rebroadcast, mul, dv); break;
       default: fail(VKH_INVALID, "unknown reduction op");
     }
+  });
+}
+
+int vkh_reduce_family(int kind, uint64_t prev, uint64_t axis, uint64_t post, int* family) {
+  return guarded([&] {
+    VKH_REQUIRE(family, "null out");
+    VKH_REQUIRE(kind == 0 || kind == 1, "reduction kind must be 0 (full) or 1 (axis)");
+    (void)prev;
+    *family = red_family(kind, axis, post);
+  });
+}
+
+int vkh_reduce_configs(int family, int* count) {
+  return guarded([&] {
+    VKH_REQUIRE(count, "null out");
+    VKH_REQUIRE(family >= VKH_RFAM_NONE && family <= VKH_RFAM_COLS, "unknown reduction family");
+    red_cfgs(family, *count);
+  });
+}
+
+int vkh_reduce_config_info(int family, int index, int* unroll, int* par) {
+  return guarded([&] {
+    VKH_REQUIRE(unroll && par, "null out");
+    VKH_REQUIRE(family >= VKH_RFAM_NONE && family <= VKH_RFAM_COLS, "unknown reduction family");
+    int n = 0;
+    const RedCfg* c = red_cfgs(family, n);
+    VKH_REQUIRE(index >= 0 && index < n, "configuration index out of range");
+    *unroll = c ? c[index].unroll : 0;
+    *par = c ? c[index].par : 0;
+  });
+}
+
+int vkh_reduce_get_config(vkh_gpu* g, int kind, uint64_t prev, uint64_t axis, uint64_t post,
+                          int* index, int* tuned) {
+  return guarded([&] {
+    VKH_REQUIRE(g && index, "null argument");
+    VKH_REQUIRE(kind == 0 || kind == 1, "reduction kind must be 0 (full) or 1 (axis)");
+    const int fam = red_family(kind, axis, post);
+    auto it = g->red_cfg.find(red_key(fam, prev, axis, post));
+    const bool have = it != g->red_cfg.end();
+    *index = have ? it->second : 0;
+    if (tuned) *tuned = have ? 1 : 0;
+  });
+}
+
+int vkh_reduce_set_config(vkh_gpu* g, int kind, uint64_t prev, uint64_t axis, uint64_t post,
+                          int index) {
+  return guarded([&] {
+    VKH_REQUIRE(g, "null GPU handle");
+    VKH_REQUIRE(kind == 0 || kind == 1, "reduction kind must be 0 (full) or 1 (axis)");
+    const int fam = red_family(kind, axis, post);
+    const auto key = red_key(fam, prev, axis, post);
+    if (index < 0) { g->red_cfg.erase(key); return; }
+    int n = 0;
+    red_cfgs(fam, n);
+    VKH_REQUIRE(index < n, "configuration index out of range");
+    if (fam == VKH_RFAM_NONE) return;  // a single kernel choice: nothing to keep
+    g->red_cfg[key] = index;
+  });
+}
+
+int vkh_reduce_autotune(vkh_gpu* g, int kind, uint64_t prev, uint64_t axis, uint64_t post,
+                        int reps, int* best, float* ms_per_config) {
+  return guarded([&] {
+    use(g);
+    VKH_REQUIRE(kind == 0 || kind == 1, "reduction kind must be 0 (full) or 1 (axis)");
+    VKH_REQUIRE(!g->capturing, "reduce_autotune: not allowed during graph capture");
+    VKH_REQUIRE(prev > 0 && axis > 0 && post > 0, "reduce_autotune: empty reduction");
+    VKH_REQUIRE(g->alloc, "reduce_autotune needs a context with its own allocator");
+    if (reps < 1) reps = 3;
+    const int fam = red_family(kind, axis, post);
+    int n = 0;
+    red_cfgs(fam, n);
+    if (fam == VKH_RFAM_NONE) {  // a single kernel choice
+      if (ms_per_config) ms_per_config[0] = -1.0f;
+      if (best) *best = 0;
+      return;
+    }
+    // scratch input and output of the reduction's own shape
+    const uint64_t total = prev * axis * post;
+    const uint64_t nout = kind == 0 ? 1 : prev * post;
+    Workspace win(g, sizeof(float) * total), wout(g, sizeof(float) * nout);
+    float* A = (float*)win.get();
+    float* O = (float*)wout.get();
+    hipLaunchKernelGGL(k_fill_noise, dim3(full_grid(total, 256)), dim3(256), 0, g->stream, A, total, 5u);
+    check_launch("reduce_autotune fill");
+    auto run = [&](int i) {
+      if (kind == 0) reduce_all<VKH_RSUM>(g, A, total, O, 0, i);
+      else reduce_axis<VKH_RSUM>(g, A, O, prev, axis, post, 0, 1.0f, 1.0f, i);
+    };
+    hipEvent_t e0, e1;
+    VKH_HIP(hipEventCreate(&e0));
+    VKH_HIP(hipEventCreate(&e1));
+    int bi = -1;
+    float bt = 0.0f;
+    try {
+      for (int i = 0; i < n; ++i) {
+        run(i);  // warm-up
+        VKH_HIP(hipEventRecord(e0, g->stream));
+        for (int r = 0; r < reps; ++r) run(i);
+        VKH_HIP(hipEventRecord(e1, g->stream));
+        VKH_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        VKH_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms /= (float)reps;
+        if (ms_per_config) ms_per_config[i] = ms;
+        if (bi < 0 || ms < bt) { bi = i; bt = ms; }
+      }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g->red_cfg[red_key(fam, prev, axis, post)] = bi;
+    if (best) *best = bi;
   });
 }
 

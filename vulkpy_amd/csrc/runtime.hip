@@ -140,6 +140,34 @@ static void init_props(vkh_gpu* g) {
   g->wave_size = prop.warpSize;
 }
 
+// ---- pinned host blocks ----------------------------------------------------------
+// Page-locked host memory (hipHostMalloc) for the host mirrors of arrays, so
+// that uploads and downloads are single DMA transfers instead of copies staged
+// through the runtime's own buffers.  Pinning is slow (the pages are locked one
+// by one), so freed blocks are cached by size class, up to HOST_CACHE_MAX
+// bytes; live pinned memory is capped at HOST_LIVE_MAX (the caller falls back
+// to ordinary memory beyond it).
+constexpr size_t HOST_CACHE_MAX = size_t(4) << 30;
+constexpr size_t HOST_LIVE_MAX = size_t(32) << 30;
+
+struct HostPool {
+  std::mutex mu;
+  std::unordered_map<size_t, std::vector<void*>> free_;
+  std::unordered_map<void*, size_t> live_;
+  size_t live_bytes = 0, cached_bytes = 0;
+
+  static size_t round(size_t n) {
+    const size_t page = 4096, large = size_t(2) << 20;
+    if (n == 0) n = 1;
+    return n < (size_t(1) << 20) ? ceil_div(n, page) * page : ceil_div(n, large) * large;
+  }
+};
+
+static HostPool& host_pool() {
+  static HostPool* pool = new HostPool;  // kept until the process ends
+  return *pool;
+}
+
 }  // namespace vkh
 
 using namespace vkh;
@@ -295,6 +323,71 @@ int vkh_download(vkh_gpu* g, void* dst, const void* src, size_t nbytes) {
     VKH_REQUIRE(dst && src, "download: null pointer");
     VKH_HIP(hipMemcpyAsync(dst, src, nbytes, hipMemcpyDeviceToHost, g->stream));
     VKH_HIP(hipStreamSynchronize(g->stream));
+  });
+}
+
+int vkh_host_alloc(size_t nbytes, void** out) {
+  return guarded([&] {
+    VKH_REQUIRE(out, "host_alloc: null out");
+    *out = nullptr;
+    HostPool& hp = host_pool();
+    const size_t sz = HostPool::round(nbytes);
+    std::lock_guard<std::mutex> lk(hp.mu);
+    auto it = hp.free_.find(sz);
+    if (it != hp.free_.end() && !it->second.empty()) {
+      *out = it->second.back();
+      it->second.pop_back();
+      hp.cached_bytes -= sz;
+    } else {
+      if (hp.live_bytes + sz > HOST_LIVE_MAX) fail(VKH_OUT_OF_MEMORY, "host_alloc: pinned memory cap reached");
+      void* p = nullptr;
+      if (hipHostMalloc(&p, sz, hipHostMallocDefault) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        fail(VKH_OUT_OF_MEMORY, "host_alloc: hipHostMalloc of " + std::to_string(sz) + " bytes failed");
+      }
+      *out = p;
+    }
+    hp.live_[*out] = sz;
+    hp.live_bytes += sz;
+  });
+}
+
+int vkh_host_free(void* p) {
+  return guarded([&] {
+    if (!p) return;
+    HostPool& hp = host_pool();
+    std::lock_guard<std::mutex> lk(hp.mu);
+    auto it = hp.live_.find(p);
+    VKH_REQUIRE(it != hp.live_.end(), "host_free: not a pinned block of this library");
+    const size_t sz = it->second;
+    hp.live_.erase(it);
+    hp.live_bytes -= sz;
+    if (hp.cached_bytes + sz <= HOST_CACHE_MAX) {
+      hp.free_[sz].push_back(p);
+      hp.cached_bytes += sz;
+    } else {
+      VKH_HIP(hipHostFree(p));
+    }
+  });
+}
+
+int vkh_host_empty_cache(void) {
+  return guarded([&] {
+    HostPool& hp = host_pool();
+    std::lock_guard<std::mutex> lk(hp.mu);
+    for (auto& kv : hp.free_)
+      for (void* p : kv.second) VKH_HIP(hipHostFree(p));
+    hp.free_.clear();
+    hp.cached_bytes = 0;
+  });
+}
+
+int vkh_host_stats(size_t* live, size_t* cached) {
+  return guarded([&] {
+    HostPool& hp = host_pool();
+    std::lock_guard<std::mutex> lk(hp.mu);
+    if (live) *live = hp.live_bytes;
+    if (cached) *cached = hp.cached_bytes;
   });
 }
 

@@ -216,6 +216,13 @@ class _Plan:
             if i > 0:
                 yield 0, 0, B, d.input_dim, d.output_dim  # dX = dY W
 
+    def _reduce_shapes(self):
+        """(prev, axis, post) of every axis reduction _enqueue issues"""
+        B = self.B
+        yield 1, B, 1                                     # batch loss
+        for d in sorted({u.dense.output_dim for u in self.units if u.dense.b.grad is not None}):
+            yield 1, B, d                                 # bias gradient (vkh_bias_grad)
+
     def _enqueue_reg_grad(self, p: Parameter):
         """grad = R.grad(value) (regularizers.py:23-192), before dW / db are added"""
         h, n, R = self.h, p.value.size, p.R
@@ -252,6 +259,9 @@ class _Plan:
             if self.seq.autotune and self.mode != GEMM_MODE["fp32"]:
                 call("vkh_gemm_autotune", self.h, self.mode, ta, tb, M, N, K, 3, None, None)
             call("vkh_gemm_reserve", self.h, self.mode, M, N, K)
+        if self.seq.autotune:
+            for prev, K, post in self._reduce_shapes():
+                call("vkh_reduce_autotune", self.h, 1, prev, K, post, 3, None, None)
         self.gpu.wait()
         call("vkh_graph_begin", self.h)
         try:
@@ -331,8 +341,11 @@ class GraphSequence:
         never does (vulkpy/nn/models.py:55-78).
     autotune : bool
         Before capturing, time every GEMM kernel configuration on each
-        product shape of the step and use the fastest
-        (``vulkpy_amd.autotune_gemm``).  BF16 modes only.
+        product shape of the step (BF16 modes; ``vulkpy_amd.autotune_gemm``)
+        and every reduction configuration of the bias gradients and the
+        batch loss (``vulkpy_amd.autotune_reduce``), and use the fastest.
+        The choice is kept on the GPU context, so eager calls of the same
+        shapes use it too.
 
     Notes
     -----

@@ -26,6 +26,7 @@ import ctypes
 import functools
 import logging
 import os
+import weakref
 from typing import Iterable, List, Optional, Tuple, Union
 
 import numpy as np
@@ -239,6 +240,40 @@ def _reshape_target(size: int, shape) -> Tuple[int, ...]:
     return tuple(shape)
 
 
+_PIN_MIN_BYTES = 1 << 20
+_UPLOAD_CHUNK_BYTES = 16 << 20
+
+
+def _host_release(ptr: int) -> None:
+    try:
+        _lib.load().vkh_host_free(ptr)
+    except Exception:
+        pass
+
+
+def _host_empty(n: int, dtype) -> np.ndarray:
+    """
+    Host mirror of ``n`` elements.  Mirrors of 1 MiB and more are page-locked
+    blocks from the library's pinned pool (uploads and downloads are then
+    single DMA transfers); smaller ones, or any once the pool refuses, are
+    ordinary memory.  A pinned block goes back to the pool when the last view
+    of it is gone.
+    """
+    dtype = np.dtype(dtype)
+    nbytes = int(n) * dtype.itemsize
+    if nbytes < _PIN_MIN_BYTES:
+        return np.empty(n, dtype=dtype)
+    p = ctypes.c_void_p()
+    try:
+        call("vkh_host_alloc", nbytes, ctypes.byref(p))
+    except (MemoryError, RuntimeError):
+        return np.empty(n, dtype=dtype)
+    buf = (ctypes.c_char * nbytes).from_address(p.value)
+    fin = weakref.finalize(buf, _host_release, p.value)
+    fin.atexit = False  # the process returns page-locked memory when it ends
+    return np.frombuffer(buf, dtype=dtype, count=n)
+
+
 class _GPUArray(Resource):
     """Common base of Array and U32Array (vkarray.py:139-188)."""
 
@@ -259,10 +294,29 @@ class _GPUArray(Resource):
 
     # ---- construction helpers -------------------------------------------------
     def _init_data(self, data, dtype):
-        arr = np.array(data, dtype=dtype)
+        # np.array(data, dtype) of the reference (vkarray.py:423), copied once
+        # into the host mirror and uploaded from there
+        arr = np.asarray(data, dtype=dtype)
         self.shape = arr.shape
         self.buffer = _DeviceBuffer(self._gpu, arr.size, dtype)
-        self._host = np.ascontiguousarray(arr).reshape(-1)
+        self._host = _host_empty(arr.size, dtype)
+        chunk = _UPLOAD_CHUNK_BYTES // self._host.itemsize
+        if (isinstance(self._host.base, ctypes.Array) and arr.flags.c_contiguous
+                and arr.size >= 2 * chunk):
+            # pinned mirror: the DMA of each chunk overlaps the host copy of
+            # the next one
+            src = arr.reshape(-1)
+            h, itemsize = self._gpu._handle, self._host.itemsize
+            for off in range(0, arr.size, chunk):
+                end = min(arr.size, off + chunk)
+                self._host[off:end] = src[off:end]
+                call("vkh_upload", h, self._ptr + off * itemsize,
+                     self._host.ctypes.data + off * itemsize, (end - off) * itemsize)
+            self.job = self._gpu._job()
+            self._upload_pending = True
+            self._host_valid = True
+            return
+        np.copyto(self._host.reshape(arr.shape), arr)
         self._host_valid = True
         self._upload_host()
 
@@ -312,7 +366,7 @@ class _GPUArray(Resource):
 
     def _ensure_host(self):
         if self._host is None or self._host.size != self.buffer.n:
-            self._host = np.empty(self.buffer.n, dtype=self._dtype)
+            self._host = _host_empty(self.buffer.n, self._dtype)
             self._host_valid = False
 
     def _sync_host(self) -> np.ndarray:
@@ -1108,3 +1162,69 @@ def autotune_gemm(gpu: GPU, M: int, N: int, K: int, *, precision: str = "bf16",
     call("vkh_gemm_autotune", gpu._handle, mode, int(trans_a), int(trans_b), int(M), int(N), int(K),
          int(reps), ctypes.byref(best), ms)
     return {"best": best.value, "ms": [round(float(x), 5) for x in ms]}
+
+
+REDUCE_FAMILY = {-1: None, 0: "all", 1: "rows", 2: "cols"}
+
+
+def reduce_configs(family: str) -> List[dict]:
+    """Kernel configurations of a reduction family (``"all"``, ``"rows"`` or
+    ``"cols"``): ``unroll`` = 128-bit loads in flight per thread and ``par`` =
+    blocks per CU (``all``, ``rows``) or target threads per CU (``cols``,
+    which sets how many chunks the reduced axis is split into)."""
+    fam = {v: k for k, v in REDUCE_FAMILY.items()}[family]
+    n = ctypes.c_int()
+    call("vkh_reduce_configs", fam, ctypes.byref(n))
+    out = []
+    for i in range(n.value):
+        u, p = ctypes.c_int(), ctypes.c_int()
+        call("vkh_reduce_config_info", fam, i, ctypes.byref(u), ctypes.byref(p))
+        out.append({"unroll": u.value, "par": p.value})
+    return out
+
+
+def _reduce_problems(shape, axis) -> List[Tuple[int, int, int, int]]:
+    """(kind, prev, K, post) of the kernel calls ``Array._reduce`` makes for a
+    reduction of an array of ``shape`` over ``axis`` (without rebroadcast)."""
+    shape = tuple(int(s) for s in shape)
+    if axis is None:
+        return [(0, 1, _prod(shape), 1)]
+    out = []
+    dims = list(shape)
+    for a0, a1 in _axis_runs(_normalize_axes(axis, len(shape))):
+        out.append((1, _prod(dims[:a0]), _prod(dims[a0:a1 + 1]), _prod(dims[a1 + 1:])))
+        dims = dims[:a0] + dims[a1 + 1:]
+    return out
+
+
+def autotune_reduce(gpu: GPU, shape, axis=None, *, reps: int = 3) -> List[dict]:
+    """
+    Time every kernel configuration of the reduction of an array of ``shape``
+    over ``axis`` (``None``, an int or a tuple of ints, as in
+    :meth:`Array.sum`) on ``gpu`` and keep the fastest for each kernel call
+    it makes; later reductions of that shape (any op, eager or captured) use
+    it.
+
+    Returns
+    -------
+    list of dict
+        one per kernel call: ``family`` (None where there is a single kernel
+        choice), ``best`` (index into :func:`reduce_configs`), ``config`` and
+        ``ms`` (mean time of each configuration).
+    """
+    out = []
+    for kind, prev, K, post in _reduce_problems(shape, axis):
+        if prev * K * post == 0:
+            continue
+        f = ctypes.c_int()
+        call("vkh_reduce_family", kind, prev, K, post, ctypes.byref(f))
+        n = ctypes.c_int()
+        call("vkh_reduce_configs", f.value, ctypes.byref(n))
+        ms = (ctypes.c_float * n.value)()
+        best = ctypes.c_int()
+        call("vkh_reduce_autotune", gpu._handle, kind, prev, K, post, int(reps), ctypes.byref(best), ms)
+        fam = REDUCE_FAMILY[f.value]
+        out.append({"family": fam, "shape": (kind, prev, K, post), "best": best.value,
+                    "config": reduce_configs(fam)[best.value] if fam else None,
+                    "ms": [round(float(x), 5) for x in ms]})
+    return out
