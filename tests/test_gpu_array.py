@@ -414,3 +414,29 @@ def test_job_wait_semantics(gpu):
     b.wait()
     assert b.job is None
     gpu.wait()
+
+
+def test_jobs_record_events_only_when_waited(gpu):
+    """Jobs are tickets: enqueueing records no event; the first wait or query
+    records one that covers every job issued so far, and later waits on
+    covered jobs reuse it."""
+    ctx = gpu._ctx
+    gpu.wait()
+    latest = ctx.latest
+    x = A(gpu, np.arange(1 << 20, dtype=np.float32))
+    results = [x * float(k) for k in range(1, 9)]
+    assert ctx.latest is latest                       # no event per operation
+    first, last = results[0].job, results[-1].job
+    assert last._ticket > first._ticket
+    first.wait()                                      # records one event ...
+    e = ctx.latest
+    assert e is not latest and e.ticket >= last._ticket
+    assert last.done()                                # ... that covers later jobs
+    last.wait()
+    assert ctx.latest is e
+    for k, r in enumerate(results, start=1):
+        np.testing.assert_array_equal(np.asarray(r), np.arange(1 << 20, dtype=np.float32) * k)
+    y = x + 1.0
+    gpu.wait()                                        # a device wait completes every ticket
+    assert y.job.done() and ctx.latest is e
+    y.job.wait(1)

@@ -132,10 +132,11 @@ def test_skinny_epilogue(gpu, M, N, K, ta, tb):
 
 @pytest.mark.parametrize("mode", ["bf16", "bf16x3"])
 @pytest.mark.parametrize("ta,tb", [(0, 1), (1, 0)])
-def test_configurations_bit_identical(gpu, mode, ta, tb):
+@pytest.mark.parametrize("M,N,K", [(300, 260, 200), (300, 200, 8192)])
+def test_configurations_bit_identical(gpu, mode, ta, tb, M, N, K):
     """Every kernel configuration accumulates each output over k in the
-    same order, so tuning never changes a result."""
-    M, N, K = 300, 260, 200
+    same order, so tuning never changes a result (the second shape splits k
+    into 8 chunks, whose bounds do not depend on the configuration)."""
     rng = np.random.default_rng(9)
     A = rng.standard_normal((K, M) if ta else (M, K)).astype(np.float32)
     B = rng.standard_normal((N, K) if tb else (K, N)).astype(np.float32)
@@ -165,3 +166,56 @@ def test_autotune_api(gpu):
     assert vk.autotune_gemm(gpu, 4096, 1, 512, precision="bf16")["ms"][0] == -1
     with pytest.raises(ValueError):
         _lib.call("vkh_gemm_set_config", gpu._handle, _lib.GEMM_MODE["bf16"], 0, 0, 8, 8, 8, 99)
+
+
+# Split k: few output tiles (of 256 x 256) and k >= 4096 run k in chunks of
+# at least 1024 as separate blocks, then sum the chunk partials in order.
+SPLITK = [(300, 200, 8192), (300, 70, 5000), (64, 48, 4100), (4096, 256, 16384)]
+
+
+@pytest.mark.parametrize("mode", ["bf16", "bf16x3"])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1)])
+@pytest.mark.parametrize("M,N,K", SPLITK[:3])
+def test_split_k_accuracy(gpu, mode, ta, tb, M, N, K):
+    rng = np.random.default_rng(M + N + K + 3 * ta + tb)
+    A = rng.standard_normal((K, M) if ta else (M, K)).astype(np.float32)
+    B = rng.standard_normal((N, K) if tb else (K, N)).astype(np.float32)
+    got = run_gemm(gpu, mode, ta, tb, A, B)
+    ref, bound = ref_gemm(ta, tb, A, B)
+    assert (np.abs(got - ref) / bound).max() <= TOL[mode]
+
+
+def test_split_k_exact_and_epilogue(gpu):
+    rng = np.random.default_rng(21)
+    # integer operands: every chunk partial and their sum are exact
+    A = rng.integers(-2, 3, (64, 8192)).astype(np.float32)
+    B = rng.integers(-2, 3, (8192, 48)).astype(np.float32)
+    for mode in ("bf16", "bf16x3"):
+        np.testing.assert_array_equal(run_gemm(gpu, mode, 0, 0, A, B), A @ B)
+    # alpha, beta * C, bias and ReLU applied once, after the chunks are summed
+    A = rng.standard_normal((300, 5000)).astype(np.float32)
+    B = rng.standard_normal((70, 5000)).astype(np.float32)
+    C0 = rng.standard_normal((300, 70)).astype(np.float32)
+    bias = rng.standard_normal(70).astype(np.float32)
+    ref, bound = ref_gemm(0, 1, A, B)
+    for mode in ("bf16", "bf16x3"):
+        got = run_gemm(gpu, mode, 0, 1, A, B, C0=C0, alpha=0.5, beta=2.0, bias=bias)
+        want = 0.5 * ref + 2.0 * C0 + bias
+        assert (np.abs(got - want) <= 0.5 * TOL[mode] * bound + 1e-5 * (1 + np.abs(want))).all()
+        got = run_gemm(gpu, mode, 0, 1, A, B, bias=bias, act="relu")
+        want = np.maximum(ref + bias, 0)
+        assert (np.abs(got - want) <= TOL[mode] * bound + 1e-5 * (1 + np.abs(want))).all()
+
+
+def test_split_k_weight_gradient_shape(gpu):
+    """dW = dY^T X of a 256 -> 4096 layer at batch 16384 (16 chunks)."""
+    M, N, K = SPLITK[3]
+    rng = np.random.default_rng(22)
+    A = rng.standard_normal((K, M)).astype(np.float32)   # dY [batch, out]
+    B = rng.standard_normal((K, N)).astype(np.float32)   # X [batch, in]
+    rows = np.arange(0, M, 97)
+    a64 = A[:, rows].T.astype(np.float64)
+    ref, bound = a64 @ B.astype(np.float64), np.abs(a64) @ np.abs(B.astype(np.float64))
+    for mode in ("bf16", "bf16x3"):
+        got = run_gemm(gpu, mode, 1, 0, A, B)[rows]
+        assert (np.abs(got - ref) / bound).max() <= TOL[mode]

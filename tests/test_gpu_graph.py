@@ -119,6 +119,30 @@ def test_graph_packed_matches_eager_with_interleaved_gemms(gpu, precision):
         gpu.wait()
 
 
+@pytest.mark.parametrize("precision", ["bf16", "bf16x3"])
+def test_graph_split_k_matches_eager(gpu, precision):
+    """At batch 8192 the weight gradients of the 64 -> 256 -> 64 layers split
+    k into 8 chunks; the captured step (chunk partials in its packing region)
+    stays identical to the eager step."""
+    dims, acts = [(64, 256), (256, 64), (64, 4)], ["relu", "sigmoid", None]
+    B = 8192
+    rng = np.random.default_rng(13)
+    x = rng.standard_normal((B, 64)).astype(np.float32)
+    y = rng.standard_normal((B, 4)).astype(np.float32)
+    X, Y = vk.Array(gpu, data=x), vk.Array(gpu, data=y)
+    eager_layers = make_net(gpu, dims, acts, 70, lambda: nn.Ridge(1e-4), precision=precision)
+    graph_layers = make_net(gpu, dims, acts, 70, lambda: nn.Ridge(1e-4), precision=precision)
+    eager = nn.Sequence(eager_layers, nn.MSELoss(), regularize=True)
+    graph = nn.GraphSequence(graph_layers, nn.MSELoss(), precision=precision, regularize=True)
+    for step in range(3):
+        _, lg = graph.train(X, Y)
+        _, le = eager.train(X, Y)
+        gpu.wait()
+        assert np.asarray(le).item() == np.asarray(lg).item(), step
+        for a, b in zip(weights(eager_layers), weights(graph_layers)):
+            np.testing.assert_array_equal(a, b)
+
+
 def test_graph_rejects_unsupported(gpu):
     with pytest.raises(ValueError):
         nn.GraphSequence([nn.ReLU()], nn.MSELoss())

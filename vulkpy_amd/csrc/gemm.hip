@@ -427,6 +427,11 @@ struct PackedArgs {
   const uint16_t* Bh;
   const uint16_t* Bl;
   uint64_t Kp;
+  // split k: block (tile, blockIdx.y = s) accumulates k in [s*kc, (s+1)*kc)
+  // and writes its raw accumulators to part[s][M][N]; k_splitk_combine then
+  // applies the epilogue.  part == nullptr: the whole k range, epilogue here.
+  uint64_t kc;
+  float* part;
   GemmArgs e;
 };
 
@@ -529,6 +534,52 @@ __device__ __forceinline__ void epilogue4(const GemmArgs& e, uint64_t m, uint64_
   }
 }
 
+// Raw accumulators C[m, n..n+3] of one k chunk into part [M][N] (16-byte
+// store when the group is whole and rows are 16-byte aligned).
+__device__ __forceinline__ void store_part4(float* part, uint64_t N, uint64_t m, uint64_t n, float4 v) {
+  float* pp = part + m * N + n;
+  if (N % 4 == 0) {
+    *reinterpret_cast<float4*>(pp) = v;
+  } else {
+    const float a[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (n + q < N) pp[q] = a[q];
+  }
+}
+
+// Sums the S k-chunk partials of each output in chunk order and applies the
+// GEMM epilogue (alpha, beta * C, bias, activation, aux derivative, BF16 copy).
+__global__ void __launch_bounds__(256)
+k_splitk_combine(const float* part, uint64_t S, GemmArgs e) {
+  const uint64_t gpr = (e.N + 3) / 4, total = e.M * gpr, MN = e.M * e.N;
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += stride) {
+    const uint64_t m = t / gpr, n = (t - m * gpr) * 4;
+    const float* pp = part + m * e.N + n;
+    float4 acc;
+    if (e.N % 4 == 0) {
+      acc = *reinterpret_cast<const float4*>(pp);
+      for (uint64_t s = 1; s < S; ++s) {
+        const float4 v = *reinterpret_cast<const float4*>(pp + s * MN);
+        acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
+      }
+    } else {
+      float a[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (n + q < e.N) {
+          float x = pp[q];
+          for (uint64_t s = 1; s < S; ++s) x += pp[s * MN + q];
+          a[q] = x;
+        }
+      }
+      acc = make_float4(a[0], a[1], a[2], a[3]);
+    }
+    epilogue4(e, m, n, acc);
+  }
+}
+
 // BM x BN tile, WGM x WGN waves of (BM/WGM) x (BN/WGN), double-buffered LDS
 // and register prefetch of the next BK-deep K slice; one barrier per slice.
 // After the K loop the LDS holds each wave's accumulators, 32 rows at a time,
@@ -600,9 +651,10 @@ k_gemm_packed(PackedArgs p) {
     }
   };
 
-  const uint64_t nk = Kp / BK;
-  sa.load(p.Ah, p.Al, m0, Kp, 0);
-  sb.load(p.Bh, p.Bl, n0, Kp, 0);
+  const uint64_t kb = p.part ? (uint64_t)blockIdx.y * p.kc : 0;
+  const uint64_t nk = ((p.part ? min(kb + p.kc, Kp) : Kp) - kb) / BK;
+  sa.load(p.Ah, p.Al, m0, Kp, kb);
+  sb.load(p.Bh, p.Bl, n0, Kp, kb);
   sa.store(As(0, 0), As(0, PARTS - 1));
   sb.store(Bs(0, 0), Bs(0, PARTS - 1));
   __syncthreads();
@@ -612,7 +664,7 @@ k_gemm_packed(PackedArgs p) {
     // Prefetch of the next slice into registers.  The final iteration loads
     // the last slice again (in bounds, never read): the body has no branch,
     // so the scheduler sees one region.
-    const uint64_t kn = (kt + 1 < nk ? kt + 1 : kt) * BK;
+    const uint64_t kn = kb + (kt + 1 < nk ? kt + 1 : kt) * BK;
     sa.load(p.Ah, p.Al, m0, Kp, kn);
     sb.load(p.Bh, p.Bl, n0, Kp, kn);
 #pragma unroll
@@ -677,8 +729,11 @@ k_gemm_packed(PackedArgs p) {
     const int cc = (lane % F4R) * 4;
     for (int rr = lane / F4R; rr < 32; rr += RPI) {
       const uint64_t m = mb + rr, n = nb + cc;
-      if (m < e.M && n < e.N)
-        epilogue4(e, m, n, *reinterpret_cast<const float4*>(st + rr * SROW + cc));
+      if (m < e.M && n < e.N) {
+        const float4 v = *reinterpret_cast<const float4*>(st + rr * SROW + cc);
+        if (p.part) store_part4(p.part + (uint64_t)blockIdx.y * e.M * e.N, e.N, m, n, v);
+        else epilogue4(e, m, n, v);
+      }
     }
     __syncthreads();
   }
@@ -745,13 +800,38 @@ __global__ void __launch_bounds__(256) k_fill_noise(float* p, uint64_t n, uint32
   }
 }
 
-// Packing area of an M x N x K product: rows padded to PK_MAXTILE, which
-// covers every configuration (so a reservation never depends on tuning).
-static size_t packed_bytes(int mode, uint64_t M, uint64_t N, uint64_t K) {
+// Split k.  A product with few output tiles and a long k leaves most CUs
+// idle (4096 x 256 x 16384: 16 tiles of 256 x 256), so k is split into S
+// chunks that run as separate blocks; k_splitk_combine sums the chunk
+// partials in chunk order.  S and the chunk bounds depend on (M, N, K) only,
+// never on the kernel configuration, so every configuration accumulates each
+// chunk in the same order and configurations still never change results.
+constexpr uint64_t SPLITK_MIN_KP = 4096;    // shorter k: no split
+constexpr uint64_t SPLITK_MIN_CHUNK = 1024;  // k per chunk at least
+constexpr uint64_t SPLITK_REF_BLOCKS = 256;  // blocks to aim for (one per CU of MI355X)
+
+static uint64_t splitk_chunk(uint64_t M, uint64_t N, uint64_t Kp) {
+  const uint64_t tiles = ceil_div(M, 256) * ceil_div(N, 256);
+  if (Kp < SPLITK_MIN_KP || tiles * 2 > SPLITK_REF_BLOCKS) return Kp;
+  const uint64_t S = std::min(ceil_div(SPLITK_REF_BLOCKS, tiles), Kp / SPLITK_MIN_CHUNK);
+  if (S < 2) return Kp;
+  return ceil_div(Kp / PK_KALIGN, S) * PK_KALIGN;
+}
+
+static size_t operand_bytes(int mode, uint64_t M, uint64_t N, uint64_t Kp) {
   const uint64_t Mp = ceil_div(M, PK_MAXTILE) * PK_MAXTILE, Np = ceil_div(N, PK_MAXTILE) * PK_MAXTILE;
-  const uint64_t Kp = ceil_div(std::max<uint64_t>(K, 1), PK_KALIGN) * PK_KALIGN;
   const uint64_t parts = mode == VKH_GEMM_BF16X3 ? 2 : 1;
-  return (size_t)((Mp + Np) * Kp * 2 * parts);
+  return (size_t)ceil_div((Mp + Np) * Kp * 2 * parts, 256) * 256;
+}
+
+// Packing area of an M x N x K product: operands with rows padded to
+// PK_MAXTILE, which covers every configuration (so a reservation never
+// depends on tuning), then the split-k partials [S][M][N] when k is split.
+static size_t packed_bytes(int mode, uint64_t M, uint64_t N, uint64_t K) {
+  const uint64_t Kp = ceil_div(std::max<uint64_t>(K, 1), PK_KALIGN) * PK_KALIGN;
+  const uint64_t kc = splitk_chunk(M, N, Kp);
+  const uint64_t S = ceil_div(Kp, kc);
+  return operand_bytes(mode, M, N, Kp) + (S > 1 ? (size_t)(S * M * N * 4) : 0);
 }
 
 // Packing area.  Eager calls share a per-context grow-only area (stream order
@@ -821,9 +901,19 @@ static void gemm_packed(vkh_gpu* g, int mode, bool ta, bool tb, GemmArgs& a, int
   a.tiles_n = (uint32_t)(Np / s.bn);
   const uint64_t nb = (uint64_t)a.tiles_m * a.tiles_n;
   VKH_REQUIRE(nb < (uint64_t(1) << 31), "GEMM too large");
-  PackedArgs pa{Ah, Al, Bh, Bl, Kp, a};
-  s.launch(dim3((unsigned)nb), g->stream, pa);
+  const uint64_t kc = splitk_chunk(a.M, a.N, Kp);
+  const uint64_t S = ceil_div(Kp, kc);
+  float* part = nullptr;
+  if (S > 1) part = reinterpret_cast<float*>(reinterpret_cast<char*>(base) + operand_bytes(mode, a.M, a.N, Kp));
+  PackedArgs pa{Ah, Al, Bh, Bl, Kp, kc, part, a};
+  s.launch(dim3((unsigned)nb, (unsigned)S), g->stream, pa);
   check_launch("gemm (packed)");
+  if (S > 1) {
+    const uint64_t groups = a.M * ceil_div(a.N, 4);
+    hipLaunchKernelGGL(k_splitk_combine, dim3(stream_grid(g, groups, 256, 8)), dim3(256), 0, g->stream,
+                       (const float*)part, S, a);
+    check_launch("gemm split-k combine");
+  }
 }
 
 // ---- skinny products: min(M, N) <= 16 ------------------------------------------------

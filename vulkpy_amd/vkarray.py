@@ -17,8 +17,9 @@ Mechanism differences:
   writes the mirror and uploads it; after writing ``arr.array`` directly call
   ``arr.flush()`` (the reference's own flush, vkarray.py:163-167).
 * Every operation enqueues kernels on the GPU's single in-order HIP stream and
-  returns at once; ``job`` is an event recorded after them.  Dependencies are
-  ordered by the stream instead of host waits (_vkarray.cc:430-432).
+  returns at once; ``job`` marks the point of the stream after them (an event
+  is recorded there only if someone waits for it).  Dependencies are ordered
+  by the stream instead of host waits (_vkarray.cc:430-432).
 """
 from __future__ import annotations
 
@@ -53,6 +54,49 @@ class _Context:
         cus, lds, wave = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         call("vkh_gpu_info", self.handle, ctypes.byref(cus), ctypes.byref(lds), ctypes.byref(wave))
         self.num_cus, self.lds_bytes, self.wave_size = cus.value, lds.value, wave.value
+        # Jobs are tickets: job k is the k-th piece of work enqueued on the
+        # stream.  A HIP event is recorded only when a job is waited for or
+        # queried and no recorded event covers it yet; on the in-order stream
+        # an event recorded later covers every earlier ticket.
+        self.ticket = 0
+        self.done_ticket = 0
+        self.latest: Optional[_Event] = None
+
+    def issue(self) -> int:
+        self.ticket += 1
+        return self.ticket
+
+    def covering(self, ticket: int) -> "_Event":
+        """A recorded event that completes after job `ticket`."""
+        e = self.latest
+        if e is None or e.ticket < ticket:
+            e = _Event(self)
+            self.latest = e
+        return e
+
+    def mark_done(self, ticket: int) -> None:
+        if ticket > self.done_ticket:
+            self.done_ticket = ticket
+
+
+class _Event:
+    """One recorded HIP event and the tickets it covers."""
+
+    __slots__ = ("ev", "ticket")
+
+    def __init__(self, ctx: "_Context"):
+        ev = ctypes.c_void_p()
+        call("vkh_event_record", ctx.handle, ctypes.byref(ev))
+        self.ev = ev.value
+        self.ticket = ctx.ticket
+
+    def __del__(self):
+        try:
+            if self.ev:
+                _lib.lib.vkh_event_destroy(self.ev)
+                self.ev = None
+        except Exception:
+            pass
 
 
 _contexts = {}
@@ -70,14 +114,20 @@ class Job:
     """
     Completion handle of enqueued GPU work (replaces vulkpy's Job,
     _vkarray.cc:392-457).
+
+    Creating a job enqueues nothing: it is a ticket on the device's in-order
+    stream, and a HIP event is recorded only when a job is waited for or
+    queried (an event records a packet that costs GPU time, about 4 us, so
+    one per operation would tax every step).  Such an event also covers work
+    enqueued after the job, so a wait may return a little later than the
+    job's own work finished; it never returns earlier.
     """
 
-    __slots__ = ("_ev",)
+    __slots__ = ("_ctx", "_ticket")
 
     def __init__(self, ctx: _Context):
-        ev = ctypes.c_void_p()
-        call("vkh_event_record", ctx.handle, ctypes.byref(ev))
-        self._ev = ev.value
+        self._ctx = ctx
+        self._ticket = ctx.issue()
 
     def wait(self, timeout_ns: Optional[int] = None):
         """
@@ -89,21 +139,24 @@ class Job:
             Timeout in nanoseconds.  ``RuntimeError("Timeout at Command Wait")``
             is raised when it expires (_vkarray.cc:446-456).
         """
+        ctx = self._ctx
+        if self._ticket <= ctx.done_ticket:
+            return
+        e = ctx.covering(self._ticket)  # held here while waiting
         t = _UINT64_MAX if timeout_ns is None else int(timeout_ns)
-        _lib.check(_lib.lib.vkh_event_wait(self._ev, t), "Error at Command Wait")
+        _lib.check(_lib.lib.vkh_event_wait(e.ev, t), "Error at Command Wait")
+        ctx.mark_done(e.ticket)
 
     def done(self) -> bool:
+        ctx = self._ctx
+        if self._ticket <= ctx.done_ticket:
+            return True
+        e = ctx.covering(self._ticket)
         flag = ctypes.c_int()
-        call("vkh_event_query", self._ev, ctypes.byref(flag))
+        call("vkh_event_query", e.ev, ctypes.byref(flag))
+        if flag.value:
+            ctx.mark_done(e.ticket)
         return bool(flag.value)
-
-    def __del__(self):
-        try:
-            if self._ev:
-                _lib.lib.vkh_event_destroy(self._ev)
-                self._ev = None
-        except Exception:
-            pass
 
 
 class GPU:
@@ -180,7 +233,9 @@ class GPU:
         """
         Wait All GPU Operations
         """
+        t = self._ctx.ticket
         call("vkh_gpu_wait", self._handle)
+        self._ctx.mark_done(t)
 
     def memory_stats(self) -> Tuple[int, int]:
         """(live, cached) bytes held by this GPU's allocator."""
