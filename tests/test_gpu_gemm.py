@@ -219,3 +219,40 @@ def test_split_k_weight_gradient_shape(gpu):
     for mode in ("bf16", "bf16x3"):
         got = run_gemm(gpu, mode, 1, 0, A, B)[rows]
         assert (np.abs(got - ref) / bound).max() <= TOL[mode]
+
+
+# Short k (K <= 16): FP32 vector kernel in every mode, epilogue included.
+SMALLK = [(300, 260, 1), (257, 300, 16), (1000, 70, 10), (64, 4097, 3), (40, 40, 0)]
+
+
+@pytest.mark.parametrize("mode", ["fp32", "bf16", "bf16x3"])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("M,N,K", SMALLK)
+def test_short_k(gpu, mode, ta, tb, M, N, K):
+    rng = np.random.default_rng(M + 5 * N + K + ta + 2 * tb)
+    A = rng.standard_normal((K, M) if ta else (M, K)).astype(np.float32)
+    B = rng.standard_normal((N, K) if tb else (K, N)).astype(np.float32)
+    bias = rng.standard_normal(N).astype(np.float32)
+    got = run_gemm(gpu, mode, ta, tb, A, B, bias=bias, act="relu")
+    ref, bound = ref_gemm(ta, tb, A, B)
+    want = np.maximum(ref + bias, 0)
+    # FP32 products summed in order: K roundings of |A||B| at most, plus the bias
+    assert (np.abs(got - want) <= max(K, 1) * 2.0 ** -23 * (bound + np.abs(bias)) + 1e-30).all()
+
+
+def test_short_k_aux_epilogue(gpu):
+    """dX = dY W times ReLU'(h): the Dense input gradient of a 1-output layer."""
+    rng = np.random.default_rng(30)
+    B_, H = 512, 300
+    dy = rng.standard_normal((B_, 1)).astype(np.float32)
+    W = rng.standard_normal((1, H)).astype(np.float32)
+    h = np.maximum(rng.standard_normal((B_, H)), 0).astype(np.float32)
+    g_dy, g_W, g_h = vk.Array(gpu, data=dy), vk.Array(gpu, data=W), vk.Array(gpu, data=h)
+    out = vk.Array(gpu, shape=(B_, H))
+    for mode in ("fp32", "bf16", "bf16x3"):
+        _lib.call("vkh_gemm_ex", gpu._handle, _lib.GEMM_MODE[mode], 0, 0, B_, H, 1, 1.0,
+                  g_dy._ptr, 1, g_W._ptr, H, 0.0, out._ptr, H, None, 0, None,
+                  g_h._ptr, H, _lib.ACT["relu"])
+        out._set_job([g_dy, g_W, g_h])
+        want = (dy * W) * (h > 0)
+        np.testing.assert_array_equal(np.asarray(out), want.astype(np.float32))

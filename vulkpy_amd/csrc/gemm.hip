@@ -1145,6 +1145,49 @@ using namespace vkh;
 
 extern "C" {
 
+// ---- products with a short k: K <= SMALLK_MAX --------------------------------------
+// C = op(A) op(B) with K of a few elements (the input gradient of a Dense
+// layer with one or ten outputs) is a pass that writes C (and reads aux or
+// beta * C): memory-bound, so like the skinny products it runs in FP32 on the
+// vector units in every mode.  Thread (row m, columns n..n+3) sums
+// op(A)[m, k] op(B)[k, n + q] over k in order and applies the GEMM epilogue.
+constexpr uint64_t SMALLK_MAX = 16;
+
+__global__ void __launch_bounds__(256)
+k_gemm_smallk(GemmArgs e, uint64_t sam, uint64_t sak, uint64_t sbk, uint64_t sbn, int vb) {
+  const uint64_t n = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * 4;
+  if (n >= e.N) return;
+  const int K = (int)e.K;
+  for (uint64_t m = blockIdx.y; m < e.M; m += gridDim.y) {
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < K; ++k) {
+      const float av = e.A[m * sam + k * sak];
+      float bv[4];
+      if (vb && n + 3 < e.N) {
+        const float4 t = *reinterpret_cast<const float4*>(e.B + k * sbk + n);
+        bv[0] = t.x; bv[1] = t.y; bv[2] = t.z; bv[3] = t.w;
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bv[q] = n + q < e.N ? e.B[k * sbk + (n + q) * sbn] : 0.f;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] = k == 0 ? av * bv[q] : acc[q] + av * bv[q];
+    }
+    epilogue4(e, m, n, make_float4(acc[0], acc[1], acc[2], acc[3]));
+  }
+}
+
+static void gemm_smallk(vkh_gpu* g, bool ta, bool tb, const GemmArgs& a) {
+  const uint64_t sam = ta ? 1 : a.lda, sak = ta ? a.lda : 1;
+  const uint64_t sbk = tb ? 1 : a.ldb, sbn = tb ? a.ldb : 1;
+  const int vb = !tb && ((uintptr_t)a.B & 15u) == 0 && a.ldb % 4 == 0;
+  const uint64_t groups = ceil_div(a.N, 4);
+  VKH_REQUIRE(ceil_div(groups, 256) < (uint64_t(1) << 31), "gemm: N too large");
+  dim3 grid((unsigned)ceil_div(groups, 256), (unsigned)std::min<uint64_t>(a.M, 65535));
+  hipLaunchKernelGGL(k_gemm_smallk, grid, dim3(256), 0, g->stream, a, sam, sak, sbk, sbn, vb);
+  check_launch("gemm (small k)");
+}
+
 static void gemm_impl(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M, uint64_t N,
                       uint64_t K, float alpha, const float* A, uint64_t lda, const float* B,
                       uint64_t ldb, float beta, float* Cp, uint64_t ldc, const float* bias, int act,
@@ -1170,6 +1213,10 @@ static void gemm_impl(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M
   VKH_REQUIRE(mode >= VKH_GEMM_FP32 && mode <= VKH_GEMM_BF16X3, "gemm: unknown mode");
   if (std::min(M, N) <= SKINNY_MAX && K > 0) {
     gemm_skinny(g, trans_a, trans_b, a);
+    return;
+  }
+  if (K <= SMALLK_MAX) {
+    gemm_smallk(g, trans_a, trans_b, a);
     return;
   }
   const int va = ((uintptr_t)A & 15u) == 0 && lda % 4 == 0;
@@ -1248,7 +1295,7 @@ int vkh_gemm_autotune(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M
     if (reps < 1) reps = 3;
     int n = 1;
     if (mode != VKH_GEMM_FP32) packed_cfgs(mode, n);
-    const bool skinny = std::min(M, N) <= SKINNY_MAX;
+    const bool skinny = std::min(M, N) <= SKINNY_MAX || K <= SMALLK_MAX;
     if (mode == VKH_GEMM_FP32 || skinny) {  // a single kernel choice
       if (ms_per_config)
         for (int i = 0; i < n; ++i) ms_per_config[i] = -1.0f;
@@ -1309,7 +1356,8 @@ int vkh_gemm_reserve(vkh_gpu* g, int mode, uint64_t M, uint64_t N, uint64_t K) {
     use(g);
     VKH_REQUIRE(mode >= VKH_GEMM_FP32 && mode <= VKH_GEMM_BF16X3, "gemm_reserve: unknown mode");
     VKH_REQUIRE(!g->capturing, "gemm_reserve: not allowed during graph capture");
-    if (mode == VKH_GEMM_FP32 || M == 0 || N == 0) return;
+    // FP32, skinny and short-k products pack nothing
+    if (mode == VKH_GEMM_FP32 || M == 0 || N == 0 || std::min(M, N) <= SKINNY_MAX || K <= SMALLK_MAX) return;
     const size_t bytes = packed_bytes(mode, M, N, K);
     g->graph_pack_hint = std::max(g->graph_pack_hint, bytes);
     pack_area(g, bytes);
