@@ -29,7 +29,8 @@ Two more sections of the same JSON line cover the rest of the metric:
     split-BF16, TFLOP/s and error against an FP64 product.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-                [--mlp-steps S] [--no-gemm]
+                [--mlp-steps S] [--no-gemm] [--no-c2]
+                [--pmc-kernels | --gemm-pmc]   (counter runs only)
 """
 from __future__ import annotations
 
@@ -406,6 +407,8 @@ def main() -> int:
     ap.add_argument("--no-gemm", action="store_true", help="skip the GEMM precision report")
     ap.add_argument("--no-c2", action="store_true",
                     help="skip the bandwidth suite (and the CPU baseline that samples it)")
+    ap.add_argument("--gemm-pmc", action="store_true",
+                    help="only run five 8192^3 BF16 products (for counter runs) and exit")
     ap.add_argument("--pmc-kernels", action="store_true",
                     help="only launch the copy and dominant kernels (for counter runs) and exit")
     args = ap.parse_args()
@@ -419,6 +422,17 @@ def main() -> int:
     import vulkpy_amd as vk
     gpu = vk.GPU(0)
     timer = Timer(gpu)
+
+    if args.gemm_pmc:
+        # for rocprofv3 --pmc runs on the matrix-core kernel alone
+        rng = np.random.default_rng(0)
+        A = vk.Array(gpu, data=rng.standard_normal((8192, 8192), dtype=np.float32))
+        Bm = vk.Array(gpu, data=rng.standard_normal((8192, 8192), dtype=np.float32))
+        for _ in range(5):
+            A.matmul(Bm, precision="bf16")
+        gpu.wait()
+        print(json.dumps({"gemm_pmc": "5 x BF16 8192^3"}))
+        return 0
 
     c2 = None
     if not args.no_c2 or args.pmc_kernels:
