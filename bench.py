@@ -201,21 +201,39 @@ def mlp_compare(vk, gpu, steps: int, warmup: int = 2):
             h = L(h)
         return float(np.asarray(nn.HuberLoss()(h, Yt)))
 
-    arms = (("fp32", lambda: nn.Sequence(layers("fp32"), nn.HuberLoss(), regularize=True)),
+    # The control arm repeats the FP32 step with one change of summation
+    # order only: the bias-gradient column sums of the 4096-wide layers split
+    # the batch into 256 chunks instead of 128 (a reduction configuration).
+    # How far its losses move from the FP32 arm's is the noise floor that
+    # any change of arithmetic order meets on this run.
+    from vulkpy_amd import _lib
+    bias_shape = (1, 1, B, H)
+    reorder_cfg = next(i for i, c in enumerate(vk.reduce_configs("cols")) if c["par"] == 1024)
+
+    def reorder(on: bool):
+        _lib.call("vkh_reduce_set_config", gpu._handle, *bias_shape, reorder_cfg if on else -1)
+
+    arms = (("fp32", lambda: nn.Sequence(layers("fp32"), nn.HuberLoss(), regularize=True), None),
+            ("fp32_reordered", lambda: nn.Sequence(layers("fp32"), nn.HuberLoss(), regularize=True),
+             reorder),
             ("bf16_graph", lambda: nn.GraphSequence(layers("bf16"), nn.HuberLoss(),
                                                     precision="bf16", regularize=True,
-                                                    autotune=True)),
+                                                    autotune=True), None),
             ("bf16x3_graph", lambda: nn.GraphSequence(layers("bf16x3"), nn.HuberLoss(),
                                                       precision="bf16x3", regularize=True,
-                                                      autotune=True)))
+                                                      autotune=True), None))
     res = {}
-    for name, model in arms:
+    for name, model, hook in arms:
         log(f"bench: MLP {name} {steps} steps")
+        if hook:
+            hook(True)
         m = model()
         sps, losses = run(m)
         res[name] = {"steps_per_s": round(sps, 2), "loss_first": losses[1], "loss_final": losses[steps],
                      "loss_at_step": {str(k): v for k, v in losses.items()},
                      "heldout_loss": heldout(m)}
+        if hook:
+            hook(False)
         log(f"bench: MLP {name} {res[name]}")
         del m
         gpu.wait()

@@ -227,11 +227,13 @@ int vkh_reduce_set_config(vkh_gpu* gpu, int kind, uint64_t prev,
 /* times every configuration of the shape's family on scratch data (a sum;
  * one warm-up and `reps` timed runs each), keeps the fastest for the shape
  * and returns its index; ms_per_config (optional, vkh_reduce_configs
- * entries) receives each mean time in milliseconds (-1 for a family with a
- * single kernel choice).  Synchronises the stream; refused during graph
- * capture. */
+ * entries) receives each mean time in milliseconds (-1 for a configuration
+ * not timed, and for a family with a single kernel choice).  same_order != 0
+ * times only the configurations that keep the default summation order, so
+ * the choice cannot change a result whatever the timings.  Synchronises the
+ * stream; refused during graph capture. */
 int vkh_reduce_autotune(vkh_gpu* gpu, int kind, uint64_t prev, uint64_t axis,
-                        uint64_t post, int reps, int* best,
+                        uint64_t post, int reps, int same_order, int* best,
                         float* ms_per_config);
 
 /* ---- gather (F17) -------------------------------------------------------- */
@@ -243,6 +245,11 @@ int vkh_gather(vkh_gpu* gpu, const float* a, uint64_t n_a, const uint32_t* idx,
 int vkh_gather_axis(vkh_gpu* gpu, const float* a, const uint32_t* idx,
                     float* c, uint64_t prev, uint64_t axis, uint64_t post,
                     uint64_t n_idx);
+/* out[k, j] = (min(idx[k], classes - 1) == j) for k < n, j < classes: the
+ * one-hot rows U32Array.to_onehot gathers from np.identity(classes)
+ * (vkarray.py:239-253), without the host identity matrix */
+int vkh_onehot(vkh_gpu* gpu, const uint32_t* idx, uint64_t n, uint32_t classes,
+               float* out);
 
 /* ---- PRNG (F18) ---------------------------------------------------------- */
 /* replaces PRNG::Xoshiro128pp ctor (_vkarray.cc:643-675): size states seeded

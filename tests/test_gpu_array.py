@@ -171,6 +171,15 @@ def test_reference_gather_kats(gpu):
     np.testing.assert_allclose(oh, np.identity(3)[[0, 1, 2, 1, 0]])
     oh = vk.U32Array(gpu, data=[[0, 1], [2, 1]]).to_onehot(3)
     np.testing.assert_allclose(oh, np.identity(3)[[[0, 1], [2, 1]]])
+    # labels past the last class take the last class, as the reference's
+    # gather of identity rows clamps them; large label sets bit for bit
+    lab = np.asarray([0, 5, 2, 3, 1], dtype=np.uint32)
+    np.testing.assert_array_equal(vk.U32Array(gpu, data=lab).to_onehot(3),
+                                  np.identity(3, dtype=np.float32)[np.minimum(lab, 2)])
+    lab = np.random.default_rng(4).integers(0, 10, 4096).astype(np.uint32)
+    got = vk.U32Array(gpu, data=lab).to_onehot(10)
+    assert got.shape == (4096, 10)
+    np.testing.assert_array_equal(got, np.identity(10, dtype=np.float32)[lab])
 
 
 def test_reference_math_kats(gpu):
@@ -196,6 +205,83 @@ def test_reference_math_kats(gpu):
     np.testing.assert_allclose(A(gpu, x) ** A(gpu, y), x ** y, rtol=1e-7)
     np.testing.assert_allclose(A(gpu, x) ** 2.7, x ** 2.7, rtol=1e-6)
     np.testing.assert_allclose(1.3 ** A(gpu, y), 1.3 ** y, rtol=1e-6)
+
+
+def test_reference_inplace_kats(gpu):
+    """The in-place and ordering tests of test/test_vulkpy.py: every unary
+    function in place (inputs and tolerances of test_i<fn>), **= with an
+    array, a scalar and broadcast operands, in-place broadcast max / min /
+    arithmetic, clamp in place with every bound kind, and the cascade /
+    chain / unrelated-buffer / write-then-op / 2-D cases."""
+    cases = {
+        "abs": ([-1, 1, 3], lambda v: np.abs(v), 1e-7), "sign": ([-1, 1, 3], np.sign, 1e-7),
+        "sin": ([1, 3, 5], np.sin, 1e-5), "cos": ([1, 3, 5], np.cos, 1e-5),
+        "tan": ([1, 3, 5], np.tan, 1e-5), "sinh": ([1, 3, 5], np.sinh, 1e-5),
+        "cosh": ([1, 3, 5], np.cosh, 1e-5), "tanh": ([1, 3, 5], np.tanh, 1e-5),
+        "asin": ([0.5, 0.3, -0.2], np.arcsin, 1e-3), "acos": ([0.5, 0.3, -0.2], np.arccos, 1e-3),
+        "atan": ([0.5, 0.3, -0.2], np.arctan, 1e-5), "asinh": ([0.5, 0.3, -0.2], np.arcsinh, 1e-5),
+        "acosh": ([1.5, 1.3, 1.2], np.arccosh, 1e-5), "atanh": ([0.5, 0.3, -0.2], np.arctanh, 1e-5),
+        "exp": ([1, 2, 3], np.exp, 1e-7), "log": ([1, 2, 3], np.log, 1e-7),
+        "exp2": ([1, 2, 3], np.exp2, 1e-7), "log2": ([1, 2, 3], np.log2, 1e-7),
+        "sqrt": ([1, 2, 3], np.sqrt, 1e-7), "invsqrt": ([1, 2, 3], lambda v: np.sqrt(1 / v), 1e-7),
+    }
+    for f, (x, ref, rtol) in cases.items():
+        a = A(gpu, x)
+        out = getattr(a, f)(inplace=True)
+        a.wait()
+        np.testing.assert_allclose(a, ref(np.asarray(x, dtype=np.float64)), rtol=rtol, err_msg=f)
+        assert out is a or out is None, f
+    x, y = np.asarray([1, 2, 3]), np.asarray([1.1, 2.2, 1.4])
+    a = A(gpu, x)
+    a **= A(gpu, y)
+    np.testing.assert_allclose(a, x ** y, rtol=1e-7)
+    a = A(gpu, x)
+    a **= 2.7
+    np.testing.assert_allclose(a, x ** 2.7, rtol=1e-6)
+    for b in ([[2, 3]], [2, 3]):                    # test_ipow_broadcast(_newaxis)
+        a = A(gpu, [[1, 2], [3, 4]])
+        a **= A(gpu, b)
+        np.testing.assert_allclose(a, [[1, 8], [9, 64]])
+    for b in ([[2, 3]], [2, 3]):                    # test_imax/imin_broadcast(_newaxis)
+        a = A(gpu, [[1, 2], [3, 4]])
+        a.max(A(gpu, b), inplace=True)
+        np.testing.assert_allclose(a, [[2, 3], [3, 4]])
+        a = A(gpu, [[1, 2], [3, 4]])
+        a.min(A(gpu, b), inplace=True)
+        np.testing.assert_allclose(a, [[1, 2], [2, 3]])
+    ones = np.ones((2, 2))
+    for op, b, want in (("__isub__", np.ones(2), np.zeros((2, 2))), ("__imul__", np.ones((2, 1)), ones),
+                        ("__iadd__", np.ones(2), 2 * ones), ("__itruediv__", np.ones((2, 1)), ones)):
+        a = A(gpu, ones)
+        a = getattr(a, op)(A(gpu, b))
+        np.testing.assert_allclose(a, want, err_msg=op)
+    x, lo, hi = np.asarray([1.2, 3.5, 10]), np.asarray([3, 3, 4]), np.asarray([6, 6, 7])
+    for lo_, hi_ in ((lo, hi), (3, hi), (lo, 7), (3, 7)):   # test_iclamp, _sv, _vs, _ss
+        a = A(gpu, x)
+        a.clamp(A(gpu, lo_) if isinstance(lo_, np.ndarray) else lo_,
+                A(gpu, hi_) if isinstance(hi_, np.ndarray) else hi_, inplace=True)
+        np.testing.assert_allclose(a, np.clip(x, lo_, hi_))
+    m = A(gpu, [[1, 2], [3, 4]])
+    m.clamp(A(gpu, [[2, 3]]), A(gpu, [2, 3]), inplace=True)  # test_iclamp_broadcast
+    np.testing.assert_allclose(m, [[2, 3], [2, 3]])
+    # ordering: cascade, unrelated buffers, chain, write then op, 2-D
+    a, b, c = A(gpu, [5, 5, 5]), A(gpu, [1, 1, 1]), A(gpu, [4, 4, 4])
+    d = a + b + c
+    d.wait()
+    np.testing.assert_allclose(d, [10, 10, 10])
+    e, f = A(gpu, [5, 5, 5]) + A(gpu, [1, 1, 1]), A(gpu, [2, 2, 2]) + A(gpu, [4, 4, 4])
+    np.testing.assert_allclose(e, f)
+    a, b = A(gpu, [1, 1, 1]), A(gpu, [2, 2, 2])
+    c = a + b
+    d = a + c
+    d.wait()
+    np.testing.assert_allclose(d, [4, 4, 4])
+    a, b = vk.Array(gpu, shape=(3,)), vk.Array(gpu, shape=(3,))
+    a[:] = 5
+    b[:] = 7
+    np.testing.assert_allclose(a + b, [12, 12, 12])
+    np.testing.assert_allclose(A(gpu, [[1, 1], [1, 1]]) + A(gpu, [[2, 2], [2, 2]]), [[3, 3], [3, 3]])
+    np.testing.assert_allclose(A(gpu, np.ones((2, 2))).broadcast_to((3, 2, 2)), np.ones((3, 2, 2)))
 
 
 # ---- oracle parity on the C1 workload ---------------------------------------------

@@ -181,4 +181,26 @@ def test_autotune_api(gpu):
     with pytest.raises(ValueError):
         set_cfg(gpu, (1, 1, 4096, 64), 99)
     with pytest.raises(ValueError):
-        _lib.call("vkh_reduce_autotune", gpu._handle, 1, 0, 5, 5, 1, None, None)
+        _lib.call("vkh_reduce_autotune", gpu._handle, 1, 0, 5, 5, 1, 0, None, None)
+
+
+def test_autotune_same_order(gpu):
+    """same_order times only the configurations with the default summation
+    order, so the tuned sums equal the default ones bit for bit."""
+    rng = np.random.default_rng(33)
+    x = rng.standard_normal((4096, 64)).astype(np.float32)
+    g = A(gpu, x)
+    default = {ax: bits(np.asarray(g.sum(axis=ax))) for ax in (0, None)}
+    tuned = []
+    try:
+        for ax in (0, None):
+            (r,) = vk.autotune_reduce(gpu, x.shape, ax, reps=2, same_order=True)
+            tuned.append(r["shape"])
+            cfgs = vk.reduce_configs(r["family"])
+            for c, t in zip(cfgs, r["ms"]):
+                assert (t > 0) == (c["par"] == cfgs[0]["par"]), (c, t)
+            assert cfgs[r["best"]]["par"] == cfgs[0]["par"]
+            np.testing.assert_array_equal(bits(np.asarray(g.sum(axis=ax))), default[ax])
+    finally:
+        for problem in tuned:
+            set_cfg(gpu, problem, -1)

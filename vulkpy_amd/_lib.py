@@ -86,6 +86,7 @@ _SIGNATURES = {
     "vkh_unary": (c_int, [P, c_int, P, P, U64]),
     "vkh_clamp": (c_int, [P, P, P, P, c_float, c_float, P, c_int, _u64p, _u64p, _u64p, _u64p]),
     "vkh_broadcast_to": (c_int, [P, P, P, c_int, _u64p, _u64p]),
+    "vkh_onehot": (c_int, [P, P, U64, c_uint32, P]),
     "vkh_reduce_all": (c_int, [P, c_int, P, U64, P, c_int]),
     "vkh_reduce_axis": (c_int, [P, c_int, P, P, U64, U64, U64, c_int, c_float, c_float]),
     "vkh_reduce_family": (c_int, [c_int, U64, U64, U64, POINTER(c_int)]),
@@ -93,7 +94,8 @@ _SIGNATURES = {
     "vkh_reduce_config_info": (c_int, [c_int, c_int, POINTER(c_int), POINTER(c_int)]),
     "vkh_reduce_get_config": (c_int, [P, c_int, U64, U64, U64, POINTER(c_int), POINTER(c_int)]),
     "vkh_reduce_set_config": (c_int, [P, c_int, U64, U64, U64, c_int]),
-    "vkh_reduce_autotune": (c_int, [P, c_int, U64, U64, U64, c_int, POINTER(c_int), POINTER(c_float)]),
+    "vkh_reduce_autotune": (c_int, [P, c_int, U64, U64, U64, c_int, c_int, POINTER(c_int),
+                                    POINTER(c_float)]),
     "vkh_gather": (c_int, [P, P, U64, P, P, U64]),
     "vkh_gather_axis": (c_int, [P, P, P, P, U64, U64, U64, U64]),
     "vkh_xoshiro_create": (c_int, [P, c_uint32, U64, POINTER(P)]),

@@ -394,6 +394,21 @@ k_gather_axis(const float* a, const uint32_t* idx, float* c, uint64_t prev, uint
   }
 }
 
+// out[k, j] = 1 if min(idx[k], classes - 1) == j else 0: the one-hot rows the
+// reference gathers from an identity matrix (vkarray.py:239-253), out-of-range
+// labels clamped like gather_axis
+__global__ void __launch_bounds__(256)
+k_onehot(const uint32_t* idx, float* out, uint64_t n, uint32_t classes) {
+  const uint64_t total = n * classes;
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += stride) {
+    const uint64_t k = t / classes;
+    const uint32_t j = (uint32_t)(t - k * classes);
+    const uint32_t lab = min(idx[k], classes - 1);
+    out[t] = lab == j ? 1.0f : 0.0f;
+  }
+}
+
 }  // namespace vkh
 
 struct vkh_xoshiro {
@@ -634,6 +649,18 @@ int vkh_gather(vkh_gpu* g, const float* a, uint64_t na, const uint32_t* idx, flo
     unsigned grid = stream_grid(g, n, 256, 8);
     hipLaunchKernelGGL(k_gather, dim3(grid), dim3(256), 0, g->stream, a, na, idx, c, n);
     check_launch("gather");
+  });
+}
+
+int vkh_onehot(vkh_gpu* g, const uint32_t* idx, uint64_t n, uint32_t classes, float* out) {
+  return guarded([&] {
+    use(g);
+    VKH_REQUIRE(classes >= 1, "onehot: need at least one class");
+    if (n == 0) return;
+    VKH_REQUIRE(idx && out, "onehot: null pointer");
+    unsigned grid = stream_grid(g, n * classes, 256, 8);
+    hipLaunchKernelGGL(k_onehot, dim3(grid), dim3(256), 0, g->stream, idx, out, n, classes);
+    check_launch("onehot");
   });
 }
 

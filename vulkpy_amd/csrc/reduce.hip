@@ -543,7 +543,7 @@ int vkh_reduce_set_config(vkh_gpu* g, int kind, uint64_t prev, uint64_t axis, ui
 }
 
 int vkh_reduce_autotune(vkh_gpu* g, int kind, uint64_t prev, uint64_t axis, uint64_t post,
-                        int reps, int* best, float* ms_per_config) {
+                        int reps, int same_order, int* best, float* ms_per_config) {
   return guarded([&] {
     use(g);
     VKH_REQUIRE(kind == 0 || kind == 1, "reduction kind must be 0 (full) or 1 (axis)");
@@ -553,7 +553,7 @@ int vkh_reduce_autotune(vkh_gpu* g, int kind, uint64_t prev, uint64_t axis, uint
     if (reps < 1) reps = 3;
     const int fam = red_family(kind, axis, post);
     int n = 0;
-    red_cfgs(fam, n);
+    const RedCfg* cfgs = red_cfgs(fam, n);
     if (fam == VKH_RFAM_NONE) {  // a single kernel choice
       if (ms_per_config) ms_per_config[0] = -1.0f;
       if (best) *best = 0;
@@ -576,8 +576,16 @@ int vkh_reduce_autotune(vkh_gpu* g, int kind, uint64_t prev, uint64_t axis, uint
     VKH_HIP(hipEventCreate(&e1));
     int bi = -1;
     float bt = 0.0f;
+    // same_order: only configurations whose summation order is the
+    // default's (every ROWS configuration; ALL and COLS with the default
+    // par), so the choice cannot change a result
+    auto candidate = [&](int i) { return !same_order || fam == VKH_RFAM_ROWS || cfgs[i].par == cfgs[0].par; };
     try {
       for (int i = 0; i < n; ++i) {
+        if (!candidate(i)) {
+          if (ms_per_config) ms_per_config[i] = -1.0f;
+          continue;
+        }
         run(i);  // warm-up
         VKH_HIP(hipEventRecord(e0, g->stream));
         for (int r = 0; r < reps; ++r) run(i);

@@ -261,7 +261,9 @@ class _Plan:
             call("vkh_gemm_reserve", self.h, self.mode, M, N, K)
         if self.seq.autotune:
             for prev, K, post in self._reduce_shapes():
-                call("vkh_reduce_autotune", self.h, 1, prev, K, post, 3, None, None)
+                # only configurations with the default summation order, so
+                # the step's results never depend on the timings
+                call("vkh_reduce_autotune", self.h, 1, prev, K, post, 3, 1, None, None)
         self.gpu.wait()
         call("vkh_graph_begin", self.h)
         try:
@@ -342,10 +344,11 @@ class GraphSequence:
     autotune : bool
         Before capturing, time every GEMM kernel configuration on each
         product shape of the step (BF16 modes; ``vulkpy_amd.autotune_gemm``)
-        and every reduction configuration of the bias gradients and the
-        batch loss (``vulkpy_amd.autotune_reduce``), and use the fastest.
-        The choice is kept on the GPU context, so eager calls of the same
-        shapes use it too.
+        and the reduction configurations of the bias gradients and the batch
+        loss that keep the default summation order
+        (``vulkpy_amd.autotune_reduce(..., same_order=True)``), and use the
+        fastest.  Neither choice changes a result.  The choice is kept on the
+        GPU context, so eager calls of the same shapes use it too.
 
     Notes
     -----

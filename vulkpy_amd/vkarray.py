@@ -558,8 +558,20 @@ class U32Array(_GPUArray):
         -------
         vulkpy_amd.Array
             One hot vector
+
+        Notes
+        -----
+        The reference gathers rows of ``np.identity(num_classes)``
+        (vkarray.py:239-253); here one kernel writes them, with labels beyond
+        the last class clamped to it as that gather does.
         """
-        return Array(self._gpu, data=np.identity(num_classes)).gather(self, axis=0)
+        num_classes = int(num_classes)
+        if num_classes < 1:
+            raise ValueError("num_classes must be positive")
+        ret = Array(self._gpu, shape=tuple(self.shape) + (num_classes,))
+        call("vkh_onehot", self._gpu._handle, self._ptr, self.size, num_classes, ret._ptr)
+        ret._set_job([self])
+        return ret
 
 
 class Shape(U32Array):
@@ -1252,13 +1264,16 @@ def _reduce_problems(shape, axis) -> List[Tuple[int, int, int, int]]:
     return out
 
 
-def autotune_reduce(gpu: GPU, shape, axis=None, *, reps: int = 3) -> List[dict]:
+def autotune_reduce(gpu: GPU, shape, axis=None, *, reps: int = 3,
+                    same_order: bool = False) -> List[dict]:
     """
     Time every kernel configuration of the reduction of an array of ``shape``
     over ``axis`` (``None``, an int or a tuple of ints, as in
     :meth:`Array.sum`) on ``gpu`` and keep the fastest for each kernel call
     it makes; later reductions of that shape (any op, eager or captured) use
-    it.
+    it.  With ``same_order`` only the configurations that keep the default
+    summation order are timed (``ms`` is -1 for the others), so the choice
+    cannot change any result.
 
     Returns
     -------
@@ -1277,7 +1292,8 @@ def autotune_reduce(gpu: GPU, shape, axis=None, *, reps: int = 3) -> List[dict]:
         call("vkh_reduce_configs", f.value, ctypes.byref(n))
         ms = (ctypes.c_float * n.value)()
         best = ctypes.c_int()
-        call("vkh_reduce_autotune", gpu._handle, kind, prev, K, post, int(reps), ctypes.byref(best), ms)
+        call("vkh_reduce_autotune", gpu._handle, kind, prev, K, post, int(reps), int(same_order),
+             ctypes.byref(best), ms)
         fam = REDUCE_FAMILY[f.value]
         out.append({"family": fam, "shape": (kind, prev, K, post), "best": best.value,
                     "config": reduce_configs(fam)[best.value] if fam else None,
