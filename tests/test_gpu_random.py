@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 import vulkpy_amd as vk
+from vulkpy_amd._lib import call
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -55,6 +56,24 @@ def test_normal_matches_oracle(gpu, size, n):
     scale = np.abs(want - np.float32(0.5)) + np.float32(0.5)
     assert np.all(np.abs(got - want) <= 4 * np.spacing(scale)), np.abs(got - want).max()
     np.testing.assert_array_equal(r.state(), o.state)
+
+
+def test_fused_normal_equals_two_pass(gpu):
+    """The fused kernel evaluates ln(1 - u) and sqrt on their generated
+    ranges (hardware log2 / sqrt with the device functions' own steps and no
+    range handling); the two-pass path (uniforms, then vkh_box_muller with the
+    device logf / sqrtf) must give the same bits.  2^25 pairs cover nearly
+    all 2^23 float uniforms u1 can take."""
+    n = 1 << 26
+    fused = np.asarray(vk.random.Xoshiro128pp(gpu, size=1 << 16, seed=7).normal(shape=(n,), mean=0.25, stddev=1.5))
+    u = vk.random.Xoshiro128pp(gpu, size=1 << 16, seed=7).random(shape=(n,))
+    two = vk.Array(gpu, shape=(n,))
+    call("vkh_box_muller", gpu._handle, u._ptr, two._ptr, n, 0.25, 1.5)
+    two._set_job([u])
+    np.testing.assert_array_equal(fused.view(np.uint32), np.asarray(two).view(np.uint32))
+    # odd length: the last pair's second value is dropped, the rest agree
+    odd = np.asarray(vk.random.Xoshiro128pp(gpu, size=1 << 16, seed=7).normal(shape=(n - 1,), mean=0.25, stddev=1.5))
+    np.testing.assert_array_equal(odd.view(np.uint32), fused[:-1].view(np.uint32))
 
 
 def test_randrange(gpu):

@@ -276,14 +276,40 @@ k_xoshiro_chunked(const uint4* init, uint4* state, const uint32_t* jumps, void* 
   if (t1 == L) state[i] = s;
 }
 
+// ln t for t in [2^-23, 1] (t = 1 - u of a generated uniform): the hardware
+// log2, times ln 2 held as two floats with the rounding error of the leading
+// product recovered by an FMA.  On this range (no zero, denormal or infinite
+// input) these are the device logf's own steps, so the result is the same
+// without its range handling.
+constexpr float LN2_HI = 0x1.62e42ep-1f, LN2_LO = 0x1.efa39ep-25f;  // hi + lo = ln 2
+__device__ __forceinline__ float log_unit(float t) {
+  const float l2 = __builtin_amdgcn_logf(t);
+  const float p = l2 * LN2_HI;
+  const float e = fmaf(l2, LN2_LO, fmaf(l2, LN2_HI, -p));
+  return p + e;
+}
+
+// Correctly rounded sqrt x for x in [0, 2^96): the hardware square root, then
+// whichever neighbour's product with it brackets x (x = 0 stays 0: the lower
+// neighbour's residual is NaN, the upper one's is -0).
+__device__ __forceinline__ float sqrt_rn(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sm = __int_as_float(__float_as_int(s) - 1), sp = __int_as_float(__float_as_int(s) + 1);
+  const float r = fmaf(-sm, s, x) <= 0.0f ? sm : s;
+  return fmaf(-sp, s, x) > 0.0f ? sp : r;
+}
+
 // Box-Muller of one pair of uniforms (prng_box_muller.comp:25-31) in the
-// reference's FP32 operation order.  log and sqrt are the device's FP32
-// functions, sin and cos come from sincos_small (the GLSL.std.450 functions
-// the reference relies on are likewise FP32 within a few ulp).
+// reference's FP32 operation order; sin and cos come from sincos_small (the
+// GLSL.std.450 functions the reference relies on are likewise FP32 within a
+// few ulp).  GEN: u1 is a generated uniform in [0, 1), so log and sqrt take
+// the range-specific forms above; otherwise (uniforms from a caller's
+// buffer) the device logf and sqrtf.
+template <bool GEN>
 __device__ __forceinline__ void box_muller(float u1, float u2, float mean, float sd, float& o1, float& o2) {
   const float t = 1.0f - u1;
-  const float l = logf(t);
-  const float r = sqrtf(-2.0f * l) * sd;
+  const float l = GEN ? log_unit(t) : logf(t);
+  const float r = (GEN ? sqrt_rn(-2.0f * l) : sqrtf(-2.0f * l)) * sd;
   const float angle = 6.28318530718f * u2;
   float s, c;
   sincos_small(angle, s, c);
@@ -307,6 +333,8 @@ __device__ __forceinline__ void store_pair(float* out, uint64_t j, uint64_t n, f
 
 // Fused uniform + Box-Muller: thread p owns states 2p and 2p+1 (S even) and
 // produces normal pairs at uniform indices j = c*S + 2p < nu (nu even).
+// PAIRED: n == nu and out 8-byte aligned, so every pair is one 8-byte store.
+template <bool PAIRED>
 __global__ void __launch_bounds__(PRNG_BLOCK)
 k_xoshiro_normal(uint4* state, float* out, uint32_t S, uint64_t nu, uint64_t n, float mean, float sd) {
   const uint64_t p = (uint64_t)blockIdx.x * PRNG_BLOCK + threadIdx.x;
@@ -317,8 +345,13 @@ k_xoshiro_normal(uint4* state, float* out, uint32_t S, uint64_t nu, uint64_t n, 
     const float u1 = to_unit(dnext(s1));
     const float u2 = to_unit(dnext(s2));
     float o1, o2;
-    box_muller(u1, u2, mean, sd, o1, o2);
-    store_pair(out, j, n, o1, o2);
+    box_muller<true>(u1, u2, mean, sd, o1, o2);
+    if constexpr (PAIRED) {
+      const vkh_f2v v = {o1, o2};
+      __builtin_nontemporal_store(v, reinterpret_cast<vkh_f2v*>(out + j));
+    } else {
+      store_pair(out, j, n, o1, o2);
+    }
   }
   state[i] = s1;
   state[i + 1] = s2;
@@ -345,7 +378,7 @@ k_xoshiro_normal_chunked(const uint4* init, uint4* state, const uint32_t* jumps,
     const float u1 = to_unit(dnext(s1));
     const float u2 = to_unit(dnext(s2));
     float o1, o2;
-    box_muller(u1, u2, mean, sd, o1, o2);
+    box_muller<true>(u1, u2, mean, sd, o1, o2);
     store_pair(out, j, n, o1, o2);
   }
   if (t1 == L) {
@@ -361,7 +394,7 @@ k_box_muller(const float* u, float* out, uint64_t npairs, uint64_t n, float mean
     const uint64_t j = 2 * p;
     const float u1 = u[j], u2 = u[j + 1];
     float o1, o2;
-    box_muller(u1, u2, mean, sd, o1, o2);
+    box_muller<false>(u1, u2, mean, sd, o1, o2);
     if (j < n) out[j] = o1;
     if (j + 1 < n) out[j + 1] = o2;
   }
@@ -623,8 +656,12 @@ int vkh_xoshiro_normal(vkh_xoshiro* r, float* out, uint64_t n, float mean, float
                            cp.Lc, cp.C, mean, sd);
       } else {
         uint64_t th = std::min<uint64_t>(r->size, nu) / 2;
-        hipLaunchKernelGGL(k_xoshiro_normal, dim3(prng_grid(th)), dim3(PRNG_BLOCK), 0, g->stream,
-                           (uint4*)r->state, out, r->size, nu, n, mean, sd);
+        if (n == nu && ((uintptr_t)out & 7u) == 0)
+          hipLaunchKernelGGL(k_xoshiro_normal<true>, dim3(prng_grid(th)), dim3(PRNG_BLOCK), 0, g->stream,
+                             (uint4*)r->state, out, r->size, nu, n, mean, sd);
+        else
+          hipLaunchKernelGGL(k_xoshiro_normal<false>, dim3(prng_grid(th)), dim3(PRNG_BLOCK), 0, g->stream,
+                             (uint4*)r->state, out, r->size, nu, n, mean, sd);
       }
       check_launch("xoshiro normal");
       return;
