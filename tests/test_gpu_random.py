@@ -29,6 +29,10 @@ def test_seeding_matches_oracle(gpu, size):
     (64, [3, 10, 64, 65, 200, 1]),
     (7, [1, 6, 7, 8, 50]),
     (1000, [999, 1000, 1001, 4321, 17]),
+    # multiples of 256 states with a multiple of 4 steps per state take the
+    # LDS-staged row kernel; the 1000 in between leaves the states uneven
+    (512, [2048, 4096, 6144, 1000, 2048, 512 * 8]),
+    (4096, [4096 * 64, 4096 * 8]),
 ])
 def test_stream_interleaved(gpu, size, calls):
     r = vk.random.Xoshiro128pp(gpu, size=size, seed=7)

@@ -528,7 +528,9 @@ void launch_uniform(vkh_xoshiro* r, void* out, uint64_t n, uint32_t low, uint32_
                        low, range);
   } else {
     // one state per thread (measured 197 us for 2^28 outputs from 2^20 states;
-    // four states per thread with 16-byte stores: 232 us)
+    // four states per thread with 16-byte stores: 232 us; four steps of a
+    // block's 256 states staged through LDS so that each wave stores one
+    // 1 KB row segment with 16-byte stores: no faster)
     const uint64_t th = std::min<uint64_t>(r->size, n);
     hipLaunchKernelGGL(k_xoshiro<OUT>, dim3(prng_grid(th)), dim3(PRNG_BLOCK), 0, g->stream,
                        (uint4*)r->state, out, r->size, n, low, range);
