@@ -314,8 +314,10 @@ int vkh_gemm_reserve(vkh_gpu* gpu, int mode, uint64_t M, uint64_t N,
  * shape uses the built-in default.  Products with min(M, N) <= 16 or
  * K <= 16 use the vector kernels and no configuration. */
 int vkh_gemm_configs(int mode, int* count);
+/* direct (optional): 1 when the configuration loads operand slices from
+ * global memory straight into LDS, 0 when it stages them through registers */
 int vkh_gemm_config_info(int mode, int index, int* bm, int* bn, int* bk,
-                         int* waves);
+                         int* waves, int* direct);
 /* the configuration a shape uses; *tuned = 1 if chosen, 0 for the default */
 int vkh_gemm_get_config(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
                         uint64_t M, uint64_t N, uint64_t K, int* index,

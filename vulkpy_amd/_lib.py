@@ -112,7 +112,7 @@ _SIGNATURES = {
     "vkh_gemm_reserve": (c_int, [P, c_int, U64, U64, U64]),
     "vkh_gemm_configs": (c_int, [c_int, POINTER(c_int)]),
     "vkh_gemm_config_info": (c_int, [c_int, c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int),
-                                     POINTER(c_int)]),
+                                     POINTER(c_int), POINTER(c_int)]),
     "vkh_gemm_get_config": (c_int, [P, c_int, c_int, c_int, U64, U64, U64, POINTER(c_int),
                                     POINTER(c_int)]),
     "vkh_gemm_set_config": (c_int, [P, c_int, c_int, c_int, U64, U64, U64, c_int]),

@@ -739,6 +739,149 @@ k_gemm_packed(PackedArgs p) {
   }
 }
 
+// BF16 variant whose operand slices go from global memory straight into LDS
+// (global_load_lds_dwordx4: no VGPR staging, no LDS store instructions).  A
+// wave instruction fills 8 unpadded rows of 128 B (one BK = 64 slice of 8
+// rows): lane l writes row l / 8, 16-byte position l % 8, and fetches the
+// global chunk (l % 8) ^ (row & 7) of that row.  The XOR swizzle takes the
+// place of the row padding: a fragment read of 8 consecutive rows at one
+// chunk touches 8 distinct positions.  The next slice is requested at the
+// start of a slice and awaited, with one barrier, at its end.  Every output
+// accumulates k in the same order as k_gemm_packed.
+constexpr int DIRECT_SMEM_BYTES = 128 * 1024;
+
+template <int BM, int BN, int WGM, int WGN>
+__global__ void __launch_bounds__(64 * WGM * WGN)
+k_gemm_direct(PackedArgs p) {
+  constexpr int BK = 64, ROWB = BK * 2;  // a slice row: 64 BF16 = 128 B
+  constexpr int WAVES = WGM * WGN;
+  constexpr int WM = BM / WGM, WN = BN / WGN;
+  static_assert(WM % 32 == 0 && WN % 32 == 0, "32 x 32 MFMA blocks per wave");
+  static_assert(BM % (8 * WAVES) == 0 && BN % (8 * WAVES) == 0, "whole 8-row loads per wave");
+  constexpr int TM = WM / 32, TN = WN / 32;
+  constexpr int A_BYTES = BM * ROWB, B_BYTES = BN * ROWB;
+  constexpr int MAIN_BYTES = 2 * (A_BYTES + B_BYTES);
+  constexpr int SROW = WN + 4;
+  constexpr int STAGE_BYTES = WAVES * 32 * SROW * 4;
+  constexpr int SMEM_BYTES = MAIN_BYTES > STAGE_BYTES ? MAIN_BYTES : STAGE_BYTES;
+  constexpr int A_LOADS = BM / 8 / WAVES, B_LOADS = BN / 8 / WAVES;  // per wave per slice
+  // The array's size must not depend on the template parameters: with a
+  // dependent size, the host pass of this template emits no launch stub once
+  // the LDS-load builtin writes into the array.
+  static_assert(SMEM_BYTES <= DIRECT_SMEM_BYTES, "LDS of the direct-load kernel");
+  __shared__ __attribute__((aligned(16))) char smem[DIRECT_SMEM_BYTES];
+  auto Abuf = [&](int b) { return smem + b * A_BYTES; };
+  auto Bbuf = [&](int b) { return smem + 2 * A_BYTES + b * B_BYTES; };
+
+  const GemmArgs& e = p.e;
+  uint32_t tm, tn;
+  tile_coords(e, tm, tn);
+  const uint64_t m0 = (uint64_t)tm * BM, n0 = (uint64_t)tn * BN;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave / WGN, wn = wave % WGN;
+  const int l32 = lane & 31, lhi = lane >> 5;
+  const int lrow = lane >> 3, lpos = lane & 7;
+  const uint64_t Kp = p.Kp;
+
+  auto load_slice = [&](int b, uint64_t k0) {
+#pragma unroll
+    for (int i = 0; i < A_LOADS; ++i) {
+      const int r0 = (i * WAVES + wave) * 8, row = r0 + lrow;
+      const uint16_t* src = p.Ah + (m0 + row) * Kp + k0 + ((lpos ^ (row & 7)) << 3);
+      __builtin_amdgcn_global_load_lds(src, smem + b * A_BYTES + r0 * ROWB, 16, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < B_LOADS; ++i) {
+      const int r0 = (i * WAVES + wave) * 8, row = r0 + lrow;
+      const uint16_t* src = p.Bh + (n0 + row) * Kp + k0 + ((lpos ^ (row & 7)) << 3);
+      __builtin_amdgcn_global_load_lds(src, smem + 2 * A_BYTES + b * B_BYTES + r0 * ROWB, 16, 0, 0);
+    }
+  };
+  // fragment of tile row `row`, 16-byte chunk c (k = 8c .. 8c + 7 of the slice)
+  auto frag = [&](const char* buf, int row, int c) {
+    return *reinterpret_cast<const bf16x8_t*>(buf + row * ROWB + ((c ^ (row & 7)) << 4));
+  };
+
+  f32x16_t acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+  constexpr int KS = BK / 16;
+  constexpr int NM = TM * TN, NDS = TM + TN;
+  bf16x8_t fa[2][TM], fb[2][TN];
+  const int arow = wm * WM + l32, brow = wn * WN + l32;
+  auto frags = [&](int buf, int fi, int s) {
+#pragma unroll
+    for (int i = 0; i < TM; ++i) fa[fi][i] = frag(Abuf(buf), arow + i * 32, 2 * s + lhi);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) fb[fi][j] = frag(Bbuf(buf), brow + j * 32, 2 * s + lhi);
+  };
+
+  const uint64_t kb = p.part ? (uint64_t)blockIdx.y * p.kc : 0;
+  const uint64_t nk = ((p.part ? min(kb + p.kc, Kp) : Kp) - kb) / BK;
+  load_slice(0, kb);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  frags(0, 0, 0);
+  int cur = 0;
+  for (uint64_t kt = 0; kt < nk; ++kt) {
+    if (kt + 1 < nk) load_slice(cur ^ 1, kb + (kt + 1) * BK);
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int c = s & 1;
+      if (s + 1 < KS) frags(cur, c ^ 1, s + 1);
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[c][i], fb[c][j], acc[i][j], 0, 0, 0);
+      if (s + 1 < KS) {
+#pragma unroll
+        for (int q = 0; q < NDS; ++q) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
+        }
+        if constexpr (NM > NDS) __builtin_amdgcn_sched_group_barrier(0x008, NM - NDS, 0);
+      }
+    }
+    // the next slice has landed in LDS and every wave is done with this one
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    cur ^= 1;
+    if (kt + 1 < nk) frags(cur, 0, 0);
+  }
+  __syncthreads();  // every wave is done with the operand tiles before staging
+
+  // epilogue, as in k_gemm_packed
+  float* st = reinterpret_cast<float*>(smem) + wave * 32 * SROW;
+  constexpr int F4R = WN / 4, RPI = 64 / F4R;
+  const uint64_t nb = n0 + wn * WN;
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r)
+        st[((r / 4) * 8 + lhi * 4 + (r % 4)) * SROW + j * 32 + l32] = acc[i][j][r];
+    __syncthreads();
+    const uint64_t mb = m0 + wm * WM + i * 32;
+    const int cc = (lane % F4R) * 4;
+    for (int rr = lane / F4R; rr < 32; rr += RPI) {
+      const uint64_t m = mb + rr, n = nb + cc;
+      if (m < e.M && n < e.N) {
+        const float4 v = *reinterpret_cast<const float4*>(st + rr * SROW + cc);
+        if (p.part) store_part4(p.part + (uint64_t)blockIdx.y * e.M * e.N, e.N, m, n, v);
+        else epilogue4(e, m, n, v);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Tile choice of the packed kernel: 256 x 256 when both sides fill at least
 // one such tile (4 waves of 128 x 128), 128 x 128 otherwise and in split mode
 // (whose two BF16 parts double the LDS footprint).
@@ -750,9 +893,14 @@ template <int BM, int BN, int BK, bool SPLIT, int WGM, int WGN>
 static void launch_packed(dim3 grid, hipStream_t s, const PackedArgs& pa) {
   hipLaunchKernelGGL((k_gemm_packed<BM, BN, BK, SPLIT, WGM, WGN>), grid, dim3(64 * WGM * WGN), 0, s, pa);
 }
+template <int BM, int BN, int WGM, int WGN>
+static void launch_direct(dim3 grid, hipStream_t s, const PackedArgs& pa) {
+  hipLaunchKernelGGL((k_gemm_direct<BM, BN, WGM, WGN>), grid, dim3(64 * WGM * WGN), 0, s, pa);
+}
 struct PackedCfg {
   int bm, bn, bk, waves;
   PackedLaunch launch;
+  int direct = 0;  // operand slices loaded from global memory straight into LDS
 };
 static const PackedCfg BF16_CFGS[] = {
     {256, 256, 64, 4, launch_packed<256, 256, 64, false, 2, 2>},
@@ -760,6 +908,7 @@ static const PackedCfg BF16_CFGS[] = {
     {128, 128, 64, 4, launch_packed<128, 128, 64, false, 2, 2>},
     {256, 128, 64, 4, launch_packed<256, 128, 64, false, 2, 2>},
     {128, 256, 64, 4, launch_packed<128, 256, 64, false, 2, 2>},
+    {256, 256, 64, 8, launch_direct<256, 256, 2, 4>, 1},
 };
 static const PackedCfg SPLIT_CFGS[] = {
     {128, 128, 32, 4, launch_packed<128, 128, 32, true, 2, 2>},
@@ -1242,19 +1391,21 @@ int vkh_gemm_configs(int mode, int* count) {
   });
 }
 
-int vkh_gemm_config_info(int mode, int index, int* bm, int* bn, int* bk, int* waves) {
+int vkh_gemm_config_info(int mode, int index, int* bm, int* bn, int* bk, int* waves, int* direct) {
   return guarded([&] {
     VKH_REQUIRE(bm && bn && bk && waves, "null out");
     VKH_REQUIRE(mode >= VKH_GEMM_FP32 && mode <= VKH_GEMM_BF16X3, "unknown mode");
     if (mode == VKH_GEMM_FP32) {
       VKH_REQUIRE(index == 0, "configuration index out of range");
       *bm = Cfg<VKH_GEMM_FP32>::BM; *bn = Cfg<VKH_GEMM_FP32>::BN; *bk = Cfg<VKH_GEMM_FP32>::BK; *waves = 4;
+      if (direct) *direct = 0;
       return;
     }
     int n = 0;
     const PackedCfg* c = packed_cfgs(mode, n);
     VKH_REQUIRE(index >= 0 && index < n, "configuration index out of range");
     *bm = c[index].bm; *bn = c[index].bn; *bk = c[index].bk; *waves = c[index].waves;
+    if (direct) *direct = c[index].direct;
   });
 }
 

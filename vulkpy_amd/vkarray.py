@@ -1196,15 +1196,17 @@ def get_matmul_precision() -> str:
 
 def gemm_configs(precision: str) -> List[dict]:
     """Kernel configurations of the matrix products in a precision mode:
-    tile ``bm`` x ``bn``, k slice ``bk`` and waves per block."""
+    tile ``bm`` x ``bn``, k slice ``bk``, waves per block and ``direct``
+    (operand slices loaded from global memory straight into LDS)."""
     mode = GEMM_MODE[precision]
     n = ctypes.c_int()
     call("vkh_gemm_configs", mode, ctypes.byref(n))
     out = []
     for i in range(n.value):
-        v = [ctypes.c_int() for _ in range(4)]
+        v = [ctypes.c_int() for _ in range(5)]
         call("vkh_gemm_config_info", mode, i, *[ctypes.byref(x) for x in v])
-        out.append({"bm": v[0].value, "bn": v[1].value, "bk": v[2].value, "waves": v[3].value})
+        out.append({"bm": v[0].value, "bn": v[1].value, "bk": v[2].value, "waves": v[3].value,
+                    "direct": bool(v[4].value)})
     return out
 
 
