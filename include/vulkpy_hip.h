@@ -344,6 +344,21 @@ int vkh_gemm_ex(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
                 float beta, float* C, uint64_t ldc, const float* bias, int act,
                 uint16_t* c_bf16, const float* aux, uint64_t ldaux,
                 int aux_act);
+/* vkh_gemm_ex with op(A) also given as BF16 in the layout the BF16 packing
+ * pass writes: a_bf16 holds rows of K BF16 values (A = op(A), trans_a = 0),
+ * with M rounded up to 256 rows allocated, K a multiple of 64 -- for
+ * instance the c_bf16 output of the GEMM that produced A.  In BF16 mode,
+ * when the matrix-core path runs, A's packing pass is skipped; otherwise
+ * a_bf16 is ignored.  The result is identical either way.  No counterpart in
+ * the reference: it lets a captured training step hand one layer's
+ * activations (or input gradients) to the next product without packing
+ * them again. */
+int vkh_gemm_bf16a(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
+                   uint64_t M, uint64_t N, uint64_t K, float alpha,
+                   const float* A, uint64_t lda, const float* B, uint64_t ldb,
+                   float beta, float* C, uint64_t ldc, const float* bias,
+                   int act, uint16_t* c_bf16, const float* aux,
+                   uint64_t ldaux, int aux_act, const uint16_t* a_bf16);
 
 /* ---- nn fused kernels (F16, F19 and the nn package compositions) ------- */
 /* y = act(x) elementwise: ReLU = x.max(0.0) (nn/layers.py:185),

@@ -256,3 +256,37 @@ def test_short_k_aux_epilogue(gpu):
         out._set_job([g_dy, g_W, g_h])
         want = (dy * W) * (h > 0)
         np.testing.assert_array_equal(np.asarray(out), want.astype(np.float32))
+
+
+def test_prepacked_bf16_operand(gpu):
+    """vkh_gemm_bf16a with op(A) given as BF16 rows (M rounded up to 256,
+    K % 64 == 0) gives the same bits as packing A, in the products the
+    captured step hands activations through: plain, with bias and ReLU, and
+    split k; other modes and layouts ignore the BF16 copy."""
+    rng = np.random.default_rng(40)
+    for (M, N, K, act) in ((300, 200, 128, "none"), (300, 260, 256, "relu"), (64, 48, 8192, "none")):
+        a = rng.standard_normal((M, K)).astype(np.float32)
+        b = rng.standard_normal((N, K)).astype(np.float32)
+        bias = rng.standard_normal(N).astype(np.float32)
+        gA, gB, gbias = vk.Array(gpu, data=a), vk.Array(gpu, data=b), vk.Array(gpu, data=bias)
+        rows = -(-M // 256) * 256
+        a_bf = vk.U32Array(gpu, shape=(rows * K // 2,))       # rows x K BF16 values
+        _lib.call("vkh_to_bf16", gpu._handle, gA._ptr, a_bf._ptr, M * K)
+        outs = {}
+        for mode in ("bf16", "bf16x3", "fp32"):
+            for pre in (False, True):
+                C = vk.Array(gpu, shape=(M, N))
+                c_bf = vk.U32Array(gpu, shape=((M * N + 1) // 2,))
+                _lib.call("vkh_gemm_bf16a", gpu._handle, _lib.GEMM_MODE[mode], 0, 1, M, N, K, 1.0,
+                          gA._ptr, K, gB._ptr, K, 0.0, C._ptr, N, gbias._ptr, _lib.ACT[act],
+                          c_bf._ptr, None, 0, 0, a_bf._ptr if pre else None)
+                C._set_job([gA, gB, a_bf])
+                outs[(mode, pre)] = (np.asarray(C).copy(), np.asarray(c_bf).copy())
+            np.testing.assert_array_equal(outs[(mode, True)][0], outs[(mode, False)][0], err_msg=mode)
+            # the BF16 copy of C is the BF16 rounding of C
+            conv = vk.U32Array(gpu, shape=((M * N + 1) // 2,))
+            gC = vk.Array(gpu, data=outs[(mode, True)][0])
+            _lib.call("vkh_to_bf16", gpu._handle, gC._ptr, conv._ptr, M * N)
+            conv._set_job([gC])
+            np.testing.assert_array_equal(np.asarray(conv)[: M * N // 2],
+                                          outs[(mode, True)][1][: M * N // 2], err_msg=mode)

@@ -1025,7 +1025,11 @@ static void pack_operand(vkh_gpu* g, const float* src, uint64_t ld, bool trans, 
   check_launch("gemm pack");
 }
 
-static void gemm_packed(vkh_gpu* g, int mode, bool ta, bool tb, GemmArgs& a, int cfg) {
+// a_pre (BF16 mode, op(A) = A, K % PK_KALIGN == 0): op(A) already in the
+// packed layout -- BF16 rows of K elements, M rounded up to PK_MAXTILE rows --
+// so its packing pass is skipped (the bits are the ones it would write)
+static void gemm_packed(vkh_gpu* g, int mode, bool ta, bool tb, GemmArgs& a, int cfg,
+                        const uint16_t* a_pre = nullptr) {
   const bool split = mode == VKH_GEMM_BF16X3;
   int ncfg = 0;
   const PackedCfg* cfgs = packed_cfgs(mode, ncfg);
@@ -1034,18 +1038,19 @@ static void gemm_packed(vkh_gpu* g, int mode, bool ta, bool tb, GemmArgs& a, int
   const uint64_t Mp = ceil_div(a.M, s.bm) * s.bm, Np = ceil_div(a.N, s.bn) * s.bn;
   const uint64_t Kp = ceil_div(std::max<uint64_t>(a.K, 1), PK_KALIGN) * PK_KALIGN;
   uint16_t* base = pack_area(g, packed_bytes(mode, a.M, a.N, a.K));
-  uint16_t* Ah = base;
-  uint16_t* Al = split ? Ah + Mp * Kp : nullptr;
-  uint16_t* Bh = split ? Al + Mp * Kp : Ah + Mp * Kp;
+  const uint16_t* Ah = base;
+  uint16_t* Al = split ? base + Mp * Kp : nullptr;
+  uint16_t* Bh = split ? Al + Mp * Kp : base + Mp * Kp;
   uint16_t* Bl = split ? Bh + Np * Kp : nullptr;
   // op(A)[m][k] and op(B)^T[n][k] = op(B)[k][n]
   if (split) {
-    pack_operand<true>(g, a.A, a.lda, ta, a.M, a.K, Ah, Al, Mp, Kp);
+    pack_operand<true>(g, a.A, a.lda, ta, a.M, a.K, base, Al, Mp, Kp);
     pack_operand<true>(g, a.B, a.ldb, !tb, a.N, a.K, Bh, Bl, Np, Kp);
   } else {
-    pack_operand<false>(g, a.A, a.lda, ta, a.M, a.K, Ah, nullptr, Mp, Kp);
+    if (!a_pre) pack_operand<false>(g, a.A, a.lda, ta, a.M, a.K, base, nullptr, Mp, Kp);
     pack_operand<false>(g, a.B, a.ldb, !tb, a.N, a.K, Bh, nullptr, Np, Kp);
   }
+  if (a_pre) Ah = a_pre;
   a.tiles_m = (uint32_t)(Mp / s.bm);
   a.tiles_n = (uint32_t)(Np / s.bn);
   const uint64_t nb = (uint64_t)a.tiles_m * a.tiles_n;
@@ -1340,7 +1345,8 @@ static void gemm_smallk(vkh_gpu* g, bool ta, bool tb, const GemmArgs& a) {
 static void gemm_impl(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M, uint64_t N,
                       uint64_t K, float alpha, const float* A, uint64_t lda, const float* B,
                       uint64_t ldb, float beta, float* Cp, uint64_t ldc, const float* bias, int act,
-                      uint16_t* c_bf16, const float* aux, uint64_t ldaux, int aux_act) {
+                      uint16_t* c_bf16, const float* aux, uint64_t ldaux, int aux_act,
+                      const uint16_t* a_bf16 = nullptr) {
   use(g);
   if (M == 0 || N == 0) return;
   VKH_REQUIRE(Cp, "gemm: null C");
@@ -1375,7 +1381,9 @@ static void gemm_impl(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M
     case VKH_GEMM_BF16:
     case VKH_GEMM_BF16X3: {
       auto it = g->gemm_cfg.find(cfg_key(mode, trans_a, trans_b, M, N, K));
-      gemm_packed(g, mode, trans_a, trans_b, a, it != g->gemm_cfg.end() ? it->second : default_cfg(mode, M, N, K));
+      const bool pre = a_bf16 && mode == VKH_GEMM_BF16 && !trans_a && K % PK_KALIGN == 0;
+      gemm_packed(g, mode, trans_a, trans_b, a, it != g->gemm_cfg.end() ? it->second : default_cfg(mode, M, N, K),
+                  pre ? a_bf16 : nullptr);
       break;
     }
     default: fail(VKH_INVALID, "gemm: unknown mode");
@@ -1531,6 +1539,16 @@ int vkh_gemm_ex(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M, uint
   return guarded([&] {
     gemm_impl(g, mode, trans_a, trans_b, M, N, K, alpha, A, lda, B, ldb, beta, Cp, ldc, bias, act,
               c_bf16, aux, ldaux, aux_act);
+  });
+}
+
+int vkh_gemm_bf16a(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M, uint64_t N, uint64_t K,
+                   float alpha, const float* A, uint64_t lda, const float* B, uint64_t ldb, float beta,
+                   float* Cp, uint64_t ldc, const float* bias, int act, uint16_t* c_bf16,
+                   const float* aux, uint64_t ldaux, int aux_act, const uint16_t* a_bf16) {
+  return guarded([&] {
+    gemm_impl(g, mode, trans_a, trans_b, M, N, K, alpha, A, lda, B, ldb, beta, Cp, ldc, bias, act,
+              c_bf16, aux, ldaux, aux_act, a_bf16);
   });
 }
 

@@ -33,7 +33,7 @@ from typing import Iterable, List, Optional, Tuple
 import numpy as np
 
 from vulkpy_amd._lib import ACT, BINOP, GEMM_MODE, UNOP, call
-from vulkpy_amd.vkarray import Array
+from vulkpy_amd.vkarray import Array, _DeviceBuffer
 from .core import Loss, Module
 from .layers import Dense, ReLU, Sigmoid
 from .losses import CrossEntropyLoss, HuberLoss, MSELoss, SoftmaxCrossEntropyLoss
@@ -66,6 +66,13 @@ def _units(layers) -> List[_Unit]:
     if not units:
         raise ValueError("GraphSequence needs at least one Dense layer")
     return units
+
+
+def _packs_a(M: int, N: int, K: int) -> bool:
+    """Whether an M x N x K product in BF16 mode packs its A operand in a
+    layout a BF16 copy of A can stand in for: the matrix-core path (neither
+    the skinny nor the short-k vector path) with K a multiple of 64."""
+    return min(M, N) > 16 and K > 16 and K % 64 == 0
 
 
 class _Plan:
@@ -112,6 +119,23 @@ class _Plan:
         self.g_out = Array(gpu, shape=(B, out_dim)) if units[-1].act else None
         self.L_rows = Array(gpu, shape=(B,))
         self.loss = Array(gpu, shape=())
+        # BF16 mode: BF16 copies of the activations hs[i] and input gradients
+        # ds[i] that the next product would pack, written by the epilogue of
+        # the product that makes them (c_bf16) and taken by the consumer as
+        # its packed operand (vkh_gemm_bf16a), which skips that packing pass
+        # with the same bits
+        self.hs_bf: List[Optional[_DeviceBuffer]] = [None] * len(units)
+        self.ds_bf: List[Optional[_DeviceBuffer]] = [None] * len(units)
+        if self.mode == GEMM_MODE["bf16"]:
+            rows = -(-B // 256) * 256
+            for i in range(len(units) - 1):   # consumer: forward product of unit i + 1
+                k = units[i].dense.output_dim
+                if _packs_a(B, units[i + 1].dense.output_dim, k):
+                    self.hs_bf[i] = _DeviceBuffer(gpu, rows * k, np.uint16)
+            for i in range(1, len(units) - 1):  # consumer: dX product of unit i
+                k = units[i].dense.output_dim
+                if _packs_a(B, units[i].dense.input_dim, k):
+                    self.ds_bf[i] = _DeviceBuffer(gpu, rows * k, np.uint16)
         # parameters, their optimizer states and device beta powers
         self.params: List[Parameter] = []
         for u in units:
@@ -138,13 +162,14 @@ class _Plan:
         h, mode, B = self.h, self.mode, self.B
         units = self.units
         # forward
-        src = self.x
-        for u, hout in zip(units, self.hs):
+        src, src_bf = self.x, None
+        for i, (u, hout) in enumerate(zip(units, self.hs)):
             d = u.dense
-            call("vkh_gemm", h, mode, 0, 1, B, d.output_dim, d.input_dim, 1.0, src._ptr, d.input_dim,
+            hb = self.hs_bf[i]
+            call("vkh_gemm_bf16a", h, mode, 0, 1, B, d.output_dim, d.input_dim, 1.0, src._ptr, d.input_dim,
                  d.w.value._ptr, d.input_dim, 0.0, hout._ptr, d.output_dim, d.b.value._ptr,
-                 ACT[u.act], None)
-            src = hout
+                 ACT[u.act], hb.ptr if hb else None, None, 0, 0, src_bf.ptr if src_bf else None)
+            src, src_bf = hout, hb
         out = self.hs[-1]
         O = units[-1].dense.output_dim
         # loss rows, batch reduction and gradient w.r.t. the network output
@@ -186,10 +211,12 @@ class _Plan:
                     call("vkh_bias_grad", h, dy._ptr, p.grad._ptr, B, d.output_dim, beta)
             if i > 0:
                 prev = units[i - 1]
-                call("vkh_gemm_ex", h, mode, 0, 0, B, d.input_dim, d.output_dim, 1.0, dy._ptr,
+                out_bf, dy_bf = self.ds_bf[i - 1], self.ds_bf[i]
+                call("vkh_gemm_bf16a", h, mode, 0, 0, B, d.input_dim, d.output_dim, 1.0, dy._ptr,
                      d.output_dim, d.w.value._ptr, d.input_dim, 0.0, self.ds[i - 1]._ptr,
-                     d.input_dim, None, 0, None,
-                     self.hs[i - 1]._ptr if prev.act else None, d.input_dim, ACT[prev.act])
+                     d.input_dim, None, 0, out_bf.ptr if out_bf else None,
+                     self.hs[i - 1]._ptr if prev.act else None, d.input_dim, ACT[prev.act],
+                     dy_bf.ptr if dy_bf else None)
         # optimizer
         for p in self.params:
             st = p.opt_state
