@@ -136,7 +136,7 @@ def build_suite(vk, gpu, A, B, col, row, P, rng_gen, ubuf, fbuf, timer):
     return ops
 
 
-def mlp_compare(vk, gpu, steps: int, warmup: int = 2):
+def mlp_compare(vk, gpu, steps: int, warmup: int = 2, uncaptured: bool = False):
     """
     Regression MLP Dense(256->4096)-ReLU-Dense(4096->4096)-ReLU-Dense(4096->1),
     HuberLoss(delta=1), Ridge(1e-4) on every weight (applied, see
@@ -151,7 +151,8 @@ def mlp_compare(vk, gpu, steps: int, warmup: int = 2):
     the steps after the first `warmup` (the graph arms tune and capture on
     their first step).  Besides the final training loss, each arm's trained
     weights are evaluated in FP32 on a held-out batch of the same
-    distribution.
+    distribution.  With `uncaptured` only the BF16 and split-BF16 steps run,
+    their kernels launched one at a time instead of as a graph replay.
     """
     from vulkpy_amd import nn
     B, D, H = 16384, 256, 4096
@@ -222,6 +223,13 @@ def mlp_compare(vk, gpu, steps: int, warmup: int = 2):
             ("bf16x3_graph", lambda: nn.GraphSequence(layers("bf16x3"), nn.HuberLoss(),
                                                       precision="bf16x3", regularize=True,
                                                       autotune=True), None))
+    if uncaptured:
+        # the graph arms' kernels launched one at a time (for kernel traces)
+        arms = tuple((name.replace("graph", "uncaptured"),
+                      lambda p=name.split("_")[0]: nn.GraphSequence(
+                          layers(p), nn.HuberLoss(), precision=p, regularize=True,
+                          autotune=True, capture=False), None)
+                     for name, _, _ in arms[2:])
     res = {}
     for name, model, hook in arms:
         log(f"bench: MLP {name} {steps} steps")
@@ -244,6 +252,9 @@ def mlp_compare(vk, gpu, steps: int, warmup: int = 2):
         "timed_steps": steps - warmup,
         "gemm_flop_per_step": 2 * (B * H * D + B * H * H + B * H) * 2 + 2 * (B * H * H + B * H),
     }
+    if uncaptured:
+        out.update(res)
+        return out
     ref = res["fp32"]
     for name, r in res.items():
         out[name] = r
@@ -286,9 +297,7 @@ def gemm_report(vk, gpu, timer, reps: int = 3, sample_every: int = 128):
             c64 = a64 @ b64
             den = np.abs(a64) @ np.abs(b64)
             for mode in ("fp32", "bf16", "bf16x3"):
-                tuned = None
-                if mode != "fp32":
-                    tuned = vk.autotune_gemm(gpu, M, Nn, K, precision=mode)
+                tuned = vk.autotune_gemm(gpu, M, Nn, K, precision=mode)
                 C = A.matmul(Bm, precision=mode)
                 gpu.wait()
                 # one untimed product keeps the GPU busy while the timed ones
@@ -310,8 +319,8 @@ def gemm_report(vk, gpu, timer, reps: int = 3, sample_every: int = 128):
                     "rel_err_max": float(f"{rel.max():.3e}"),
                     "rel_err_mean": float(f"{rel.mean():.3e}"),
                     "normwise_err": float(f"{np.linalg.norm(c - c64) / np.linalg.norm(c64):.3e}"),
-                    "config": (vk.gemm_configs(mode)[tuned["best"]] if tuned else None),
-                    "autotune_ms": tuned["ms"] if tuned else None,
+                    "config": vk.gemm_configs(mode)[tuned["best"]],
+                    "autotune_ms": tuned["ms"],
                 })
                 log(f"bench: GEMM {out[-1]}")
                 del C
@@ -425,10 +434,16 @@ def main() -> int:
     ap.add_argument("--no-gemm", action="store_true", help="skip the GEMM precision report")
     ap.add_argument("--no-c2", action="store_true",
                     help="skip the bandwidth suite (and the CPU baseline that samples it)")
-    ap.add_argument("--gemm-pmc", action="store_true",
-                    help="only run five 8192^3 BF16 products (for counter runs) and exit")
+    ap.add_argument("--gemm-pmc", nargs="?", const="bf16", default=None,
+                    choices=("fp32", "bf16", "bf16x3"),
+                    help="only run five 8192^3 products in this precision (default bf16; for "
+                         "counter runs) and exit")
     ap.add_argument("--pmc-kernels", action="store_true",
                     help="only launch the copy and dominant kernels (for counter runs) and exit")
+    ap.add_argument("--mlp-trace", type=int, default=0, metavar="S",
+                    help="only run S steps of the BF16 and split-BF16 MLP steps with their "
+                         "kernels launched one at a time, not as graph replays (for kernel "
+                         "traces), and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -447,9 +462,12 @@ def main() -> int:
         A = vk.Array(gpu, data=rng.standard_normal((8192, 8192), dtype=np.float32))
         Bm = vk.Array(gpu, data=rng.standard_normal((8192, 8192), dtype=np.float32))
         for _ in range(5):
-            A.matmul(Bm, precision="bf16")
+            A.matmul(Bm, precision=args.gemm_pmc)
         gpu.wait()
-        print(json.dumps({"gemm_pmc": "5 x BF16 8192^3"}))
+        print(json.dumps({"gemm_pmc": f"5 x {args.gemm_pmc} 8192^3"}))
+        return 0
+    if args.mlp_trace > 0:
+        print(json.dumps({"mlp_trace": mlp_compare(vk, gpu, args.mlp_trace, uncaptured=True)}))
         return 0
 
     c2 = None

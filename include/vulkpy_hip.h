@@ -288,9 +288,10 @@ int vkh_box_muller(vkh_gpu* gpu, const float* u, float* out, uint64_t n,
  * batch_affine.comp (nn/layers.py:94-101).
  * Paths: min(M, N) <= 16 (skinny) and K <= 16 (short k) are memory-bound and
  * run in FP32 on the vector units in every mode; otherwise FP32 mode uses
- * FP32 matrix cores and the BF16 modes pack their operands for the BF16
- * matrix cores, splitting k into chunks (partials summed in chunk order)
- * when fewer than 128 tiles of 256 x 256 cover C and K >= 4096. */
+ * FP32 matrix cores on the operands as stored, and the BF16 modes pack their
+ * operands for the BF16 matrix cores, splitting k into chunks (partials
+ * summed in chunk order) when fewer than 128 tiles of 256 x 256 cover C and
+ * K >= 4096. */
 int vkh_gemm(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
              uint64_t M, uint64_t N, uint64_t K, float alpha,
              const float* A, uint64_t lda, const float* B, uint64_t ldb,
@@ -308,10 +309,11 @@ int vkh_gemm_reserve(vkh_gpu* gpu, int mode, uint64_t M, uint64_t N,
                      uint64_t K);
 
 /* ---- GEMM autotuning (no counterpart in the reference) -------------------
- * The BF16 and split-BF16 products have several kernel configurations (tile
- * BM x BN, k slice BK, waves per block); FP32 has one.  A context keeps one
- * chosen configuration per (mode, trans_a, trans_b, M, N, K); without one a
- * shape uses the built-in default.  Products with min(M, N) <= 16 or
+ * Every mode has several kernel configurations (tile BM x BN, k slice BK,
+ * waves per block).  Within a mode every configuration sums each output's k
+ * in the same order, so the choice never changes a result.  A context keeps
+ * one chosen configuration per (mode, trans_a, trans_b, M, N, K); without
+ * one a shape uses the built-in default.  Products with min(M, N) <= 16 or
  * K <= 16 use the vector kernels and no configuration. */
 int vkh_gemm_configs(int mode, int* count);
 /* direct (optional): 1 when the configuration loads operand slices from

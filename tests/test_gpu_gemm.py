@@ -130,13 +130,21 @@ def test_skinny_epilogue(gpu, M, N, K, ta, tb):
     np.testing.assert_allclose(np.asarray(gC), ref * (aux > 0), rtol=1e-5, atol=1e-5)
 
 
-@pytest.mark.parametrize("mode", ["bf16", "bf16x3"])
-@pytest.mark.parametrize("ta,tb", [(0, 1), (1, 0)])
-@pytest.mark.parametrize("M,N,K", [(300, 260, 200), (300, 200, 8192)])
+@pytest.mark.parametrize("mode,ta,tb,M,N,K", [
+    (mode, ta, tb, M, N, K)
+    for mode in ("bf16", "bf16x3") for ta, tb in ((0, 1), (1, 0))
+    for M, N, K in ((300, 260, 200), (300, 200, 8192))
+] + [
+    # FP32: every layout, 16-byte runs (first shape) and element loads (odd
+    # extents, second shape)
+    ("fp32", ta, tb, M, N, K)
+    for ta in (0, 1) for tb in (0, 1) for M, N, K in ((300, 260, 200), (301, 259, 203))
+])
 def test_configurations_bit_identical(gpu, mode, ta, tb, M, N, K):
     """Every kernel configuration accumulates each output over k in the
-    same order, so tuning never changes a result (the second shape splits k
-    into 8 chunks, whose bounds do not depend on the configuration)."""
+    same order, so tuning never changes a result (in BF16 modes the shape
+    with K = 8192 splits k into 8 chunks, whose bounds do not depend on the
+    configuration)."""
     rng = np.random.default_rng(9)
     A = rng.standard_normal((K, M) if ta else (M, K)).astype(np.float32)
     B = rng.standard_normal((N, K) if tb else (K, N)).astype(np.float32)
@@ -162,7 +170,11 @@ def test_autotune_api(gpu):
     _lib.call("vkh_gemm_get_config", gpu._handle, _lib.GEMM_MODE["bf16"], 0, 1, 512, 384, 256,
               ctypes.byref(idx), ctypes.byref(tuned))
     assert (idx.value, tuned.value) == (r["best"], 1)
-    # skinny and FP32 shapes have a single kernel choice
+    r = vk.autotune_gemm(gpu, 640, 512, 384, precision="fp32", trans_a=True, reps=2)
+    assert len(r["ms"]) == len(vk.gemm_configs("fp32")) >= 2
+    assert all(t > 0 for t in r["ms"]) and r["ms"][r["best"]] == min(r["ms"])
+    _lib.call("vkh_gemm_set_config", gpu._handle, _lib.GEMM_MODE["fp32"], 1, 0, 640, 512, 384, -1)
+    # skinny shapes have a single kernel choice
     assert vk.autotune_gemm(gpu, 4096, 1, 512, precision="bf16")["ms"][0] == -1
     with pytest.raises(ValueError):
         _lib.call("vkh_gemm_set_config", gpu._handle, _lib.GEMM_MODE["bf16"], 0, 0, 8, 8, 8, 99)

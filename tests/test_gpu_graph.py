@@ -143,6 +143,32 @@ def test_graph_split_k_matches_eager(gpu, precision):
             np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("precision", ["bf16", "bf16x3", "fp32"])
+def test_graph_uncaptured_matches_captured(gpu, precision):
+    """capture=False issues the step's kernels one launch at a time: the
+    losses and weights equal the captured step's after every step, BF16
+    hand-off buffers included."""
+    dims, acts = [(64, 320), (320, 192), (192, 3)], ["relu", "relu", None]
+    B = 512
+    rng = np.random.default_rng(17)
+    x = rng.standard_normal((B, 64)).astype(np.float32)
+    y = rng.standard_normal((B, 3)).astype(np.float32)
+    X, Y = vk.Array(gpu, data=x), vk.Array(gpu, data=y)
+    cap_layers = make_net(gpu, dims, acts, 90, lambda: nn.Ridge(1e-4), precision=precision)
+    unc_layers = make_net(gpu, dims, acts, 90, lambda: nn.Ridge(1e-4), precision=precision)
+    cap = nn.GraphSequence(cap_layers, nn.HuberLoss(), precision=precision, regularize=True)
+    unc = nn.GraphSequence(unc_layers, nn.HuberLoss(), precision=precision, regularize=True,
+                           capture=False)
+    for step in range(3):
+        _, lc = cap.train(X, Y)
+        _, lu = unc.train(X, Y)
+        gpu.wait()
+        assert unc._plan.graph is None
+        assert np.asarray(lc).item() == np.asarray(lu).item(), step
+        for a, b in zip(weights(cap_layers), weights(unc_layers)):
+            np.testing.assert_array_equal(a, b)
+
+
 def test_graph_rejects_unsupported(gpu):
     with pytest.raises(ValueError):
         nn.GraphSequence([nn.ReLU()], nn.MSELoss())

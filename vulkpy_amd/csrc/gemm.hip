@@ -34,25 +34,7 @@ __device__ __forceinline__ uint16_t f2bf(float x) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-constexpr int GEMM_THREADS = 256;
-
-template <int MODE>
-struct Cfg;
-template <>
-struct Cfg<VKH_GEMM_FP32> {
-  static constexpr int BM = 128, BN = 128, BK = 16, PAD = 4;
-  using T = float;  // LDS element
-};
-template <>
-struct Cfg<VKH_GEMM_BF16> {
-  static constexpr int BM = 128, BN = 128, BK = 32, PAD = 8;
-  using T = uint16_t;
-};
-template <>
-struct Cfg<VKH_GEMM_BF16X3> {
-  static constexpr int BM = 128, BN = 128, BK = 32, PAD = 8;
-  using T = uint16_t;
-};
+typedef __attribute__((ext_vector_type(4))) float f32x4_t;
 
 struct GemmArgs {
   const float* A;
@@ -103,84 +85,6 @@ __device__ __forceinline__ float4 load_row4(const float* row, uint64_t c, uint64
   return v;
 }
 
-// Staging of one operand tile (ROWS x BK) into LDS laid out [ROWS][BK+PAD]
-// with K contiguous.  KMAJOR: the operand is stored with K contiguous
-// (A row-major [M,K], or B stored [N,K]); otherwise rows are contiguous along
-// the tile's row dimension (A stored [K,M], or B row-major [K,N]).
-template <int MODE, int ROWS, bool KMAJOR>
-struct Stager {
-  using C = Cfg<MODE>;
-  static constexpr int BK = C::BK;
-  static constexpr int NV = ROWS * BK / 4 / GEMM_THREADS;  // float4 per thread
-  float4 r[NV];
-
-  __device__ __forceinline__ void load(const float* P, uint64_t ld, uint64_t row0, uint64_t nrows,
-                                       uint64_t k0, uint64_t K, bool vec) {
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      const int idx = threadIdx.x + i * GEMM_THREADS;
-      if (KMAJOR) {
-        const int row = idx / (BK / 4), c4 = idx % (BK / 4);
-        const uint64_t gr = row0 + row;
-        if (gr < nrows) r[i] = load_row4(P + gr * ld, k0 + c4 * 4, K, vec);
-        else r[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-      } else {
-        const int kr = idx / (ROWS / 4), c4 = idx % (ROWS / 4);
-        const uint64_t gk = k0 + kr;
-        if (gk < K) r[i] = load_row4(P + gk * ld, row0 + c4 * 4, nrows, vec);
-        else r[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-      }
-    }
-  }
-
-  // hi (and lo for split mode) tiles
-  __device__ __forceinline__ void store(typename C::T* hi, typename C::T* lo) {
-    constexpr int LD = BK + C::PAD;
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      const int idx = threadIdx.x + i * GEMM_THREADS;
-      const float v[4] = {r[i].x, r[i].y, r[i].z, r[i].w};
-      int base, step;
-      if (KMAJOR) {
-        const int row = idx / (BK / 4), c4 = idx % (BK / 4);
-        base = row * LD + c4 * 4;
-        step = 1;
-      } else {
-        const int kr = idx / (ROWS / 4), c4 = idx % (ROWS / 4);
-        base = (c4 * 4) * LD + kr;
-        step = LD;
-      }
-      if constexpr (MODE == VKH_GEMM_FP32) {
-        if (KMAJOR) *reinterpret_cast<float4*>(&hi[base]) = r[i];
-        else
-#pragma unroll
-          for (int q = 0; q < 4; ++q) hi[base + q * step] = v[q];
-      } else {
-        uint16_t h[4], l[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          h[q] = f2bf(v[q]);
-          if constexpr (MODE == VKH_GEMM_BF16X3) l[q] = f2bf(v[q] - bf2f(h[q]));
-        }
-        if (KMAJOR) {
-          uint2 hv = make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16));
-          *reinterpret_cast<uint2*>(&hi[base]) = hv;
-          if constexpr (MODE == VKH_GEMM_BF16X3) {
-            uint2 lv = make_uint2((uint32_t)l[0] | ((uint32_t)l[1] << 16), (uint32_t)l[2] | ((uint32_t)l[3] << 16));
-            *reinterpret_cast<uint2*>(&lo[base]) = lv;
-          }
-        } else {
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            hi[base + q * step] = h[q];
-            if constexpr (MODE == VKH_GEMM_BF16X3) lo[base + q * step] = l[q];
-          }
-        }
-      }
-    }
-  }
-};
-
 // XCD-aware tile order: consecutive remapped ids share an XCD, then tiles are
 // grouped by GROUP_M rows of tiles.
 __device__ __forceinline__ void tile_coords(const GemmArgs& p, uint32_t& tm, uint32_t& tn) {
@@ -194,133 +98,6 @@ __device__ __forceinline__ void tile_coords(const GemmArgs& p, uint32_t& tm, uin
   const uint32_t gsize = min(p.tiles_m - first_m, GROUP_M);
   tm = first_m + (b % width) % gsize;
   tn = (b % width) / gsize;
-}
-
-template <int MODE, bool TA, bool TB>
-__global__ void __launch_bounds__(GEMM_THREADS, 2)
-k_gemm(GemmArgs p, int vec_a, int vec_b) {
-  using C = Cfg<MODE>;
-  using T = typename C::T;
-  constexpr int BM = C::BM, BN = C::BN, BK = C::BK, LD = BK + C::PAD;
-  constexpr bool SPLIT = MODE == VKH_GEMM_BF16X3;
-  __shared__ __attribute__((aligned(16))) T As[BM * LD];
-  __shared__ __attribute__((aligned(16))) T Bs[BN * LD];
-  __shared__ __attribute__((aligned(16))) T Asl[SPLIT ? BM * LD : 1];
-  __shared__ __attribute__((aligned(16))) T Bsl[SPLIT ? BN * LD : 1];
-
-  uint32_t tm, tn;
-  tile_coords(p, tm, tn);
-  const uint64_t m0 = (uint64_t)tm * BM, n0 = (uint64_t)tn * BN;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = wave >> 1, wn = wave & 1;  // 2 x 2 waves, 64 x 64 each
-  const int l32 = lane & 31, lhi = lane >> 5;
-
-  Stager<MODE, BM, !TA> sa;
-  Stager<MODE, BN, TB> sb;
-
-  f32x16_t acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
-
-  const uint64_t K = p.K;
-  sa.load(p.A, p.lda, m0, p.M, 0, K, vec_a);
-  sb.load(p.B, p.ldb, n0, p.N, 0, K, vec_b);
-
-  for (uint64_t k0 = 0; k0 < K; k0 += BK) {
-    __syncthreads();  // previous slice fully consumed
-    sa.store(As, Asl);
-    sb.store(Bs, Bsl);
-    __syncthreads();
-    if (k0 + BK < K) {  // prefetch the next slice while the MFMAs run
-      sa.load(p.A, p.lda, m0, p.M, k0 + BK, K, vec_a);
-      sb.load(p.B, p.ldb, n0, p.N, k0 + BK, K, vec_b);
-    }
-    if constexpr (MODE == VKH_GEMM_FP32) {
-#pragma unroll
-      for (int ks = 0; ks < BK; ks += 2) {
-        float a[2], b[2];
-#pragma unroll
-        for (int i = 0; i < 2; ++i) a[i] = As[(wm * 64 + i * 32 + l32) * LD + ks + lhi];
-#pragma unroll
-        for (int j = 0; j < 2; ++j) b[j] = Bs[(wn * 64 + j * 32 + l32) * LD + ks + lhi];
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int j = 0; j < 2; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b[j], acc[i][j], 0, 0, 0);
-      }
-    } else {
-#pragma unroll
-      for (int ks = 0; ks < BK; ks += 16) {
-        bf16x8_t a[2], b[2], al[2], bl[2];
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-          const int off = (wm * 64 + i * 32 + l32) * LD + ks + lhi * 8;
-          a[i] = *reinterpret_cast<const bf16x8_t*>(&As[off]);
-          if constexpr (SPLIT) al[i] = *reinterpret_cast<const bf16x8_t*>(&Asl[off]);
-        }
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          const int off = (wn * 64 + j * 32 + l32) * LD + ks + lhi * 8;
-          b[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[off]);
-          if constexpr (SPLIT) bl[j] = *reinterpret_cast<const bf16x8_t*>(&Bsl[off]);
-        }
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int j = 0; j < 2; ++j) {
-            if constexpr (SPLIT) {
-              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], b[j], acc[i][j], 0, 0, 0);
-              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], bl[j], acc[i][j], 0, 0, 0);
-            }
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-          }
-      }
-    }
-  }
-
-  // epilogue: acc[i][j][r] is C[m, n] with
-  //   m = m0 + wm*64 + i*32 + (r/4)*8 + lhi*4 + r%4,  n = n0 + wn*64 + j*32 + l32
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const uint64_t n = n0 + wn * 64 + j * 32 + l32;
-      if (n >= p.N) continue;
-      const float bn = p.bias ? p.bias[n] : 0.f;
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const uint64_t m = m0 + wm * 64 + i * 32 + (r / 4) * 8 + lhi * 4 + (r % 4);
-        if (m >= p.M) continue;
-        float v = p.alpha * acc[i][j][r];
-        float* cp = p.C + m * p.ldc + n;
-        if (p.beta != 0.f) v += p.beta * *cp;
-        if (p.bias) v += bn;
-        v = act_apply(p.act, v);
-        if (p.aux) v = act_deriv_mul(p.aux_act, p.aux[m * p.ldaux + n], v);
-        *cp = v;
-        if (p.Cbf) p.Cbf[m * p.ldc + n] = f2bf(v);
-      }
-    }
-}
-
-template <int MODE>
-static void launch_mode(vkh_gpu* g, bool ta, bool tb, GemmArgs& a, int va, int vb) {
-  using C = Cfg<MODE>;
-  a.tiles_m = (uint32_t)ceil_div(a.M, C::BM);
-  a.tiles_n = (uint32_t)ceil_div(a.N, C::BN);
-  const uint64_t nb = (uint64_t)a.tiles_m * a.tiles_n;
-  VKH_REQUIRE(nb < (uint64_t(1) << 31), "GEMM too large");
-  dim3 grid((unsigned)nb), block(GEMM_THREADS);
-  if (!ta && !tb) hipLaunchKernelGGL((k_gemm<MODE, false, false>), grid, block, 0, g->stream, a, va, vb);
-  else if (!ta && tb) hipLaunchKernelGGL((k_gemm<MODE, false, true>), grid, block, 0, g->stream, a, va, vb);
-  else if (ta && !tb) hipLaunchKernelGGL((k_gemm<MODE, true, false>), grid, block, 0, g->stream, a, va, vb);
-  else hipLaunchKernelGGL((k_gemm<MODE, true, true>), grid, block, 0, g->stream, a, va, vb);
-  check_launch("gemm");
 }
 
 // ---- packed BF16 path ---------------------------------------------------------------
@@ -548,6 +325,286 @@ __device__ __forceinline__ void store_part4(float* part, uint64_t N, uint64_t m,
   }
 }
 
+// Epilogue of a block whose waves each hold TM x TN 32 x 32 accumulator
+// blocks of a WM x WN sub-tile at (wm, wn): acc[i][j][r] is C[m, n] with
+// m = m0 + wm*WM + i*32 + (r/4)*8 + lhi*4 + r%4 and n = n0 + wn*WN + j*32 +
+// l32.  Each wave writes rows i*32 .. i*32+31 of its sub-tile to its own
+// 32 x (WN+4) float staging slice of `smem`, then reads them back as 4-float
+// row segments (WN/4 lanes per row), so C is accessed in whole row segments.
+// With `part` the raw sums go to that k chunk's partial matrix instead.  The
+// caller's operand tiles in `smem` must be dead (a barrier before the call).
+template <int TM, int TN, int WM, int WN>
+__device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM][TN], const GemmArgs& e,
+                                                float* part, uint64_t m0, uint64_t n0, int wm, int wn) {
+  constexpr int SROW = WN + 4, F4R = WN / 4, RPI = 64 / F4R;
+  static_assert(64 % F4R == 0, "whole rows per wave pass");
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int l32 = lane & 31, lhi = lane >> 5;
+  float* st = smem + wave * 32 * SROW;
+  const uint64_t nb = n0 + wn * WN;
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r)
+        st[((r / 4) * 8 + lhi * 4 + (r % 4)) * SROW + j * 32 + l32] = acc[i][j][r];
+    __syncthreads();
+    const uint64_t mb = m0 + wm * WM + i * 32;
+    const int cc = (lane % F4R) * 4;
+    for (int rr = lane / F4R; rr < 32; rr += RPI) {
+      const uint64_t m = mb + rr, n = nb + cc;
+      if (m < e.M && n < e.N) {
+        const float4 v = *reinterpret_cast<const float4*>(st + rr * SROW + cc);
+        if (part) store_part4(part, e.N, m, n, v);
+        else epilogue4(e, m, n, v);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---- FP32 path ----------------------------------------------------------------------
+// FP32 products on v_mfma_f32_32x32x2_f32 with the operands read as stored
+// (no packing pass).  A BM x BN tile per block, WGM x WGN waves of
+// (BM/WGM) x (BN/WGN); 32-deep k slices go through registers into
+// double-buffered LDS with one barrier per slice, the next slice's global
+// loads in flight while the current one is multiplied.  Each operand keeps
+// its global layout in LDS:
+//   * K-major (A row-major [M][K], B stored [N][K]): rows of 32 floats
+//     (128 B) whose 16-byte chunks are XOR-swizzled by row (chunk c of row r
+//     sits at position c ^ (r & 7)), so 8 consecutive rows read at one chunk
+//     touch 8 distinct bank groups;
+//   * otherwise (A stored [K][M], B row-major [K][N]): 32 k rows of the tile
+//     width plus 8 floats, so the two half-waves of a fragment read (k rows 4
+//     apart) fall on different banks.
+// The MFMA takes one float of A and one of B per lane: lane (r, h), r =
+// lane % 32, h = lane / 32, supplies row (column) r at its k.  In the 8-deep
+// k group g of a slice, MFMA t = 0..3 takes k = 8g + 4h + t, so a K-major
+// operand feeds the whole group with one 16-byte read per 32-row block.
+// Every output sums its k in this fixed order; slices beyond K are zeros,
+// so K needs no padding.
+//
+// Operand loads never branch per lane: a lane whose element is out of range
+// loads from a block of zeros instead, so the compiler issues every load of
+// a slice back to back and waits for them only at the LDS store.  (With
+// per-lane guarded loads it waits for each load before issuing the next.)
+constexpr int F32_BK = 32;
+
+static __device__ __attribute__((aligned(16))) float k_f32_zeros[4];
+
+template <int ROWS, bool KMAJ, int THREADS>
+struct F32Stager {
+  static constexpr int NV = ROWS * F32_BK / 4 / THREADS;  // float4 per thread
+  static_assert(NV * 4 * THREADS == ROWS * F32_BK, "whole float4 per thread");
+  static constexpr int LD = KMAJ ? F32_BK : ROWS + 8;     // LDS row (floats)
+  static constexpr int FLOATS = KMAJ ? ROWS * F32_BK : F32_BK * LD;
+  f32x4_t r[NV];
+
+  // Rows row0.. (K-major: the operand's rows; otherwise its columns) at k0,
+  // as runs of 4 elements along the contiguous dimension.  VEC: the operand
+  // is 16-byte aligned, its leading dimension and its contiguous extent are
+  // multiples of 4, so a run is wholly in or out of range.
+  template <bool VEC>
+  __device__ __forceinline__ void load(const float* P, uint64_t ld, uint64_t row0, uint64_t nrows,
+                                       uint64_t k0, uint64_t K) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int idx = threadIdx.x + i * THREADS;
+      uint64_t gr, gc, rlim, clim;  // run row, first column, and their limits
+      if constexpr (KMAJ) {
+        gr = row0 + idx / 8; gc = k0 + (idx % 8) * 4; rlim = nrows; clim = K;
+      } else {
+        gr = k0 + idx / (ROWS / 4); gc = row0 + (idx % (ROWS / 4)) * 4; rlim = K; clim = nrows;
+      }
+      const float* q = P + gr * ld + gc;
+      if constexpr (VEC) {
+        const bool ok = gr < rlim && gc < clim;
+        r[i] = *reinterpret_cast<const f32x4_t*>(ok ? q : k_f32_zeros);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const bool ok = gr < rlim && gc + e < clim;
+          r[i][e] = *(ok ? q + e : k_f32_zeros);
+        }
+      }
+    }
+  }
+
+  __device__ __forceinline__ void store(float* s) const {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int idx = threadIdx.x + i * THREADS;
+      int off;
+      if constexpr (KMAJ) {
+        const int row = idx / 8, c = idx % 8;
+        off = row * F32_BK + ((c ^ (row & 7)) << 2);
+      } else {
+        const int kr = idx / (ROWS / 4), c4 = idx % (ROWS / 4);
+        off = kr * LD + c4 * 4;
+      }
+      *reinterpret_cast<f32x4_t*>(s + off) = r[i];
+    }
+  }
+
+  // values of tile row `row` for the MFMAs t = 0..3 of k group g, half-wave h
+  __device__ __forceinline__ static f32x4_t frag(const float* s, int row, int g, int h) {
+    if constexpr (KMAJ) {
+      const int c = 2 * g + h;
+      return *reinterpret_cast<const f32x4_t*>(s + row * F32_BK + ((c ^ (row & 7)) << 2));
+    } else {
+      const float* q = s + (8 * g + 4 * h) * LD + row;
+      return f32x4_t{q[0], q[LD], q[2 * LD], q[3 * LD]};
+    }
+  }
+};
+
+template <bool AK, bool BKM, bool VEC, int BM, int BN, int WGM, int WGN>
+__global__ void __launch_bounds__(64 * WGM * WGN, 2)
+k_gemm_f32(GemmArgs p) {
+  constexpr int THREADS = 64 * WGM * WGN, WAVES = WGM * WGN;
+  constexpr int WM = BM / WGM, WN = BN / WGN;
+  static_assert(WM % 32 == 0 && WN % 32 == 0, "32 x 32 MFMA blocks per wave");
+  constexpr int TM = WM / 32, TN = WN / 32;
+  using SA = F32Stager<BM, AK, THREADS>;
+  using SB = F32Stager<BN, BKM, THREADS>;
+  constexpr int MAIN = 2 * (SA::FLOATS + SB::FLOATS), STAGE = WAVES * 32 * (WN + 4);
+  __shared__ __attribute__((aligned(16))) float smem[MAIN > STAGE ? MAIN : STAGE];
+  auto As = [&](int b) { return smem + b * SA::FLOATS; };
+  auto Bs = [&](int b) { return smem + 2 * SA::FLOATS + b * SB::FLOATS; };
+
+  uint32_t tm, tn;
+  tile_coords(p, tm, tn);
+  const uint64_t m0 = (uint64_t)tm * BM, n0 = (uint64_t)tn * BN;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave / WGN, wn = wave % WGN;
+  const int l32 = lane & 31, lhi = lane >> 5;
+  const int arow = wm * WM + l32, brow = wn * WN + l32;
+
+  SA sa;
+  SB sb;
+  f32x16_t acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+  // fragments of one k group, double-buffered: group g+1 is read while the
+  // MFMAs of group g issue
+  constexpr int G = F32_BK / 8;                                // k groups per slice
+  constexpr int NM = 4 * TM * TN;                              // MFMAs per group
+  constexpr int NDS = (AK ? TM : 2 * TM) + (BKM ? TN : 2 * TN);  // LDS reads per group
+  constexpr int WPM = (SA::NV + SB::NV + NM - 1) / NM;         // LDS stores per MFMA
+  static_assert(NDS <= NM, "fragment reads fit between the MFMAs of a group");
+  f32x4_t fa[2][TM], fb[2][TN];
+  auto frags = [&](int buf, int fi, int g) {
+#pragma unroll
+    for (int i = 0; i < TM; ++i) fa[fi][i] = SA::frag(As(buf), arow + i * 32, g, lhi);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) fb[fi][j] = SB::frag(Bs(buf), brow + j * 32, g, lhi);
+  };
+
+  const uint64_t K = p.K, nk = (K + F32_BK - 1) / F32_BK;
+  sa.template load<VEC>(p.A, p.lda, m0, p.M, 0, K);
+  sb.template load<VEC>(p.B, p.ldb, n0, p.N, 0, K);
+  sa.store(As(0));
+  sb.store(Bs(0));
+  __syncthreads();
+  frags(0, 0, 0);
+  int cur = 0;
+  for (uint64_t kt = 0; kt < nk; ++kt) {
+    // the next slice (past K: zeros) into registers
+    sa.template load<VEC>(p.A, p.lda, m0, p.M, (kt + 1) * F32_BK, K);
+    sb.template load<VEC>(p.B, p.ldb, n0, p.N, (kt + 1) * F32_BK, K);
+    // keep the loads here: left to itself the scheduler sinks them to the
+    // LDS store at the end of the slice, where their latency is exposed
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int c = g & 1;
+      if (g + 1 < G) frags(cur, c ^ 1, g + 1);
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[c][i][t], fb[c][j][t], acc[i][j], 0, 0, 0);
+      // every group but the last alternates one MFMA with one fragment read
+      // of the next group, so LDS latency hides behind matrix-core work
+      if (g + 1 < G) {
+#pragma unroll
+        for (int q = 0; q < NDS; ++q) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, NM - NDS, 0);
+      }
+    }
+    // the last group's MFMAs alternate with the LDS stores of the next slice
+    sa.store(As(cur ^ 1));
+    sb.store(Bs(cur ^ 1));
+#pragma unroll
+    for (int q = 0; q < NM; ++q) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);    // MFMA
+      __builtin_amdgcn_sched_group_barrier(0x200, WPM, 0);  // DS write
+    }
+    __syncthreads();
+    cur ^= 1;
+    frags(cur, 0, 0);
+  }
+  __syncthreads();  // every wave is done with the operand tiles before staging
+  staged_epilogue<TM, TN, WM, WN>(smem, acc, p, nullptr, m0, n0, wm, wn);
+}
+
+// Kernel configurations of the FP32 path (tile and wave grid).  They differ
+// in how the output is tiled only: every output sums its k in the same order,
+// so the choice never changes a result.
+typedef void (*F32Launch)(dim3, hipStream_t, const GemmArgs&, bool ta, bool tb, bool vec);
+template <int BM, int BN, int WGM, int WGN>
+static void launch_f32(dim3 grid, hipStream_t s, const GemmArgs& a, bool ta, bool tb, bool vec) {
+  const dim3 block(64 * WGM * WGN);
+  // A is K-major unless transposed, B is K-major when transposed
+#define VKH_F32_LAUNCH(AK, BKM)                                                                       \
+  do {                                                                                                \
+    if (vec) hipLaunchKernelGGL((k_gemm_f32<AK, BKM, true, BM, BN, WGM, WGN>), grid, block, 0, s, a); \
+    else hipLaunchKernelGGL((k_gemm_f32<AK, BKM, false, BM, BN, WGM, WGN>), grid, block, 0, s, a);   \
+  } while (0)
+  if (!ta && !tb) VKH_F32_LAUNCH(true, false);
+  else if (!ta && tb) VKH_F32_LAUNCH(true, true);
+  else if (ta && !tb) VKH_F32_LAUNCH(false, false);
+  else VKH_F32_LAUNCH(false, true);
+#undef VKH_F32_LAUNCH
+}
+struct F32Cfg {
+  int bm, bn, waves;
+  F32Launch launch;
+};
+static const F32Cfg F32_CFGS[] = {
+    {128, 128, 4, launch_f32<128, 128, 2, 2>},
+    {256, 128, 8, launch_f32<256, 128, 4, 2>},
+    {128, 256, 8, launch_f32<128, 256, 2, 4>},
+    {256, 256, 8, launch_f32<256, 256, 2, 4>},
+};
+constexpr int F32_NCFG = sizeof(F32_CFGS) / sizeof(F32_CFGS[0]);
+
+static void gemm_f32(vkh_gpu* g, bool ta, bool tb, GemmArgs& a, int cfg) {
+  const F32Cfg& c = F32_CFGS[cfg];
+  a.tiles_m = (uint32_t)ceil_div(a.M, (uint64_t)c.bm);
+  a.tiles_n = (uint32_t)ceil_div(a.N, (uint64_t)c.bn);
+  const uint64_t nb = (uint64_t)a.tiles_m * a.tiles_n;
+  VKH_REQUIRE(nb < (uint64_t(1) << 31), "GEMM too large");
+  // 16-byte runs: aligned operands whose leading dimension and contiguous
+  // extent (A: K unless transposed, else M; B: N unless transposed, else K)
+  // are multiples of 4
+  const bool vec = ((uintptr_t)a.A & 15u) == 0 && a.lda % 4 == 0 && (ta ? a.M : a.K) % 4 == 0 &&
+                   ((uintptr_t)a.B & 15u) == 0 && a.ldb % 4 == 0 && (tb ? a.K : a.N) % 4 == 0;
+  c.launch(dim3((unsigned)nb), g->stream, a, ta, tb, vec);
+  check_launch("gemm");
+}
+
 // Sums the S k-chunk partials of each output in chunk order and applies the
 // GEMM epilogue (alpha, beta * C, bias, activation, aux derivative, BF16 copy).
 __global__ void __launch_bounds__(256)
@@ -708,35 +765,9 @@ k_gemm_packed(PackedArgs p) {
     if (kt + 1 < nk) frags(cur, 0, 0);
   }
   __syncthreads();  // every wave is done with the operand tiles before staging
-
-  // Epilogue.  The loop above ended with a barrier, so the operand tiles are
-  // dead.  acc[i][j][r] is C[m, n] with m = wave row + i*32 + (r/4)*8 +
-  // lhi*4 + r%4 and n = wave col + j*32 + l32; each wave writes rows
-  // i*32 .. i*32+31 of its sub-tile to its own 32 x SROW staging slice, then
-  // reads them back as 4-float row segments (WN/4 lanes per row).
-  float* st = reinterpret_cast<float*>(smem) + wave * 32 * SROW;
-  constexpr int F4R = WN / 4, RPI = 64 / F4R;
-  const uint64_t nb = n0 + wn * WN;
-#pragma unroll
-  for (int i = 0; i < TM; ++i) {
-#pragma unroll
-    for (int j = 0; j < TN; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r)
-        st[((r / 4) * 8 + lhi * 4 + (r % 4)) * SROW + j * 32 + l32] = acc[i][j][r];
-    __syncthreads();
-    const uint64_t mb = m0 + wm * WM + i * 32;
-    const int cc = (lane % F4R) * 4;
-    for (int rr = lane / F4R; rr < 32; rr += RPI) {
-      const uint64_t m = mb + rr, n = nb + cc;
-      if (m < e.M && n < e.N) {
-        const float4 v = *reinterpret_cast<const float4*>(st + rr * SROW + cc);
-        if (p.part) store_part4(p.part + (uint64_t)blockIdx.y * e.M * e.N, e.N, m, n, v);
-        else epilogue4(e, m, n, v);
-      }
-    }
-    __syncthreads();
-  }
+  staged_epilogue<TM, TN, WM, WN>(reinterpret_cast<float*>(smem), acc, e,
+                                  p.part ? p.part + (uint64_t)blockIdx.y * e.M * e.N : nullptr,
+                                  m0, n0, wm, wn);
 }
 
 // BF16 variant whose operand slices go from global memory straight into LDS
@@ -855,31 +886,9 @@ k_gemm_direct(PackedArgs p) {
     if (kt + 1 < nk) frags(cur, 0, 0);
   }
   __syncthreads();  // every wave is done with the operand tiles before staging
-
-  // epilogue, as in k_gemm_packed
-  float* st = reinterpret_cast<float*>(smem) + wave * 32 * SROW;
-  constexpr int F4R = WN / 4, RPI = 64 / F4R;
-  const uint64_t nb = n0 + wn * WN;
-#pragma unroll
-  for (int i = 0; i < TM; ++i) {
-#pragma unroll
-    for (int j = 0; j < TN; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r)
-        st[((r / 4) * 8 + lhi * 4 + (r % 4)) * SROW + j * 32 + l32] = acc[i][j][r];
-    __syncthreads();
-    const uint64_t mb = m0 + wm * WM + i * 32;
-    const int cc = (lane % F4R) * 4;
-    for (int rr = lane / F4R; rr < 32; rr += RPI) {
-      const uint64_t m = mb + rr, n = nb + cc;
-      if (m < e.M && n < e.N) {
-        const float4 v = *reinterpret_cast<const float4*>(st + rr * SROW + cc);
-        if (p.part) store_part4(p.part + (uint64_t)blockIdx.y * e.M * e.N, e.N, m, n, v);
-        else epilogue4(e, m, n, v);
-      }
-    }
-    __syncthreads();
-  }
+  staged_epilogue<TM, TN, WM, WN>(reinterpret_cast<float*>(smem), acc, e,
+                                  p.part ? p.part + (uint64_t)blockIdx.y * e.M * e.N : nullptr,
+                                  m0, n0, wm, wn);
 }
 
 // Tile choice of the packed kernel: 256 x 256 when both sides fill at least
@@ -928,9 +937,22 @@ static const PackedCfg* packed_cfgs(int mode, int& count) {
 // 1.43 ms for 4 waves; 65536 x 1024 x 1024: 0.294 against 0.348 ms), split
 // mode the 8-wave 256 x 128 tile for long K (8192^3: 3.55 against 4.18 ms)
 // and 128 x 128 otherwise (K = 1024: 0.59 against 0.74 ms).
+// FP32 takes the 256 x 256 tile when at least 512 such tiles cover C (8192^3:
+// 141 against 126-135 TFLOP/s for the other tiles; 65536 x 1024 x 1024: 124
+// against 100-113), the 128 x 128 tile otherwise (4096 x 1024 x 784: 92
+// against 31 TFLOP/s for 256 x 256, which leaves most CUs idle).
 static int default_cfg(int mode, uint64_t M, uint64_t N, uint64_t K) {
+  if (mode == VKH_GEMM_FP32) return ceil_div(M, (uint64_t)256) * ceil_div(N, (uint64_t)256) >= 512 ? 3 : 0;
   if (mode == VKH_GEMM_BF16) return (M >= 256 && N >= 256) ? 1 : 2;
   return (M >= 256 && N >= 128 && K >= 4096) ? 1 : 0;
+}
+
+// number of kernel configurations of a mode
+static int mode_cfgs(int mode) {
+  if (mode == VKH_GEMM_FP32) return F32_NCFG;
+  int n = 0;
+  packed_cfgs(mode, n);
+  return n;
 }
 
 static std::array<uint64_t, 6> cfg_key(int mode, int ta, int tb, uint64_t M, uint64_t N, uint64_t K) {
@@ -1374,10 +1396,12 @@ static void gemm_impl(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M
     gemm_smallk(g, trans_a, trans_b, a);
     return;
   }
-  const int va = ((uintptr_t)A & 15u) == 0 && lda % 4 == 0;
-  const int vb = ((uintptr_t)B & 15u) == 0 && ldb % 4 == 0;
   switch (mode) {
-    case VKH_GEMM_FP32: launch_mode<VKH_GEMM_FP32>(g, trans_a, trans_b, a, va, vb); break;
+    case VKH_GEMM_FP32: {
+      auto it = g->gemm_cfg.find(cfg_key(mode, trans_a, trans_b, M, N, K));
+      gemm_f32(g, trans_a, trans_b, a, it != g->gemm_cfg.end() ? it->second : default_cfg(mode, M, N, K));
+      break;
+    }
     case VKH_GEMM_BF16:
     case VKH_GEMM_BF16X3: {
       auto it = g->gemm_cfg.find(cfg_key(mode, trans_a, trans_b, M, N, K));
@@ -1394,8 +1418,7 @@ int vkh_gemm_configs(int mode, int* count) {
   return guarded([&] {
     VKH_REQUIRE(count, "null out");
     VKH_REQUIRE(mode >= VKH_GEMM_FP32 && mode <= VKH_GEMM_BF16X3, "unknown mode");
-    if (mode == VKH_GEMM_FP32) { *count = 1; return; }
-    packed_cfgs(mode, *count);
+    *count = mode_cfgs(mode);
   });
 }
 
@@ -1404,8 +1427,9 @@ int vkh_gemm_config_info(int mode, int index, int* bm, int* bn, int* bk, int* wa
     VKH_REQUIRE(bm && bn && bk && waves, "null out");
     VKH_REQUIRE(mode >= VKH_GEMM_FP32 && mode <= VKH_GEMM_BF16X3, "unknown mode");
     if (mode == VKH_GEMM_FP32) {
-      VKH_REQUIRE(index == 0, "configuration index out of range");
-      *bm = Cfg<VKH_GEMM_FP32>::BM; *bn = Cfg<VKH_GEMM_FP32>::BN; *bk = Cfg<VKH_GEMM_FP32>::BK; *waves = 4;
+      VKH_REQUIRE(index >= 0 && index < F32_NCFG, "configuration index out of range");
+      const F32Cfg& c = F32_CFGS[index];
+      *bm = c.bm; *bn = c.bn; *bk = F32_BK; *waves = c.waves;
       if (direct) *direct = 0;
       return;
     }
@@ -1424,7 +1448,7 @@ int vkh_gemm_get_config(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t
     VKH_REQUIRE(mode >= VKH_GEMM_FP32 && mode <= VKH_GEMM_BF16X3, "unknown mode");
     auto it = g->gemm_cfg.find(cfg_key(mode, trans_a, trans_b, M, N, K));
     const bool have = it != g->gemm_cfg.end();
-    *index = have ? it->second : (mode == VKH_GEMM_FP32 ? 0 : default_cfg(mode, M, N, K));
+    *index = have ? it->second : default_cfg(mode, M, N, K);
     if (tuned) *tuned = have ? 1 : 0;
   });
 }
@@ -1434,8 +1458,7 @@ int vkh_gemm_set_config(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t
   return guarded([&] {
     VKH_REQUIRE(g, "null GPU handle");
     VKH_REQUIRE(mode >= VKH_GEMM_FP32 && mode <= VKH_GEMM_BF16X3, "unknown mode");
-    int n = 1;
-    if (mode != VKH_GEMM_FP32) packed_cfgs(mode, n);
+    const int n = mode_cfgs(mode);
     const auto key = cfg_key(mode, trans_a, trans_b, M, N, K);
     if (index < 0) { g->gemm_cfg.erase(key); return; }
     VKH_REQUIRE(index < n, "configuration index out of range");
@@ -1452,13 +1475,12 @@ int vkh_gemm_autotune(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M
     VKH_REQUIRE(M > 0 && N > 0 && K > 0, "gemm_autotune: empty product");
     VKH_REQUIRE(g->alloc, "gemm_autotune needs a context with its own allocator");
     if (reps < 1) reps = 3;
-    int n = 1;
-    if (mode != VKH_GEMM_FP32) packed_cfgs(mode, n);
+    const int n = mode_cfgs(mode);
     const bool skinny = std::min(M, N) <= SKINNY_MAX || K <= SMALLK_MAX;
-    if (mode == VKH_GEMM_FP32 || skinny) {  // a single kernel choice
+    if (skinny) {  // a single kernel choice
       if (ms_per_config)
         for (int i = 0; i < n; ++i) ms_per_config[i] = -1.0f;
-      if (best) *best = mode == VKH_GEMM_FP32 ? 0 : default_cfg(mode, M, N, K);
+      if (best) *best = default_cfg(mode, M, N, K);
       return;
     }
     // scratch operands in the product's own layout
@@ -1482,14 +1504,15 @@ int vkh_gemm_autotune(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M
     int bi = -1;
     float bt = 0.0f;
     try {
+      auto run = [&](int i) {
+        GemmArgs t = a;
+        if (mode == VKH_GEMM_FP32) gemm_f32(g, trans_a, trans_b, t, i);
+        else gemm_packed(g, mode, trans_a, trans_b, t, i);
+      };
       for (int i = 0; i < n; ++i) {
-        GemmArgs w = a;
-        gemm_packed(g, mode, trans_a, trans_b, w, i);  // warm-up (and packing-area growth)
+        run(i);  // warm-up (and packing-area growth)
         VKH_HIP(hipEventRecord(e0, g->stream));
-        for (int r = 0; r < reps; ++r) {
-          GemmArgs t = a;
-          gemm_packed(g, mode, trans_a, trans_b, t, i);
-        }
+        for (int r = 0; r < reps; ++r) run(i);
         VKH_HIP(hipEventRecord(e1, g->stream));
         VKH_HIP(hipEventSynchronize(e1));
         float ms = 0.0f;

@@ -283,7 +283,7 @@ class _Plan:
         # kernel configuration (when tuning) and the graph's GEMM packing
         # region for every product of the step
         for ta, tb, M, N, K in self._gemm_shapes():
-            if self.seq.autotune and self.mode != GEMM_MODE["fp32"]:
+            if self.seq.autotune:
                 call("vkh_gemm_autotune", self.h, self.mode, ta, tb, M, N, K, 3, None, None)
             call("vkh_gemm_reserve", self.h, self.mode, M, N, K)
         if self.seq.autotune:
@@ -292,6 +292,8 @@ class _Plan:
                 # the step's results never depend on the timings
                 call("vkh_reduce_autotune", self.h, 1, prev, K, post, 3, 1, None, None)
         self.gpu.wait()
+        if not self.seq.capture:
+            return
         call("vkh_graph_begin", self.h)
         try:
             self._enqueue()
@@ -310,7 +312,10 @@ class _Plan:
         h = self.h
         call("vkh_copy", h, self.x._ptr, x._ptr, self.x.buffer.nbytes)
         call("vkh_copy", h, self.y._ptr, y._ptr, self.y.buffer.nbytes)
-        call("vkh_graph_launch", h, self.graph)
+        if self.graph is None:
+            self._enqueue()
+        else:
+            call("vkh_graph_launch", h, self.graph)
         for p in self.params:
             st = p.opt_state
             if isinstance(st, AdamState):
@@ -370,12 +375,17 @@ class GraphSequence:
         never does (vulkpy/nn/models.py:55-78).
     autotune : bool
         Before capturing, time every GEMM kernel configuration on each
-        product shape of the step (BF16 modes; ``vulkpy_amd.autotune_gemm``)
+        product shape of the step (``vulkpy_amd.autotune_gemm``)
         and the reduction configurations of the bias gradients and the batch
         loss that keep the default summation order
         (``vulkpy_amd.autotune_reduce(..., same_order=True)``), and use the
         fastest.  Neither choice changes a result.  The choice is kept on the
         GPU context, so eager calls of the same shapes use it too.
+    capture : bool
+        Capture the step as a hipGraph (default).  With False every ``train``
+        call issues the same kernels, with the same arguments and results,
+        one launch at a time: for profilers and debuggers that do not follow
+        graph replays.
 
     Notes
     -----
@@ -384,7 +394,8 @@ class GraphSequence:
     """
 
     def __init__(self, layers: Iterable[Module], loss: Loss, *,
-                 precision: str = "bf16", regularize: bool = True, autotune: bool = False):
+                 precision: str = "bf16", regularize: bool = True, autotune: bool = False,
+                 capture: bool = True):
         if precision not in GEMM_MODE:
             raise ValueError(f"unknown precision {precision!r}")
         self.L: Tuple[Module, ...] = tuple(layers)
@@ -392,6 +403,7 @@ class GraphSequence:
         self.precision = precision
         self.regularize = regularize
         self.autotune = autotune
+        self.capture = capture
         self._plan: Optional[_Plan] = None
         units = _units(self.L)
         self.gpu = units[0].dense.w.value._gpu
