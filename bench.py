@@ -30,7 +30,8 @@ Two more sections of the same JSON line cover the rest of the metric:
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
                 [--mlp-steps S] [--no-gemm] [--no-c2]
-                [--pmc-kernels | --gemm-pmc]   (counter runs only)
+                [--pmc-kernels | --gemm-pmc [fp32|bf16|bf16x3]]   (counter runs only)
+                [--mlp-trace S]   (kernel traces of the MLP steps only)
 """
 from __future__ import annotations
 
