@@ -289,18 +289,20 @@ int vkh_box_muller(vkh_gpu* gpu, const float* u, float* out, uint64_t n,
  * Paths: min(M, N) <= 16 (skinny) and K <= 16 (short k) are memory-bound and
  * run in FP32 on the vector units in every mode; otherwise FP32 mode uses
  * FP32 matrix cores on the operands as stored, and the BF16 modes pack their
- * operands for the BF16 matrix cores, splitting k into chunks (partials
- * summed in chunk order) when fewer than 128 tiles of 256 x 256 cover C and
- * K >= 4096. */
+ * operands for the BF16 matrix cores.  k is split into chunks (partials
+ * summed in chunk order) when few output tiles cover C: in BF16 modes fewer
+ * than 128 tiles of 256 x 256 with K >= 4096, in FP32 fewer than 256 tiles of
+ * 128 x 128 with K >= 1024. */
 int vkh_gemm(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
              uint64_t M, uint64_t N, uint64_t K, float alpha,
              const float* A, uint64_t lda, const float* B, uint64_t ldb,
              float beta, float* C, uint64_t ldc, const float* bias, int act,
              uint16_t* c_bf16);
 
-/* BF16 / BF16X3 GEMMs pack their operands into a BF16 area: a per-context
- * area that grows on demand for eager calls, and a region of the graph's
- * arena for calls captured into a graph.  This grows the former for an
+/* BF16 / BF16X3 GEMMs pack their operands into a BF16 area, and products
+ * whose k is split (every mode) keep their chunk partials there: a
+ * per-context area that grows on demand for eager calls, and a region of the
+ * graph's arena for calls captured into a graph.  This grows the former for an
  * M x N x K product ahead of time and records the size so that the next
  * vkh_graph_begin reserves a large enough region (a capture cannot allocate;
  * a captured GEMM that does not fit fails and records its size for the next

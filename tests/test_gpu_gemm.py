@@ -135,10 +135,11 @@ def test_skinny_epilogue(gpu, M, N, K, ta, tb):
     for mode in ("bf16", "bf16x3") for ta, tb in ((0, 1), (1, 0))
     for M, N, K in ((300, 260, 200), (300, 200, 8192))
 ] + [
-    # FP32: every layout, 16-byte runs (first shape) and element loads (odd
-    # extents, second shape)
+    # FP32: every layout, 16-byte runs (first shape), element loads (odd
+    # extents, second shape) and k split into 8 chunks (third shape)
     ("fp32", ta, tb, M, N, K)
-    for ta in (0, 1) for tb in (0, 1) for M, N, K in ((300, 260, 200), (301, 259, 203))
+    for ta in (0, 1) for tb in (0, 1)
+    for M, N, K in ((300, 260, 200), (301, 259, 203), (300, 200, 8192))
 ])
 def test_configurations_bit_identical(gpu, mode, ta, tb, M, N, K):
     """Every kernel configuration accumulates each output over k in the
@@ -180,12 +181,14 @@ def test_autotune_api(gpu):
         _lib.call("vkh_gemm_set_config", gpu._handle, _lib.GEMM_MODE["bf16"], 0, 0, 8, 8, 8, 99)
 
 
-# Split k: few output tiles (of 256 x 256) and k >= 4096 run k in chunks of
-# at least 1024 as separate blocks, then sum the chunk partials in order.
+# Split k: few output tiles (BF16 modes: of 256 x 256 with k >= 4096, chunks
+# of at least 1024; FP32: of 128 x 128 with k >= 1024, chunks of at least
+# 256) run k in chunks as separate blocks, then sum the chunk partials in
+# order.
 SPLITK = [(300, 200, 8192), (300, 70, 5000), (64, 48, 4100), (4096, 256, 16384)]
 
 
-@pytest.mark.parametrize("mode", ["bf16", "bf16x3"])
+@pytest.mark.parametrize("mode", ["fp32", "bf16", "bf16x3"])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1)])
 @pytest.mark.parametrize("M,N,K", SPLITK[:3])
 def test_split_k_accuracy(gpu, mode, ta, tb, M, N, K):
@@ -202,7 +205,7 @@ def test_split_k_exact_and_epilogue(gpu):
     # integer operands: every chunk partial and their sum are exact
     A = rng.integers(-2, 3, (64, 8192)).astype(np.float32)
     B = rng.integers(-2, 3, (8192, 48)).astype(np.float32)
-    for mode in ("bf16", "bf16x3"):
+    for mode in ("fp32", "bf16", "bf16x3"):
         np.testing.assert_array_equal(run_gemm(gpu, mode, 0, 0, A, B), A @ B)
     # alpha, beta * C, bias and ReLU applied once, after the chunks are summed
     A = rng.standard_normal((300, 5000)).astype(np.float32)
@@ -210,7 +213,7 @@ def test_split_k_exact_and_epilogue(gpu):
     C0 = rng.standard_normal((300, 70)).astype(np.float32)
     bias = rng.standard_normal(70).astype(np.float32)
     ref, bound = ref_gemm(0, 1, A, B)
-    for mode in ("bf16", "bf16x3"):
+    for mode in ("fp32", "bf16", "bf16x3"):
         got = run_gemm(gpu, mode, 0, 1, A, B, C0=C0, alpha=0.5, beta=2.0, bias=bias)
         want = 0.5 * ref + 2.0 * C0 + bias
         assert (np.abs(got - want) <= 0.5 * TOL[mode] * bound + 1e-5 * (1 + np.abs(want))).all()
@@ -228,7 +231,7 @@ def test_split_k_weight_gradient_shape(gpu):
     rows = np.arange(0, M, 97)
     a64 = A[:, rows].T.astype(np.float64)
     ref, bound = a64 @ B.astype(np.float64), np.abs(a64) @ np.abs(B.astype(np.float64))
-    for mode in ("bf16", "bf16x3"):
+    for mode in ("fp32", "bf16", "bf16x3"):
         got = run_gemm(gpu, mode, 1, 0, A, B)[rows]
         assert (np.abs(got - ref) / bound).max() <= TOL[mode]
 

@@ -461,7 +461,7 @@ struct F32Stager {
 
 template <bool AK, bool BKM, bool VEC, int BM, int BN, int WGM, int WGN>
 __global__ void __launch_bounds__(64 * WGM * WGN, 2)
-k_gemm_f32(GemmArgs p) {
+k_gemm_f32(GemmArgs p, float* part, uint64_t kc) {
   constexpr int THREADS = 64 * WGM * WGN, WAVES = WGM * WGN;
   constexpr int WM = BM / WGM, WN = BN / WGN;
   static_assert(WM % 32 == 0 && WN % 32 == 0, "32 x 32 MFMA blocks per wave");
@@ -506,9 +506,14 @@ k_gemm_f32(GemmArgs p) {
     for (int j = 0; j < TN; ++j) fb[fi][j] = SB::frag(Bs(buf), brow + j * 32, g, lhi);
   };
 
-  const uint64_t K = p.K, nk = (K + F32_BK - 1) / F32_BK;
-  sa.template load<VEC>(p.A, p.lda, m0, p.M, 0, K);
-  sb.template load<VEC>(p.B, p.ldb, n0, p.N, 0, K);
+  // k range of this block: all of k, or chunk blockIdx.y of kc (a multiple of
+  // F32_BK, so the slices tile the chunk) when k is split
+  const uint64_t K = p.K;
+  const uint64_t kb = part ? (uint64_t)blockIdx.y * kc : 0;
+  const uint64_t ke = part ? min(kb + kc, K) : K;
+  const uint64_t nk = (ke - kb + F32_BK - 1) / F32_BK;
+  sa.template load<VEC>(p.A, p.lda, m0, p.M, kb, K);
+  sb.template load<VEC>(p.B, p.ldb, n0, p.N, kb, K);
   sa.store(As(0));
   sb.store(Bs(0));
   __syncthreads();
@@ -516,8 +521,9 @@ k_gemm_f32(GemmArgs p) {
   int cur = 0;
   for (uint64_t kt = 0; kt < nk; ++kt) {
     // the next slice (past K: zeros) into registers
-    sa.template load<VEC>(p.A, p.lda, m0, p.M, (kt + 1) * F32_BK, K);
-    sb.template load<VEC>(p.B, p.ldb, n0, p.N, (kt + 1) * F32_BK, K);
+    // (the prefetch after a chunk's last slice is stored but never used)
+    sa.template load<VEC>(p.A, p.lda, m0, p.M, kb + (kt + 1) * F32_BK, K);
+    sb.template load<VEC>(p.B, p.ldb, n0, p.N, kb + (kt + 1) * F32_BK, K);
     // keep the loads here: left to itself the scheduler sinks them to the
     // LDS store at the end of the slice, where their latency is exposed
     __builtin_amdgcn_sched_barrier(0);
@@ -556,21 +562,26 @@ k_gemm_f32(GemmArgs p) {
     frags(cur, 0, 0);
   }
   __syncthreads();  // every wave is done with the operand tiles before staging
-  staged_epilogue<TM, TN, WM, WN>(smem, acc, p, nullptr, m0, n0, wm, wn);
+  staged_epilogue<TM, TN, WM, WN>(smem, acc, p, part ? part + (uint64_t)blockIdx.y * p.M * p.N : nullptr,
+                                  m0, n0, wm, wn);
 }
 
 // Kernel configurations of the FP32 path (tile and wave grid).  They differ
 // in how the output is tiled only: every output sums its k in the same order,
 // so the choice never changes a result.
-typedef void (*F32Launch)(dim3, hipStream_t, const GemmArgs&, bool ta, bool tb, bool vec);
+typedef void (*F32Launch)(dim3, hipStream_t, const GemmArgs&, bool ta, bool tb, bool vec, float* part,
+                          uint64_t kc);
 template <int BM, int BN, int WGM, int WGN>
-static void launch_f32(dim3 grid, hipStream_t s, const GemmArgs& a, bool ta, bool tb, bool vec) {
+static void launch_f32(dim3 grid, hipStream_t s, const GemmArgs& a, bool ta, bool tb, bool vec, float* part,
+                       uint64_t kc) {
   const dim3 block(64 * WGM * WGN);
   // A is K-major unless transposed, B is K-major when transposed
-#define VKH_F32_LAUNCH(AK, BKM)                                                                       \
-  do {                                                                                                \
-    if (vec) hipLaunchKernelGGL((k_gemm_f32<AK, BKM, true, BM, BN, WGM, WGN>), grid, block, 0, s, a); \
-    else hipLaunchKernelGGL((k_gemm_f32<AK, BKM, false, BM, BN, WGM, WGN>), grid, block, 0, s, a);   \
+#define VKH_F32_LAUNCH(AK, BKM)                                                                   \
+  do {                                                                                            \
+    if (vec)                                                                                      \
+      hipLaunchKernelGGL((k_gemm_f32<AK, BKM, true, BM, BN, WGM, WGN>), grid, block, 0, s, a, part, kc); \
+    else                                                                                          \
+      hipLaunchKernelGGL((k_gemm_f32<AK, BKM, false, BM, BN, WGM, WGN>), grid, block, 0, s, a, part, kc); \
   } while (0)
   if (!ta && !tb) VKH_F32_LAUNCH(true, false);
   else if (!ta && tb) VKH_F32_LAUNCH(true, true);
@@ -589,21 +600,6 @@ static const F32Cfg F32_CFGS[] = {
     {256, 256, 8, launch_f32<256, 256, 2, 4>},
 };
 constexpr int F32_NCFG = sizeof(F32_CFGS) / sizeof(F32_CFGS[0]);
-
-static void gemm_f32(vkh_gpu* g, bool ta, bool tb, GemmArgs& a, int cfg) {
-  const F32Cfg& c = F32_CFGS[cfg];
-  a.tiles_m = (uint32_t)ceil_div(a.M, (uint64_t)c.bm);
-  a.tiles_n = (uint32_t)ceil_div(a.N, (uint64_t)c.bn);
-  const uint64_t nb = (uint64_t)a.tiles_m * a.tiles_n;
-  VKH_REQUIRE(nb < (uint64_t(1) << 31), "GEMM too large");
-  // 16-byte runs: aligned operands whose leading dimension and contiguous
-  // extent (A: K unless transposed, else M; B: N unless transposed, else K)
-  // are multiples of 4
-  const bool vec = ((uintptr_t)a.A & 15u) == 0 && a.lda % 4 == 0 && (ta ? a.M : a.K) % 4 == 0 &&
-                   ((uintptr_t)a.B & 15u) == 0 && a.ldb % 4 == 0 && (tb ? a.K : a.N) % 4 == 0;
-  c.launch(dim3((unsigned)nb), g->stream, a, ta, tb, vec);
-  check_launch("gemm");
-}
 
 // Sums the S k-chunk partials of each output in chunk order and applies the
 // GEMM epilogue (alpha, beta * C, bias, activation, aux derivative, BF16 copy).
@@ -1092,6 +1088,52 @@ static void gemm_packed(vkh_gpu* g, int mode, bool ta, bool tb, GemmArgs& a, int
   }
 }
 
+// FP32 split k: as above, for fewer than 256 tiles of 128 x 128 (the
+// smallest FP32 tile, 2 blocks per CU) and K >= 1024, in chunks of at least
+// 256 that are multiples of the 32-deep slice, aiming at 512 blocks.  The
+// chunk bounds depend on (M, N, K) only.  4096 x 256 x 16384 (a weight
+// gradient over a batch of 16384): 8 chunks, 31 -> 121-126 TFLOP/s.
+constexpr uint64_t F32_SPLITK_MIN_K = 1024, F32_SPLITK_MIN_CHUNK = 256, F32_SPLITK_BLOCKS = 512;
+
+static uint64_t f32_splitk_chunk(uint64_t M, uint64_t N, uint64_t K) {
+  const uint64_t tiles = ceil_div(M, 128) * ceil_div(N, 128);
+  if (K < F32_SPLITK_MIN_K || tiles * 2 > F32_SPLITK_BLOCKS) return K;
+  const uint64_t S = std::min(ceil_div(F32_SPLITK_BLOCKS, tiles), K / F32_SPLITK_MIN_CHUNK);
+  if (S < 2) return K;
+  return ceil_div(ceil_div(K, (uint64_t)F32_BK), S) * F32_BK;
+}
+
+// split-k partials [S][M][N] of an FP32 product (0 when k is not split)
+static size_t f32_split_bytes(uint64_t M, uint64_t N, uint64_t K) {
+  const uint64_t S = ceil_div(K, f32_splitk_chunk(M, N, K));
+  return S > 1 ? (size_t)(S * M * N * 4) : 0;
+}
+
+static void gemm_f32(vkh_gpu* g, bool ta, bool tb, GemmArgs& a, int cfg) {
+  VKH_REQUIRE(cfg >= 0 && cfg < F32_NCFG, "gemm: configuration index out of range");
+  const F32Cfg& c = F32_CFGS[cfg];
+  a.tiles_m = (uint32_t)ceil_div(a.M, (uint64_t)c.bm);
+  a.tiles_n = (uint32_t)ceil_div(a.N, (uint64_t)c.bn);
+  const uint64_t nb = (uint64_t)a.tiles_m * a.tiles_n;
+  VKH_REQUIRE(nb < (uint64_t(1) << 31), "GEMM too large");
+  // 16-byte runs: aligned operands whose leading dimension and contiguous
+  // extent (A: K unless transposed, else M; B: N unless transposed, else K)
+  // are multiples of 4
+  const bool vec = ((uintptr_t)a.A & 15u) == 0 && a.lda % 4 == 0 && (ta ? a.M : a.K) % 4 == 0 &&
+                   ((uintptr_t)a.B & 15u) == 0 && a.ldb % 4 == 0 && (tb ? a.K : a.N) % 4 == 0;
+  const uint64_t kc = f32_splitk_chunk(a.M, a.N, a.K);
+  const uint64_t S = ceil_div(a.K, kc);
+  float* part = S > 1 ? reinterpret_cast<float*>(pack_area(g, f32_split_bytes(a.M, a.N, a.K))) : nullptr;
+  c.launch(dim3((unsigned)nb, (unsigned)S), g->stream, a, ta, tb, vec, part, kc);
+  check_launch("gemm");
+  if (S > 1) {
+    const uint64_t groups = a.M * ceil_div(a.N, 4);
+    hipLaunchKernelGGL(k_splitk_combine, dim3(stream_grid(g, groups, 256, 8)), dim3(256), 0, g->stream,
+                       (const float*)part, S, a);
+    check_launch("gemm split-k combine");
+  }
+}
+
 // ---- skinny products: min(M, N) <= 16 ------------------------------------------------
 // C = op(A) op(B) with one small output dimension Q is a pass over the large
 // operand (the Dense layer with one or ten outputs, its weight gradient):
@@ -1538,9 +1580,11 @@ int vkh_gemm_reserve(vkh_gpu* g, int mode, uint64_t M, uint64_t N, uint64_t K) {
     use(g);
     VKH_REQUIRE(mode >= VKH_GEMM_FP32 && mode <= VKH_GEMM_BF16X3, "gemm_reserve: unknown mode");
     VKH_REQUIRE(!g->capturing, "gemm_reserve: not allowed during graph capture");
-    // FP32, skinny and short-k products pack nothing
-    if (mode == VKH_GEMM_FP32 || M == 0 || N == 0 || std::min(M, N) <= SKINNY_MAX || K <= SMALLK_MAX) return;
-    const size_t bytes = packed_bytes(mode, M, N, K);
+    // skinny and short-k products pack nothing; FP32 needs room only for
+    // split-k partials
+    if (M == 0 || N == 0 || std::min(M, N) <= SKINNY_MAX || K <= SMALLK_MAX) return;
+    const size_t bytes = mode == VKH_GEMM_FP32 ? f32_split_bytes(M, N, K) : packed_bytes(mode, M, N, K);
+    if (bytes == 0) return;
     g->graph_pack_hint = std::max(g->graph_pack_hint, bytes);
     pack_area(g, bytes);
   });
