@@ -178,6 +178,70 @@ def test_dense_matches_vulkpy_backward(dev, gpu):
     np.testing.assert_array_equal(b.grad.cpu().numpy(), np.asarray(layer.b.grad))
 
 
+def f64_relu(x):
+    return np.maximum(x, 0.0)
+
+
+def test_c4_full_size_autograd_matches_vulkpy_backward(dev, gpu):
+    """The config-4 network at full size (784 -> 1024 -> 1024 -> 10, ReLU,
+    Sigmoid, softmax cross entropy, batch 4096): the torch.autograd wrappers'
+    gradients equal those of vulkpy's own layer-by-layer backward (nn.Dense /
+    ReLU / Sigmoid / SoftmaxCrossEntropyLoss on vk.Array) bit for bit.  The
+    forward is within 1e-4 of a float64 forward of the same weights, and the
+    gradients within 1e-4 (relative to each gradient's largest entry) of a
+    float64 backward from the GPU's own activations (a float64 forward would
+    flip ReLU masks where a pre-activation is within FP32 rounding of 0)."""
+    rng = np.random.default_rng(44)
+    B, dims, C = 4096, [(784, 1024), (1024, 1024), (1024, 10)], 10
+    x = rng.standard_normal((B, 784)).astype(np.float32)
+    t = np.identity(C, dtype=np.float32)[rng.integers(0, C, B)]
+    layers = [nn.Dense(gpu, i, o, w_init=nn.HeNormal(gpu, i, seed=70 + k)) for k, (i, o) in enumerate(dims)]
+    for k, L in enumerate(layers):
+        L.b.value[:] = (0.1 * rng.standard_normal(dims[k][1])).astype(np.float32)
+        L.zero_grad()
+    relu, sig, lossf = nn.ReLU(), nn.Sigmoid(), nn.SoftmaxCrossEntropyLoss()
+    X, T = vk.Array(gpu, data=x), vk.Array(gpu, data=t)
+    h1_vk = relu(layers[0](X))
+    h2_vk = sig(layers[1](h1_vk))
+    z = layers[2](h2_vk)
+    loss_vk = float(np.asarray(lossf(z, T)))
+    h1g, h2g, zg = (np.asarray(a).astype(np.float64) for a in (h1_vk, h2_vk, z))
+    dz = lossf.grad()
+    layers[0].backward(relu.backward(layers[1].backward(sig.backward(layers[2].backward(dz)))))
+    gpu.wait()
+    vk_grads = [np.asarray(p.grad).copy() for L in layers for p in (L.w, L.b)]
+    weights = [np.asarray(p.value).copy() for L in layers for p in (L.w, L.b)]
+
+    P = [torch.tensor(w, device=dev, requires_grad=True) for w in weights]
+    Xt, Tt = torch.tensor(x, device=dev), torch.tensor(t, device=dev)
+    zt = ta.dense(ta.sigmoid(ta.dense(ta.relu(ta.dense(Xt, P[0], P[1])), P[2], P[3])), P[4], P[5])
+    loss_t = ta.softmax_cross_entropy(zt, Tt)
+    loss_t.backward()
+    torch.cuda.synchronize()
+    for k, (p, g) in enumerate(zip(P, vk_grads)):
+        np.testing.assert_array_equal(p.grad.cpu().numpy(), g, err_msg=f"parameter {k}")
+    assert abs(float(loss_t.item()) - loss_vk) <= 1e-6 * abs(loss_vk)
+
+    # float64 forward of the same weights
+    W1, b1, W2, b2, W3, b3 = [w.astype(np.float64) for w in weights]
+    x64, t64 = x.astype(np.float64), t.astype(np.float64)
+    h1 = f64_relu(x64 @ W1.T + b1)
+    h2 = 1.0 / (1.0 + np.exp(-(h1 @ W2.T + b2)))
+    z64 = h2 @ W3.T + b3
+    for k, (g, r) in enumerate(((h1g, h1), (h2g, h2), (zg, z64))):
+        assert np.abs(g - r).max() <= 1e-4 * np.abs(r).max(), ("forward", k)
+    sm = f64_softmax(z64)
+    loss64 = np.mean(np.sum(-t64 * np.log(sm + 1e-8), axis=1))
+    assert abs(loss_vk - loss64) <= 1e-5 * abs(loss64)
+    # float64 backward from the GPU's activations
+    d3 = (f64_softmax(zg) - t64) / B
+    d2 = (d3 @ W3) * h2g * (1.0 - h2g)
+    d1 = (d2 @ W2) * (h1g > 0)
+    ref = [d1.T @ x64, d1.sum(0), d2.T @ h1g, d2.sum(0), d3.T @ h2g, d3.sum(0)]
+    for k, (g, r) in enumerate(zip(vk_grads, ref)):
+        assert np.abs(g - r).max() <= 1e-4 * np.abs(r).max(), ("gradient", k)
+
+
 def test_rejects_bad_inputs(dev):
     x = torch.zeros((4, 3), device=dev, dtype=torch.float64)
     with pytest.raises(TypeError):
