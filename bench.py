@@ -18,13 +18,13 @@ value = algorithmic bytes of all ops / wall time of the timed steps (GB/s,
 inputs resident in HBM).  The dominant kernel (broadcast A + col) is also
 timed per launch with HIP events on the library's stream for the roofline.
 The CPU baseline times the oracle (C + OpenMP, the reference's algorithms) on
-the first 512 rows of the same operands.
+the first 512 rows of the same operands, and NumPy beside it.
 
 Two more sections of the same JSON line cover the rest of the metric:
   * "mlp": the 256-4096-4096-1 regression MLP (batch 16384, Huber, Ridge,
     Adam), plain FP32 step against the BF16 step captured as one hipGraph:
     steps/s of each, the speed-up and the final losses after the same number
-    of steps;
+    of steps, and the matrix-core utilisation of each product of the step;
   * "gemm": 8192^3 and 65536 x 1024 x 1024 products in FP32, BF16 and
     split-BF16, TFLOP/s and error against an FP64 product.
 
@@ -137,7 +137,60 @@ def build_suite(vk, gpu, A, B, col, row, P, rng_gen, ubuf, fbuf, timer):
     return ops
 
 
-def mlp_compare(vk, gpu, steps: int, warmup: int = 2, uncaptured: bool = False):
+PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0, "bf16x3": 2500.0 / 3}  # split: 3 BF16 products
+
+
+def mlp_gemm_utilisation(vk, gpu, timer, B: int, dims, reps: int = 3) -> dict:
+    """
+    Matrix-core utilisation of each product of the MLP step: the forward,
+    weight-gradient and input-gradient products of every Dense layer (the
+    first layer's input gradient is not computed), timed alone with HIP events
+    over `reps` launches behind an untimed one, in the configuration the step
+    uses (configurations chosen by autotuning stay on the context), operand
+    packing included.  frac = TFLOP/s / the mode's dense peak (split-BF16: a
+    third of the BF16 peak, since it runs three products).  Products with
+    min(M, N) <= 16 or K <= 16 run on the vector units (path "vector").
+    """
+    from vulkpy_amd import _lib
+    rng = np.random.default_rng(7)
+    shapes = []
+    for i, (din, dout) in enumerate(dims):
+        shapes.append((f"forward {i + 1}", 0, 1, B, dout, din))
+        shapes.append((f"weight gradient {i + 1}", 1, 0, dout, din, B))
+        if i > 0:
+            shapes.append((f"input gradient {i + 1}", 0, 0, B, din, dout))
+    out = {}
+    for mode in ("fp32", "bf16", "bf16x3"):
+        rows = []
+        for name, ta, tb, M, N, K in shapes:
+            A = vk.Array(gpu, data=rng.standard_normal((K, M) if ta else (M, K), dtype=np.float32))
+            Bm = vk.Array(gpu, data=rng.standard_normal((N, K) if tb else (K, N), dtype=np.float32))
+            C = vk.Array(gpu, shape=(M, N))
+
+            def run():
+                _lib.call("vkh_gemm", gpu._handle, _lib.GEMM_MODE[mode], ta, tb, M, N, K, 1.0,
+                          A._ptr, M if ta else K, Bm._ptr, K if tb else N, 0.0, C._ptr, N,
+                          None, 0, None)
+            run()
+            gpu.wait()
+            run()
+            t = timer.start()
+            for _ in range(reps):
+                run()
+            timer.stop(t)
+            ms = timer.mean_ms() / reps
+            tflops = 2.0 * M * N * K / (ms * 1e-3) / 1e12
+            rows.append({"product": name, "M": M, "N": N, "K": K,
+                         "path": "vector" if min(M, N) <= 16 or K <= 16 else "mfma",
+                         "ms": round(ms, 4), "tflops": round(tflops, 1),
+                         "frac": round(tflops / PEAK_TFLOPS[mode], 3)})
+            del A, Bm, C
+        out[mode] = rows
+        gpu.wait()
+    return out
+
+
+def mlp_compare(vk, gpu, steps: int, warmup: int = 2, uncaptured: bool = False, timer=None):
     """
     Regression MLP Dense(256->4096)-ReLU-Dense(4096->4096)-ReLU-Dense(4096->1),
     HuberLoss(delta=1), Ridge(1e-4) on every weight (applied, see
@@ -256,6 +309,9 @@ def mlp_compare(vk, gpu, steps: int, warmup: int = 2, uncaptured: bool = False):
     if uncaptured:
         out.update(res)
         return out
+    if timer is not None:
+        log("bench: MLP per-product matrix-core utilisation")
+        out["gemm_utilisation"] = mlp_gemm_utilisation(vk, gpu, timer, B, dims)
     ref = res["fp32"]
     for name, r in res.items():
         out[name] = r
@@ -412,6 +468,25 @@ def cpu_baseline(a, col, row, p, seconds: float = 10.0):
             total_bytes += nb
         reps += 1
     dt = time.perf_counter() - t0
+    # NumPy on the same sample (single-threaded ufuncs; no Xoshiro128++ in NumPy)
+    np_suite = [
+        (12 * n, lambda: a + b),
+        (8 * n + 4 * rows, lambda: a + col),
+        (8 * n + 4 * a.shape[1], lambda: a * row),
+        (4 * n + 4 * a.shape[1], lambda: a.sum(axis=0)),
+        (4 * n + 4 * rows, lambda: a.sum(axis=1)),
+        (4 * n + 4 * a.shape[1], lambda: a.max(axis=0)),
+        (4 * n + 4 * rows, lambda: p.prod(axis=1)),
+    ]
+    for _, f in np_suite:
+        f()
+    np_bytes, np_reps, t1 = 0, 0, time.perf_counter()
+    while time.perf_counter() - t1 < seconds / 4:
+        for nb, f in np_suite:
+            f()
+            np_bytes += nb
+        np_reps += 1
+    np_dt = time.perf_counter() - t1
     return {
         "value": round(total_bytes / dt / 1e9, 3),
         "unit": "GB/s",
@@ -420,6 +495,9 @@ def cpu_baseline(a, col, row, p, seconds: float = 10.0):
         "sample": (f"oracle (C + OpenMP restatement of the reference shaders) on rows 0..{rows - 1} "
                    f"({rows}x{a.shape[1]}) of the same operands: same-shape add, broadcast add/mul, "
                    f"axis sum/max/prod, Xoshiro float; {reps} passes in {dt:.1f} s"),
+        "numpy": {"value": round(np_bytes / np_dt / 1e9, 3), "unit": "GB/s", "threads": 1,
+                  "sample": (f"NumPy ufuncs on the same sample and ops except Xoshiro; "
+                             f"{np_reps} passes in {np_dt:.1f} s")},
     }
 
 
@@ -478,7 +556,7 @@ def main() -> int:
             return 0
     mlp = None
     if args.mlp_steps > 0:
-        mlp = mlp_compare(vk, gpu, args.mlp_steps)
+        mlp = mlp_compare(vk, gpu, args.mlp_steps, timer=timer)
     gemm = None
     if not args.no_gemm:
         log("bench: GEMM report")
