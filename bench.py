@@ -16,7 +16,9 @@ numpy.random.default_rng(0), broadcast operands col (8192, 1) and row
 
 value = algorithmic bytes of all ops / wall time of the timed steps (GB/s,
 inputs resident in HBM).  The dominant kernel (broadcast A + col) is also
-timed per launch with HIP events on the library's stream for the roofline.
+timed per launch in the timed region for the roofline, with HIP events that
+its own dispatch records at the kernel's start and end on the library's
+stream (vkh_timer_next_kernel).
 The CPU baseline times the oracle (C + OpenMP, the reference's algorithms) on
 the first 512 rows of the same operands, and NumPy beside it.
 
@@ -78,6 +80,13 @@ class Timer:
         self._lib.call("vkh_timer_record", self.gpu._handle, ctypes.byref(ev))
         self.pairs.append((start, ev.value))
 
+    def next_kernel(self):
+        """Time the next elementwise kernel alone: its own dispatch records
+        the pair at the kernel's start and end (no event records around it)."""
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        self._lib.call("vkh_timer_next_kernel", self.gpu._handle, ctypes.byref(a), ctypes.byref(b))
+        self.pairs.append((a.value, b.value))
+
     def mean_ms(self) -> float:
         ms = ctypes.c_float()
         total = 0.0
@@ -107,10 +116,8 @@ def build_suite(vk, gpu, A, B, col, row, P, rng_gen, ubuf, fbuf, timer):
                  ("max", lambda x, y: x.max(y)), ("min", lambda x, y: x.min(y))):
         if o == "add":
             def timed(f=f):
-                t = timer.start()
-                r = f(A, col)
-                timer.stop(t)
-                return r
+                timer.next_kernel()
+                return f(A, col)
             add("add_col", 8 * N + colb, timed)
         else:
             add(f"{o}_col", 8 * N + colb, (lambda f: lambda: f(A, col))(f))

@@ -154,6 +154,14 @@ int vkh_event_destroy(vkh_event* ev);
  * no counterpart in the reference) and the milliseconds between two of them */
 int vkh_timer_record(vkh_gpu* gpu, vkh_event** out);
 int vkh_event_elapsed_ms(vkh_event* start, vkh_event* end, float* ms);
+/* two timing events that the next elementwise kernel launched on the context
+ * outside a graph capture (binary, unary, broadcast and clamp ops: F1-F10)
+ * records itself at its own start and end (hipExtLaunchKernelGGL), so
+ * vkh_event_elapsed_ms(start, stop) is that kernel's execution time alone,
+ * without the event records around it.  Arming again replaces an unused
+ * pair.  Destroy the events only after that kernel has run.  Refused during
+ * graph capture.  No counterpart in the reference. */
+int vkh_timer_next_kernel(vkh_gpu* gpu, vkh_event** start, vkh_event** stop);
 
 /* ---- elementwise (F1-F10) ------------------------------------------------ */
 /* F1/F2 same-shape binary and in-place (c may equal a):

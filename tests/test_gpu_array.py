@@ -526,3 +526,33 @@ def test_jobs_record_events_only_when_waited(gpu):
     gpu.wait()                                        # a device wait completes every ticket
     assert y.job.done() and ctx.latest is e
     y.job.wait(1)
+
+
+def test_timer_next_kernel(gpu):
+    """vkh_timer_next_kernel: the next elementwise kernel records the pair at
+    its own start and end.  Its result is unchanged; its time is positive and
+    no longer than that of ordinary events recorded around the same launch;
+    the pair is used once (a second op leaves it alone)."""
+    import ctypes
+    from vulkpy_amd import _lib
+    rng = np.random.default_rng(31)
+    a = rng.standard_normal((2048, 4096), dtype=np.float32)
+    c = rng.standard_normal((2048, 1), dtype=np.float32)
+    ga, gc = A(gpu, a), A(gpu, c)
+    gpu.wait()
+    h = gpu._handle
+    s, e = ctypes.c_void_p(), ctypes.c_void_p()
+    _lib.call("vkh_timer_next_kernel", h, ctypes.byref(s), ctypes.byref(e))
+    o0, o1 = ctypes.c_void_p(), ctypes.c_void_p()
+    _lib.call("vkh_timer_record", h, ctypes.byref(o0))
+    r = ga + gc
+    _lib.call("vkh_timer_record", h, ctypes.byref(o1))
+    r2 = ga * 2.0  # not timed: the pair was used by the add
+    ms, outer = ctypes.c_float(), ctypes.c_float()
+    _lib.call("vkh_event_elapsed_ms", s, e, ctypes.byref(ms))
+    _lib.call("vkh_event_elapsed_ms", o0, o1, ctypes.byref(outer))
+    for ev in (s, e, o0, o1):
+        _lib.lib.vkh_event_destroy(ev)
+    np.testing.assert_array_equal(np.asarray(r), a + c)
+    np.testing.assert_array_equal(np.asarray(r2), a * np.float32(2.0))
+    assert 0.0 < ms.value <= outer.value, (ms.value, outer.value)

@@ -80,6 +80,7 @@ _SIGNATURES = {
     "vkh_event_destroy": (c_int, [P]),
     "vkh_timer_record": (c_int, [P, POINTER(P)]),
     "vkh_event_elapsed_ms": (c_int, [P, P, POINTER(c_float)]),
+    "vkh_timer_next_kernel": (c_int, [P, POINTER(P), POINTER(P)]),
     "vkh_binary": (c_int, [P, c_int, P, P, P, U64]),
     "vkh_binary_scalar": (c_int, [P, c_int, P, c_float, P, U64]),
     "vkh_binary_bcast": (c_int, [P, c_int, P, P, P, c_int, _u64p, _u64p, _u64p]),

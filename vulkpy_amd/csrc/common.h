@@ -120,6 +120,9 @@ struct vkh_gpu {
   // reduction kernel configuration per (family, prev, axis, post), chosen by
   // vkh_reduce_autotune or vkh_reduce_set_config (reduce.hip)
   std::map<std::array<uint64_t, 4>, int> red_cfg;
+  // events armed by vkh_timer_next_kernel: the next elementwise kernel's
+  // dispatch records them at its own start and end (elementwise.hip)
+  hipEvent_t ktime_start = nullptr, ktime_stop = nullptr;
 };
 
 struct vkh_event {

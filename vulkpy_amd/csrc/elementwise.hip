@@ -9,6 +9,8 @@
 // each operand is FULL (same layout as the output), ROW (indexed by column),
 // COL (indexed by row) or SCALAR.  Those run with 128-bit loads/stores and a
 // grid-stride loop; anything else takes a general strided N-d kernel.
+#include <hip/hip_ext.h>
+
 #include "common.h"
 #include "fastdiv.h"
 
@@ -124,17 +126,31 @@ k_ew2d_kind(F f, Opnd a, Opnd b, Opnd c, float* out, uint32_t total4, FastDiv co
   }
 }
 
+// Every elementwise launch goes through here.  When vkh_timer_next_kernel has
+// armed the context's timing events, the kernel is launched with
+// hipExtLaunchKernelGGL, whose dispatch records them at the kernel's own
+// start and end, and the events are disarmed; otherwise a plain launch.
+template <typename... KArgs, typename... Args>
+static void ew_launch(vkh_gpu* g, void (*kernel)(KArgs...), dim3 grid, Args... args) {
+  if (g->ktime_start && !g->capturing) {
+    hipEvent_t a = g->ktime_start, b = g->ktime_stop;
+    g->ktime_start = g->ktime_stop = nullptr;
+    hipExtLaunchKernelGGL(kernel, grid, dim3(EW_BLOCK), 0, g->stream, a, b, 0, args...);
+  } else {
+    hipLaunchKernelGGL(kernel, grid, dim3(EW_BLOCK), 0, g->stream, args...);
+  }
+}
+
 // Launches k_ew2d_kind when the operand kinds are one of the streaming
 // combinations (at least one full-size operand); returns false otherwise.
 template <class F>
-static bool launch_kind(dim3 grid, hipStream_t s, const F& f, const Opnd* op, float* out, uint32_t total4,
+static bool launch_kind(dim3 grid, vkh_gpu* g, const F& f, const Opnd* op, float* out, uint32_t total4,
                         FastDiv fd) {
   const int ka = op[0].kind, kb = op[1].kind, kc = op[2].kind;
-#define VKH_KIND(A, B, C)                                                                       \
-  if (ka == A && kb == B && kc == C) {                                                          \
-    hipLaunchKernelGGL((k_ew2d_kind<F, A, B, C>), grid, dim3(EW_BLOCK), 0, s, f, op[0], op[1], \
-                       op[2], out, total4, fd);                                                 \
-    return true;                                                                                \
+#define VKH_KIND(A, B, C)                                                                     \
+  if (ka == A && kb == B && kc == C) {                                                        \
+    ew_launch(g, k_ew2d_kind<F, A, B, C>, grid, f, op[0], op[1], op[2], out, total4, fd);     \
+    return true;                                                                              \
   }
   if constexpr (F::nin == 1) {
     VKH_KIND(K_FULL, K_SCALAR, K_SCALAR)
@@ -311,25 +327,21 @@ static void run_ew(vkh_gpu* g, const F& f, const std::vector<InSpec>& ins,
       if (total4 < (uint64_t(1) << 31)) {
         unsigned grid = full_grid(total4, EW_BLOCK);
         FastDiv fd((uint32_t)cols4);
-        if (launch_kind(dim3(grid), g->stream, f, op, out, (uint32_t)total4, fd))
+        if (launch_kind(dim3(grid), g, f, op, out, (uint32_t)total4, fd))
           ;  // specialised kernel
         else if (need_rc)
-          hipLaunchKernelGGL((k_ew2d_vec<F, true>), dim3(grid), dim3(EW_BLOCK), 0, g->stream,
-                             f, op[0], op[1], op[2], out, (uint32_t)total4, fd);
+          ew_launch(g, k_ew2d_vec<F, true>, dim3(grid), f, op[0], op[1], op[2], out, (uint32_t)total4, fd);
         else
-          hipLaunchKernelGGL((k_ew2d_vec<F, false>), dim3(grid), dim3(EW_BLOCK), 0, g->stream,
-                             f, op[0], op[1], op[2], out, (uint32_t)total4, fd);
+          ew_launch(g, k_ew2d_vec<F, false>, dim3(grid), f, op[0], op[1], op[2], out, (uint32_t)total4, fd);
       } else {
         unsigned grid = stream_grid(g, total4, EW_BLOCK, 8);
-        hipLaunchKernelGGL((k_ew2d_vec64<F>), dim3(grid), dim3(EW_BLOCK), 0, g->stream,
-                           f, op[0], op[1], op[2], out, total4, cols4);
+        ew_launch(g, k_ew2d_vec64<F>, dim3(grid), f, op[0], op[1], op[2], out, total4, cols4);
       }
       check_launch("elementwise (vector)");
     }
     if (vec_elems < pl.total) {
       unsigned grid = stream_grid(g, pl.total - vec_elems, EW_BLOCK, 8);
-      hipLaunchKernelGGL((k_ew2d_scalar<F>), dim3(grid), dim3(EW_BLOCK), 0, g->stream,
-                         f, op[0], op[1], op[2], out, vec_elems, pl.total, cols);
+      ew_launch(g, k_ew2d_scalar<F>, dim3(grid), f, op[0], op[1], op[2], out, vec_elems, pl.total, cols);
       check_launch("elementwise (scalar)");
     }
     return;
@@ -344,7 +356,7 @@ static void run_ew(vkh_gpu* g, const F& f, const std::vector<InSpec>& ins,
     for (int j = 0; j < nd; ++j) d.st[k][j] = (k < nin && ins[k].p) ? pl.st[k][j] : 0;
   }
   unsigned grid = stream_grid(g, pl.total, EW_BLOCK, 8);
-  hipLaunchKernelGGL((k_ewnd<F>), dim3(grid), dim3(EW_BLOCK), 0, g->stream, f, d, out, pl.total);
+  ew_launch(g, k_ewnd<F>, dim3(grid), f, d, out, pl.total);
   check_launch("elementwise (n-d)");
 }
 
@@ -413,14 +425,13 @@ static void run_unary(vkh_gpu* g, const float* a, float* b, uint64_t n) {
     vec = n4 * 4;
     if (n4) {
       unsigned grid = full_grid(n4, EW_BLOCK);
-      hipLaunchKernelGGL(k_unary_vec<FN>, dim3(grid), dim3(EW_BLOCK), 0, g->stream,
-                         (const float4*)a, (float4*)b, n4);
+      ew_launch(g, k_unary_vec<FN>, dim3(grid), (const float4*)a, (float4*)b, n4);
       check_launch("unary (vector)");
     }
   }
   if (vec < n) {
     unsigned grid = stream_grid(g, n - vec, EW_BLOCK, 8);
-    hipLaunchKernelGGL(k_unary<FN>, dim3(grid), dim3(EW_BLOCK), 0, g->stream, a, b, vec, n);
+    ew_launch(g, k_unary<FN>, dim3(grid), a, b, vec, n);
     check_launch("unary");
   }
 }

@@ -436,6 +436,29 @@ int vkh_timer_record(vkh_gpu* g, vkh_event** out) {
   });
 }
 
+int vkh_timer_next_kernel(vkh_gpu* g, vkh_event** start, vkh_event** stop) {
+  return guarded([&] {
+    use(g);
+    VKH_REQUIRE(start && stop, "null out");
+    VKH_REQUIRE(!g->capturing, "timer_next_kernel: not allowed during graph capture");
+    vkh_event* ev[2] = {nullptr, nullptr};
+    for (auto& e : ev) {
+      e = new vkh_event();
+      e->device = g->device;
+      const hipError_t r = hipEventCreate(&e->ev);
+      if (r != hipSuccess) {
+        if (ev[0] && ev[0] != e) { (void)hipEventDestroy(ev[0]->ev); delete ev[0]; }
+        delete e;
+        check_hip(r, "hipEventCreate");
+      }
+    }
+    g->ktime_start = ev[0]->ev;
+    g->ktime_stop = ev[1]->ev;
+    *start = ev[0];
+    *stop = ev[1];
+  });
+}
+
 int vkh_event_elapsed_ms(vkh_event* a, vkh_event* b, float* ms) {
   return guarded([&] {
     VKH_REQUIRE(a && b && ms, "null argument");
