@@ -417,15 +417,14 @@ struct F32Stager {
       } else {
         gr = k0 + idx / (ROWS / 4); gc = row0 + (idx % (ROWS / 4)) * 4; rlim = K; clim = nrows;
       }
-      const float* q = P + gr * ld + gc;
       if constexpr (VEC) {
         const bool ok = gr < rlim && gc < clim;
-        r[i] = *reinterpret_cast<const f32x4_t*>(ok ? q : k_f32_zeros);
+        r[i] = *reinterpret_cast<const f32x4_t*>(ok ? P + (gr * ld + gc) : k_f32_zeros);
       } else {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const bool ok = gr < rlim && gc + e < clim;
-          r[i][e] = *(ok ? q + e : k_f32_zeros);
+          r[i][e] = *(ok ? P + (gr * ld + gc + e) : k_f32_zeros);
         }
       }
     }
