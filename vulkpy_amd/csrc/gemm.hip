@@ -913,6 +913,9 @@ static const PackedCfg BF16_CFGS[] = {
     {256, 128, 64, 4, launch_packed<256, 128, 64, false, 2, 2>},
     {128, 256, 64, 4, launch_packed<128, 256, 64, false, 2, 2>},
     {256, 256, 64, 8, launch_direct<256, 256, 2, 4>, 1},
+    // (direct loads with 4 waves of 128 x 128, a third less LDS fragment
+    // traffic, one wave per SIMD: 894 against 977 TFLOP/s at 8192^3 and
+    // slower on every MLP shape, so not kept)
 };
 static const PackedCfg SPLIT_CFGS[] = {
     {128, 128, 32, 4, launch_packed<128, 128, 32, true, 2, 2>},
