@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/vulkpy_hip.h"
+#include "bmath.h"
 
 namespace vkh {
 
@@ -234,10 +235,10 @@ __device__ __forceinline__ float binop_apply(int op, float a, float b) {
     case VKH_DIV: return a / b;
     case VKH_MAX: return np_max(a, b);
     case VKH_MIN: return np_min(a, b);
-    case VKH_POW: return (float)pow((double)a, (double)b);
+    case VKH_POW: return pow_f32(a, b);
     case VKH_RSUB: return b - a;
     case VKH_RDIV: return b / a;
-    case VKH_RPOW: return (float)pow((double)b, (double)a);
+    case VKH_RPOW: return pow_f32(b, a);
   }
   return 0.0f;
 }
@@ -250,10 +251,10 @@ __device__ __forceinline__ float binop(float a, float b) {
   else if constexpr (OP == VKH_DIV) return a / b;
   else if constexpr (OP == VKH_MAX) return np_max(a, b);
   else if constexpr (OP == VKH_MIN) return np_min(a, b);
-  else if constexpr (OP == VKH_POW) return (float)pow((double)a, (double)b);
+  else if constexpr (OP == VKH_POW) return pow_f32(a, b);
   else if constexpr (OP == VKH_RSUB) return b - a;
   else if constexpr (OP == VKH_RDIV) return b / a;
-  else return (float)pow((double)b, (double)a);
+  else return pow_f32(b, a);
 }
 
 // Dispatch a runtime binary op code to a template parameter.

@@ -11,6 +11,7 @@
 // grid-stride loop; anything else takes a general strided N-d kernel.
 #include <hip/hip_ext.h>
 
+#include "bmath.h"
 #include "common.h"
 #include "fastdiv.h"
 
@@ -387,14 +388,14 @@ __device__ __forceinline__ float unop(float x) {
   else if constexpr (FN == VKH_ATAN) return (float)atan((double)x);
   else if constexpr (FN == VKH_SINH) return (float)sinh((double)x);
   else if constexpr (FN == VKH_COSH) return (float)cosh((double)x);
-  else if constexpr (FN == VKH_TANH) return (float)tanh((double)x);
+  else if constexpr (FN == VKH_TANH) return tanh_f32(x);
   else if constexpr (FN == VKH_ASINH) return (float)asinh((double)x);
   else if constexpr (FN == VKH_ACOSH) return (float)acosh((double)x);
   else if constexpr (FN == VKH_ATANH) return (float)atanh((double)x);
   else if constexpr (FN == VKH_EXP) return (float)exp((double)x);
-  else if constexpr (FN == VKH_LOG) return (float)log((double)x);
+  else if constexpr (FN == VKH_LOG) return log_f32(x);
   else if constexpr (FN == VKH_EXP2) return (float)exp2((double)x);
-  else if constexpr (FN == VKH_LOG2) return (float)log2((double)x);
+  else if constexpr (FN == VKH_LOG2) return log2_f32(x);
   else if constexpr (FN == VKH_SQRT) return sqrtf(x);
   else return (float)(1.0 / sqrt((double)x));
 }
