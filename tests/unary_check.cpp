@@ -1,11 +1,13 @@
-// Host check of tanh_f32, log_f32 and log2_f32 (vulkpy_amd/csrc/bmath.h):
-// every FP32 input whose bit pattern is a multiple of the stride (argv[1];
-// 1 = every input) in the ranges where the functions compute -- tanh on
-// [0, 9.5) and both signs, log / log2 on all positive finite floats -- against
-// the FP64 library functions.  Prints the largest error of each in units of
-// the last place of the FP64 reference rounded to float, the number of
-// inputs, and the number of special-value mismatches (0, -0, negative, inf,
-// NaN, 1, powers of two).
+// Host check of the FP32 transcendentals of vulkpy_amd/csrc/bmath.h (tanh,
+// log, log2, pow, sinh, cosh, asinh, acosh, atanh) against the FP64 library
+// functions: every FP32 input whose bit pattern is a multiple of the stride
+// (argv[1]; 1 = every input) in the range where each function computes (both
+// signs where the function is defined for both), and for pow every positive
+// finite base at 64 times the stride against 31 exponents.  Prints the
+// largest error of each function in units of the last place of the FP64
+// reference rounded to float, in the order above, then the number of inputs
+// checked and the number of special-value mismatches (signed zeros,
+// negatives, infinities, NaN, 1, powers of two, domain edges).
 #include "../vulkpy_amd/csrc/bmath.h"
 
 #include <cmath>
@@ -32,51 +34,93 @@ static bool same(float a, float b) {
   return (std::isnan(a) && std::isnan(b)) || (a == b && std::signbit(a) == std::signbit(b));
 }
 
+struct Fn {
+  float (*f)(float);
+  double (*ref)(double);
+  uint32_t lo, hi;  // bit patterns [lo, hi) of the magnitudes checked
+  bool both_signs;
+};
+
+static double d_tanh(double x) { return std::tanh(x); }
+static double d_log(double x) { return std::log(x); }
+static double d_log2(double x) { return std::log2(x); }
+static double d_sinh(double x) { return std::sinh(x); }
+static double d_cosh(double x) { return std::cosh(x); }
+static double d_asinh(double x) { return std::asinh(x); }
+static double d_acosh(double x) { return std::acosh(x); }
+static double d_atanh(double x) { return std::atanh(x); }
+
 int main(int argc, char** argv) {
   const uint64_t stride = argc > 1 ? std::strtoull(argv[1], nullptr, 10) : 1;
-  double et = 0, el = 0, e2 = 0;
+  const Fn fns[] = {
+      {vkh::tanh_f32, d_tanh, 0u, 0x41180000u, true},        // |x| < 9.5
+      {vkh::log_f32, d_log, 1u, 0x7f800000u, false},         // positive finite
+      {vkh::log2_f32, d_log2, 1u, 0x7f800000u, false},
+      {nullptr, nullptr, 0u, 0u, false},                      // pow, below
+      {vkh::sinh_f32, d_sinh, 0u, 0x42b30000u, true},        // |x| < 89.5
+      {vkh::cosh_f32, d_cosh, 0u, 0x42b30000u, true},
+      {vkh::asinh_f32, d_asinh, 0u, 0x7f800000u, true},      // all finite
+      {vkh::acosh_f32, d_acosh, 0x3f800000u, 0x7f800000u, false},  // [1, max]
+      {vkh::atanh_f32, d_atanh, 0u, 0x3f800000u, true},      // |x| < 1
+  };
+  constexpr int NF = sizeof(fns) / sizeof(fns[0]);
+  double err[NF] = {};
   uint64_t n = 0;
-  const uint32_t tanh_end = 0x41180000u;  // 9.5f
-  for (uint64_t b = 0; b < tanh_end; b += stride, ++n) {
-    const float x = from_bits((uint32_t)b);
-    const double r = std::tanh((double)x);
-    et = std::fmax(et, ulp_err(vkh::tanh_f32(x), r));
-    et = std::fmax(et, ulp_err(vkh::tanh_f32(-x), -r));
+  for (int i = 0; i < NF; ++i) {
+    const Fn& F = fns[i];
+    if (!F.f) continue;
+    double e = 0;
+    uint64_t c = 0;
+#pragma omp parallel for reduction(max : e) reduction(+ : c) schedule(static)
+    for (int64_t b = F.lo; b < (int64_t)F.hi; b += (int64_t)stride) {
+      const float x = from_bits((uint32_t)b);
+      const double r = F.ref((double)x);
+      e = std::fmax(e, ulp_err(F.f(x), r));
+      ++c;
+      if (F.both_signs) {
+        e = std::fmax(e, ulp_err(F.f(-x), F.ref(-(double)x)));
+        ++c;
+      }
+    }
+    err[i] = e;
+    n += c;
   }
-  for (uint64_t b = 1; b < 0x7f800000u; b += stride, ++n) {
-    const float x = from_bits((uint32_t)b);
-    el = std::fmax(el, ulp_err(vkh::log_f32(x), std::log((double)x)));
-    e2 = std::fmax(e2, ulp_err(vkh::log2_f32(x), std::log2((double)x)));
+  // pow: positive finite bases at 64 times the stride against a fixed set of
+  // exponents
+  const float exps[] = {0.0f, 1.0f, -1.0f, 0.5f, -0.5f, 2.0f, 3.0f, -2.0f, 0.37f, 1.1f, 2.7f, -3.3f,
+                        7.25f, -7.25f, 12.5f, 31.0f, -31.0f, 1e-3f, -1e-3f, 1e-7f, 100.0f, -100.0f,
+                        150.0f, -150.0f, 0.999f, 1.001f, 23.4f, -23.4f, 64.0f, -64.0f, 1e4f};
+  {
+    double e = 0;
+    uint64_t c = 0;
+#pragma omp parallel for reduction(max : e) reduction(+ : c) schedule(static)
+    for (int64_t b = 1; b < 0x7f800000; b += 64 * (int64_t)stride) {
+      const float a = from_bits((uint32_t)b);
+      for (float x : exps) {
+        e = std::fmax(e, ulp_err(vkh::pow_f32(a, x), std::pow((double)a, (double)x)));
+        ++c;
+      }
+    }
+    err[3] = e;
+    n += c;
   }
   int bad = 0;
-  const float specials[] = {0.0f, -0.0f, -1.0f, -INFINITY, INFINITY, NAN, 1.0f, 1e-45f, 3.4e38f};
-  for (float x : specials) {
-    bad += !same(vkh::tanh_f32(x), (float)std::tanh((double)x));
-    bad += !same(vkh::log_f32(x), (float)std::log((double)x));
-    bad += !same(vkh::log2_f32(x), (float)std::log2((double)x));
+  const float specials[] = {0.0f, -0.0f, -1.0f, 1.0f, -INFINITY, INFINITY, NAN, 1e-45f, -1e-45f,
+                            3.4e38f, -3.4e38f, 100.0f, -100.0f, 0.99999994f, 1.0000001f, 2.0f};
+  for (const Fn& F : fns) {
+    if (!F.f) continue;
+    for (float x : specials) bad += !same(F.f(x), (float)F.ref((double)x));
   }
   for (int k = -149; k < 128; ++k) {
     const float x = std::ldexp(1.0f, k);
     bad += vkh::log2_f32(x) != (float)k;
   }
   bad += vkh::log_f32(1.0f) != 0.0f;
-  // pow: positive finite bases at 64 times the stride against a fixed set of
-  // exponents, and the special cases against the FP64 pow rounded
-  const float exps[] = {0.0f, 1.0f, -1.0f, 0.5f, -0.5f, 2.0f, 3.0f, -2.0f, 0.37f, 1.1f, 2.7f, -3.3f,
-                        7.25f, -7.25f, 12.5f, 31.0f, -31.0f, 1e-3f, -1e-3f, 1e-7f, 100.0f, -100.0f,
-                        150.0f, -150.0f, 0.999f, 1.001f, 23.4f, -23.4f, 64.0f, -64.0f, 1e4f};
-  double ep = 0;
-  for (uint64_t b = 1; b < 0x7f800000u; b += 64 * stride) {
-    const float a = from_bits((uint32_t)b);
-    for (float e : exps) {
-      ep = std::fmax(ep, ulp_err(vkh::pow_f32(a, e), std::pow((double)a, (double)e)));
-      ++n;
-    }
-  }
   const float pbases[] = {0.0f, -0.0f, -2.0f, -0.5f, 1.0f, INFINITY, -INFINITY, NAN, 2.0f};
   const float pexps[] = {0.0f, -0.0f, 1.0f, 2.0f, -3.0f, 0.5f, INFINITY, -INFINITY, NAN};
   for (float a : pbases)
-    for (float e : pexps) bad += !same(vkh::pow_f32(a, e), (float)std::pow((double)a, (double)e));
-  std::printf("%.4f %.4f %.4f %.4f %llu %d\n", et, el, e2, ep, (unsigned long long)n, bad);
+    for (float x : pexps) bad += !same(vkh::pow_f32(a, x), (float)std::pow((double)a, (double)x));
+  for (int i = 0; i < NF; ++i) std::printf("%.4f ", err[i]);
+  std::printf("%llu %d\n", (unsigned long long)n, bad);
   return 0;
 }

@@ -52,27 +52,39 @@ __host__ __device__ inline float tanh_f32(float x) {
   return copysignf((float)((e - 1.0) / (e + 1.0)), x);
 }
 
-// Natural log of the FP64 value of an FP32 input, before rounding: with
-// x = m 2^k, m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s) = 2 (s + s^3/3 +
-// ... + s^15/15) for s = (m - 1) / (m + 1), |s| <= 0.1716, whose omitted
-// terms are below 2^-45 relative.  k_out receives k and the result is log m;
-// x = 1 gives exactly 0.  x must be positive and finite.
-__host__ __device__ inline double log_mant(float x, int& k_out) {
-  int k;
-  double m = frexp((double)x, &k);  // m in [0.5, 1)
-  if (m < 0.70710678118654752) {
-    m *= 2.0;
-    --k;
-  }
-  const double s = (m - 1.0) / (m + 1.0), z = s * s;
+// atanh(s) = s + s^3/3 + ... + s^15/15 for |s| <= 0.1716, whose omitted
+// terms are below 2^-45 relative
+__host__ __device__ inline double atanh_series(double s) {
+  const double z = s * s;
   double p = fma(z, 1.0 / 15.0, 1.0 / 13.0);
   p = fma(z, p, 1.0 / 11.0);
   p = fma(z, p, 1.0 / 9.0);
   p = fma(z, p, 1.0 / 7.0);
   p = fma(z, p, 1.0 / 5.0);
   p = fma(z, p, 1.0 / 3.0);
+  return fma(s * z, p, s);
+}
+
+// Natural log before rounding: with x = m 2^k, m in [sqrt(1/2), sqrt(2)),
+// log m = 2 atanh(s) for s = (m - 1) / (m + 1), |s| <= 0.1716.  k_out
+// receives k and the result is log m; x = 1 gives exactly 0.  x must be
+// positive and finite.
+__host__ __device__ inline double log_mant(double x, int& k_out) {
+  int k;
+  double m = frexp(x, &k);  // m in [0.5, 1)
+  if (m < 0.70710678118654752) {
+    m *= 2.0;
+    --k;
+  }
   k_out = k;
-  return fma(2.0 * s * z, p, 2.0 * s);
+  return 2.0 * atanh_series((m - 1.0) / (m + 1.0));
+}
+
+// log of a positive finite double, FP64 error below 2^-44 relative
+__host__ __device__ inline double log_pos(double x) {
+  int k;
+  const double lm = log_mant(x, k);
+  return fma((double)k, 0.69314718055994531, lm);
 }
 
 // log and log2 of FP32 x through log_mant: k ln 2 + log m and k + log m /
@@ -106,6 +118,60 @@ __host__ __device__ inline float pow_f32(float a, float b) {
     return (float)exp((double)b * fma((double)k, 0.69314718055994531, lm));
   }
   return (float)pow((double)a, (double)b);
+}
+
+// sinh: |x| < 2^-5: x (1 + x^2/6 + x^4/120 + x^6/5040), omitted terms below
+// 2^-45 relative; otherwise (e - 1/e) / 2 with e = exp(|x|), where the
+// difference (>= 2^-4) keeps all but 5 bits of e's precision.  Overflow to
+// inf, NaN and the sign of zero follow the FP64 function.
+__host__ __device__ inline float sinh_f32(float x) {
+  const double d = (double)x, a = fabs(d);
+  if (a < 0x1p-5) {
+    const double z = d * d;
+    return (float)(d * fma(z, fma(z, fma(z, 1.0 / 5040.0, 1.0 / 120.0), 1.0 / 6.0), 1.0));
+  }
+  const double e = exp(a);
+  return copysignf((float)(0.5 * (e - 1.0 / e)), x);
+}
+
+// cosh: (e + 1/e) / 2 with e = exp(|x|) (no cancellation)
+__host__ __device__ inline float cosh_f32(float x) {
+  const double e = exp(fabs((double)x));
+  return (float)(0.5 * (e + 1.0 / e));
+}
+
+// asinh: |x| < 2^-5: x (1 - x^2/6 + 3 x^4/40 - 5 x^6/112), omitted terms
+// below 2^-44 relative; otherwise log(|x| + sqrt(x^2 + 1)) (x^2 fits FP64
+// for every FP32 x), whose argument is at least 1 + 2^-5.
+__host__ __device__ inline float asinh_f32(float x) {
+  const double d = (double)x, a = fabs(d);
+  if (a < 0x1p-5) {
+    const double z = d * d;
+    return (float)(d * fma(z, fma(z, fma(z, -5.0 / 112.0, 3.0 / 40.0), -1.0 / 6.0), 1.0));
+  }
+  if (!(a < INFINITY)) return x;  // inf, NaN
+  return copysignf((float)log_pos(a + sqrt(fma(a, a, 1.0))), x);
+}
+
+// acosh (x >= 1): log(x + sqrt(x^2 - 1)); x^2 - 1 is exact in FP64 for FP32
+// x, so near x = 1 the result keeps its relative precision.  x < 1 and NaN
+// give NaN, inf gives inf.
+__host__ __device__ inline float acosh_f32(float x) {
+  const double d = (double)x;
+  if (!(d >= 1.0)) return NAN;
+  if (d == INFINITY) return x;
+  return (float)log_pos(d + sqrt(fma(d, d, -1.0)));
+}
+
+// atanh (|x| <= 1): |x| <= 0.1716: the series; otherwise
+// log((1 + |x|) / (1 - |x|)) / 2, where 1 +- |x| are exact in FP64 and the
+// log is at least 0.34.  +-1 give +-inf, |x| > 1 and NaN give NaN.
+__host__ __device__ inline float atanh_f32(float x) {
+  const double d = (double)x, a = fabs(d);
+  if (a <= 0.1716) return (float)atanh_series(d);
+  if (!(a <= 1.0)) return NAN;
+  if (a == 1.0) return copysignf(INFINITY, x);
+  return copysignf((float)(0.5 * log_pos((1.0 + a) / (1.0 - a))), x);
 }
 
 }  // namespace vkh

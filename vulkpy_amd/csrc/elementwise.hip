@@ -386,12 +386,12 @@ __device__ __forceinline__ float unop(float x) {
   else if constexpr (FN == VKH_ASIN) return (float)asin((double)x);
   else if constexpr (FN == VKH_ACOS) return (float)acos((double)x);
   else if constexpr (FN == VKH_ATAN) return (float)atan((double)x);
-  else if constexpr (FN == VKH_SINH) return (float)sinh((double)x);
-  else if constexpr (FN == VKH_COSH) return (float)cosh((double)x);
+  else if constexpr (FN == VKH_SINH) return sinh_f32(x);
+  else if constexpr (FN == VKH_COSH) return cosh_f32(x);
   else if constexpr (FN == VKH_TANH) return tanh_f32(x);
-  else if constexpr (FN == VKH_ASINH) return (float)asinh((double)x);
-  else if constexpr (FN == VKH_ACOSH) return (float)acosh((double)x);
-  else if constexpr (FN == VKH_ATANH) return (float)atanh((double)x);
+  else if constexpr (FN == VKH_ASINH) return asinh_f32(x);
+  else if constexpr (FN == VKH_ACOSH) return acosh_f32(x);
+  else if constexpr (FN == VKH_ATANH) return atanh_f32(x);
   else if constexpr (FN == VKH_EXP) return (float)exp((double)x);
   else if constexpr (FN == VKH_LOG) return log_f32(x);
   else if constexpr (FN == VKH_EXP2) return (float)exp2((double)x);
