@@ -119,11 +119,11 @@ def test_graph_packed_matches_eager_with_interleaved_gemms(gpu, precision):
         gpu.wait()
 
 
-@pytest.mark.parametrize("precision", ["bf16", "bf16x3"])
+@pytest.mark.parametrize("precision", ["bf16", "bf16x3", "fp32"])
 def test_graph_split_k_matches_eager(gpu, precision):
     """At batch 8192 the weight gradients of the 64 -> 256 -> 64 layers split
-    k into 8 chunks; the captured step (chunk partials in its packing region)
-    stays identical to the eager step."""
+    k into chunks (8 in the BF16 modes, 32 in FP32); the captured step (chunk
+    partials in its packing region) stays identical to the eager step."""
     dims, acts = [(64, 256), (256, 64), (64, 4)], ["relu", "sigmoid", None]
     B = 8192
     rng = np.random.default_rng(13)
