@@ -357,8 +357,9 @@ int vkh_gemm_ex(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
                 uint16_t* c_bf16, const float* aux, uint64_t ldaux,
                 int aux_act);
 /* vkh_gemm_ex with op(A) also given as BF16 in the layout the BF16 packing
- * pass writes: a_bf16 holds rows of K BF16 values (A = op(A), trans_a = 0),
- * with M rounded up to 256 rows allocated, K a multiple of 64 -- for
+ * pass writes: a_bf16 holds the M rows of op(A), K BF16 values each, with M
+ * rounded up to 256 rows allocated (the extra rows are read but reach no
+ * element of C), K a multiple of 64 -- for
  * instance the c_bf16 output of the GEMM that produced A.  In BF16 mode,
  * when the matrix-core path runs, A's packing pass is skipped; otherwise
  * a_bf16 is ignored.  The result is identical either way.  No counterpart in
@@ -371,6 +372,24 @@ int vkh_gemm_bf16a(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
                    float beta, float* C, uint64_t ldc, const float* bias,
                    int act, uint16_t* c_bf16, const float* aux,
                    uint64_t ldaux, int aux_act, const uint16_t* a_bf16);
+/* vkh_gemm_bf16a that also takes op(B) as BF16 and can write C^T as BF16.
+ * b_bf16 (optional) holds the N rows of op(B)^T, K BF16 values each, N
+ * rounded up to 256 rows allocated (as for a_bf16), K a multiple of 64;
+ * in BF16 mode on the matrix-core path B's packing pass is skipped.
+ * c_bf16t (optional) receives C^T rounded to BF16: c_bf16t[n * ldt + m],
+ * ldt >= M -- with ldt = M = the next product's K it is that product's
+ * packed op(A) or op(B)^T when C is its op(A)^T or op(B).  The FP32 result
+ * is the one vkh_gemm_ex computes.  No counterpart in the reference: it
+ * lets a captured training step hand activations and input gradients to
+ * the weight-gradient products, which read them transposed, without a
+ * transposing pack. */
+int vkh_gemm_bf16io(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
+                    uint64_t M, uint64_t N, uint64_t K, float alpha,
+                    const float* A, uint64_t lda, const float* B, uint64_t ldb,
+                    float beta, float* C, uint64_t ldc, const float* bias,
+                    int act, uint16_t* c_bf16, const float* aux,
+                    uint64_t ldaux, int aux_act, const uint16_t* a_bf16,
+                    const uint16_t* b_bf16, uint16_t* c_bf16t, uint64_t ldt);
 
 /* ---- nn fused kernels (F16, F19 and the nn package compositions) ------- */
 /* y = act(x) elementwise: ReLU = x.max(0.0) (nn/layers.py:185),

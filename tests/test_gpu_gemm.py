@@ -305,3 +305,56 @@ def test_prepacked_bf16_operand(gpu):
             conv._set_job([gC])
             np.testing.assert_array_equal(np.asarray(conv)[: M * N // 2],
                                           outs[(mode, True)][1][: M * N // 2], err_msg=mode)
+
+
+def bf16_bits(x):
+    """round-to-nearest-even BF16 bits of finite float32 values"""
+    b = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    return ((b + 0x7FFF + ((b >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_bf16io_transposed_output_and_packed_b(gpu, ta, tb):
+    """vkh_gemm_bf16io: c_bf16t[n * ldt + m] is the BF16 rounding of C[m, n]
+    on every path (matrix cores, split k, skinny, short k), for a 16-byte
+    aligned and an unaligned ldt, and nothing else of c_bf16t is written;
+    op(A) and op(B)^T given as packed BF16 rows leave C's bits unchanged."""
+    rng = np.random.default_rng(41 + 2 * ta + tb)
+    shapes = ((300, 260, 256, "relu"), (256, 192, 8192, "none"), (8, 300, 128, "none"),
+              (300, 8, 64, "none"), (300, 260, 8, "none"), (130, 70, 64, "sigmoid"))
+    for M, N, K, act in shapes:
+        A = rng.standard_normal((K, M) if ta else (M, K)).astype(np.float32)
+        Bm = rng.standard_normal((N, K) if tb else (K, N)).astype(np.float32)
+        bias = rng.standard_normal(N).astype(np.float32)
+        gA, gB, gbias = vk.Array(gpu, data=A), vk.Array(gpu, data=Bm), vk.Array(gpu, data=bias)
+        h, mode = gpu._handle, _lib.GEMM_MODE["bf16"]
+
+        def run(a_pre=None, b_pre=None, ct=None, ldt=0):
+            C = vk.Array(gpu, shape=(M, N))
+            _lib.call("vkh_gemm_bf16io", h, mode, ta, tb, M, N, K, 1.0, gA._ptr, A.shape[1], gB._ptr,
+                      Bm.shape[1], 0.0, C._ptr, N, gbias._ptr, _lib.ACT[act], None, None, 0, 0,
+                      a_pre._ptr if a_pre else None, b_pre._ptr if b_pre else None,
+                      ct._ptr if ct else None, ldt)
+            C._set_job([gA, gB] + [t for t in (a_pre, b_pre, ct) if t])
+            if ct:
+                ct._set_job([])   # written by the product: drop the host mirror
+            return np.asarray(C).copy()
+
+        C_ref = run()
+        want_t = bf16_bits(C_ref).T
+        for ldt in ((M + 7) // 8 * 8 + 8, M + 5):
+            ct = vk.U32Array(gpu, data=np.full((N * ldt + 1) // 2, 0xFFFFFFFF, np.uint32))
+            np.testing.assert_array_equal(run(ct=ct, ldt=ldt), C_ref, err_msg=(M, N, K))
+            got = np.asarray(ct).view(np.uint16)[: N * ldt].reshape(N, ldt)
+            np.testing.assert_array_equal(got[:, :M], want_t, err_msg=(M, N, K, ldt))
+            assert (got[:, M:] == 0xFFFF).all(), (M, N, K, ldt)
+        # packed operands: op(A) rows and op(B)^T rows, 256-row padded, zero tails
+        pre = []
+        for rowsmat in (A.T if ta else A, Bm if tb else Bm.T):
+            R = rowsmat.shape[0]
+            buf = np.zeros(-(-R // 256) * 256 * K, np.uint16)
+            buf[: R * K] = bf16_bits(rowsmat).ravel()
+            pre.append(vk.U32Array(gpu, data=buf.view(np.uint32)))
+        np.testing.assert_array_equal(run(a_pre=pre[0]), C_ref, err_msg=(M, N, K))
+        np.testing.assert_array_equal(run(b_pre=pre[1]), C_ref, err_msg=(M, N, K))
+        np.testing.assert_array_equal(run(a_pre=pre[0], b_pre=pre[1]), C_ref, err_msg=(M, N, K))

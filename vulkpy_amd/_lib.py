@@ -123,7 +123,9 @@ _SIGNATURES = {
                             c_float, P, U64, P, c_int, P, P, U64, c_int]),
     "vkh_gemm_bf16a": (c_int, [P, c_int, c_int, c_int, U64, U64, U64, c_float, P, U64, P, U64,
                                c_float, P, U64, P, c_int, P, P, U64, c_int, P]),
-    "vkh_activation": (c_int, [P, c_int, P, P, U64]),
+    "vkh_gemm_bf16io": (c_int, [P, c_int, c_int, c_int, U64, U64, U64, c_float, P, U64, P, U64,
+                                c_float, P, U64, P, c_int, P, P, U64, c_int, P, P, P, U64]),
+    "vkh_activation":(c_int, [P, c_int, P, P, U64]),
     "vkh_activation_backward": (c_int, [P, c_int, P, P, P, U64]),
     "vkh_softmax": (c_int, [P, P, P, U64, U64]),
     "vkh_softmax_backward": (c_int, [P, P, P, P, U64, U64]),

@@ -50,6 +50,8 @@ struct GemmArgs {
   int aux_act;
   int vec_c;  // C, bias, aux, Cbf rows allow 16-byte (Cbf: 8-byte) accesses
   uint32_t tiles_m, tiles_n;
+  uint16_t* Cbft;  // C^T as BF16: Cbft[n * ldt + m] (optional)
+  uint64_t ldt;
 };
 
 __device__ __forceinline__ float act_apply(int act, float v) {
@@ -249,7 +251,11 @@ struct PackedStager {
 // Epilogue of 4 consecutive outputs C[m, n..n+3] from their accumulators, in
 // the order alpha*acc (+ beta*C) (+ bias[n]), activation, times the aux
 // activation's derivative; 16-byte accesses when the whole group is in range.
-__device__ __forceinline__ void epilogue4(const GemmArgs& e, uint64_t m, uint64_t n, float4 acc) {
+// Returns the final values.  The transposed BF16 copy is written here, one
+// element at a time, unless write_t is false (the staged epilogue writes it
+// in whole runs).
+__device__ __forceinline__ float4 epilogue4(const GemmArgs& e, uint64_t m, uint64_t n, float4 acc,
+                                            bool write_t = true) {
   const bool full = e.vec_c && n + 3 < e.N;
   const int cnt = full ? 4 : (int)min<uint64_t>(4, e.N - n);
   float* cp = e.C + m * e.ldc + n;
@@ -309,6 +315,12 @@ __device__ __forceinline__ void epilogue4(const GemmArgs& e, uint64_t m, uint64_
       }
     }
   }
+  if (e.Cbft && write_t) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (q < cnt) e.Cbft[(n + q) * e.ldt + m] = f2bf(v[q]);
+  }
+  return make_float4(v[0], v[1], v[2], v[3]);
 }
 
 // Raw accumulators C[m, n..n+3] of one k chunk into part [M][N] (16-byte
@@ -332,7 +344,11 @@ __device__ __forceinline__ void store_part4(float* part, uint64_t N, uint64_t m,
 // 32 x (WN+4) float staging slice of `smem`, then reads them back as 4-float
 // row segments (WN/4 lanes per row), so C is accessed in whole row segments.
 // With `part` the raw sums go to that k chunk's partial matrix instead.  The
-// caller's operand tiles in `smem` must be dead (a barrier before the call).
+// transposed BF16 copy (e.Cbft) takes a second pass over the slice: the row
+// pass puts the final values back, then each group of four lanes converts
+// one column of 32 rows and writes it as one 64-byte run of C^T (16 bytes a
+// lane, so a store instruction covers 16 whole runs).  The caller's operand tiles
+// in `smem` must be dead (a barrier before the call).
 template <int TM, int TN, int WM, int WN>
 __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM][TN], const GemmArgs& e,
                                                 float* part, uint64_t m0, uint64_t n0, int wm, int wn) {
@@ -342,6 +358,8 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
   const int l32 = lane & 31, lhi = lane >> 5;
   float* st = smem + wave * 32 * SROW;
   const uint64_t nb = n0 + wn * WN;
+  const bool tr = e.Cbft && !part;
+  const bool tr_vec = e.ldt % 8 == 0 && ((uintptr_t)e.Cbft & 15u) == 0;
 #pragma unroll
   for (int i = 0; i < TM; ++i) {
 #pragma unroll
@@ -356,11 +374,37 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
       const uint64_t m = mb + rr, n = nb + cc;
       if (m < e.M && n < e.N) {
         const float4 v = *reinterpret_cast<const float4*>(st + rr * SROW + cc);
-        if (part) store_part4(part, e.N, m, n, v);
-        else epilogue4(e, m, n, v);
+        if (part) {
+          store_part4(part, e.N, m, n, v);
+        } else {
+          const float4 f = epilogue4(e, m, n, v, false);
+          if (tr) *reinterpret_cast<float4*>(st + rr * SROW + cc) = f;
+        }
       }
     }
     __syncthreads();
+    if (tr) {
+      // four lanes per column, 8 rows (16 bytes) each: one store covers 16
+      // columns in whole 64-byte runs
+      const int q = lane & 3;
+      for (int c = lane >> 2; c < WN; c += 16) {
+        const uint64_t n = nb + c;
+        if (n >= e.N) continue;
+        uint16_t* dst = e.Cbft + n * e.ldt + mb + q * 8;
+        const float* col = st + q * 8 * SROW + c;
+        if (tr_vec && mb + 31 < e.M) {
+          uint32_t w[4];
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+            w[t] = f2bf(col[(2 * t) * SROW]) | ((uint32_t)f2bf(col[(2 * t + 1) * SROW]) << 16);
+          *reinterpret_cast<uint4*>(dst) = make_uint4(w[0], w[1], w[2], w[3]);
+        } else {
+          for (int r = 0; r < 8; ++r)
+            if (mb + q * 8 + r < e.M) dst[r] = f2bf(col[r * SROW]);
+        }
+      }
+      __syncthreads();
+    }
   }
 }
 
@@ -1045,11 +1089,12 @@ static void pack_operand(vkh_gpu* g, const float* src, uint64_t ld, bool trans, 
   check_launch("gemm pack");
 }
 
-// a_pre (BF16 mode, op(A) = A, K % PK_KALIGN == 0): op(A) already in the
-// packed layout -- BF16 rows of K elements, M rounded up to PK_MAXTILE rows --
-// so its packing pass is skipped (the bits are the ones it would write)
+// a_pre / b_pre (BF16 mode, K % PK_KALIGN == 0): op(A) / op(B)^T already in
+// the packed layout -- BF16 rows of K elements, M / N rounded up to
+// PK_MAXTILE rows -- so that operand's packing pass is skipped (the bits are
+// the ones it would write)
 static void gemm_packed(vkh_gpu* g, int mode, bool ta, bool tb, GemmArgs& a, int cfg,
-                        const uint16_t* a_pre = nullptr) {
+                        const uint16_t* a_pre = nullptr, const uint16_t* b_pre = nullptr) {
   const bool split = mode == VKH_GEMM_BF16X3;
   int ncfg = 0;
   const PackedCfg* cfgs = packed_cfgs(mode, ncfg);
@@ -1060,17 +1105,19 @@ static void gemm_packed(vkh_gpu* g, int mode, bool ta, bool tb, GemmArgs& a, int
   uint16_t* base = pack_area(g, packed_bytes(mode, a.M, a.N, a.K));
   const uint16_t* Ah = base;
   uint16_t* Al = split ? base + Mp * Kp : nullptr;
-  uint16_t* Bh = split ? Al + Mp * Kp : base + Mp * Kp;
-  uint16_t* Bl = split ? Bh + Np * Kp : nullptr;
+  uint16_t* Bp = split ? Al + Mp * Kp : base + Mp * Kp;
+  const uint16_t* Bh = Bp;
+  uint16_t* Bl = split ? Bp + Np * Kp : nullptr;
   // op(A)[m][k] and op(B)^T[n][k] = op(B)[k][n]
   if (split) {
     pack_operand<true>(g, a.A, a.lda, ta, a.M, a.K, base, Al, Mp, Kp);
-    pack_operand<true>(g, a.B, a.ldb, !tb, a.N, a.K, Bh, Bl, Np, Kp);
+    pack_operand<true>(g, a.B, a.ldb, !tb, a.N, a.K, Bp, Bl, Np, Kp);
   } else {
     if (!a_pre) pack_operand<false>(g, a.A, a.lda, ta, a.M, a.K, base, nullptr, Mp, Kp);
-    pack_operand<false>(g, a.B, a.ldb, !tb, a.N, a.K, Bh, nullptr, Np, Kp);
+    if (!b_pre) pack_operand<false>(g, a.B, a.ldb, !tb, a.N, a.K, Bp, nullptr, Np, Kp);
   }
   if (a_pre) Ah = a_pre;
+  if (b_pre) Bh = b_pre;
   a.tiles_m = (uint32_t)(Mp / s.bm);
   a.tiles_n = (uint32_t)(Np / s.bn);
   const uint64_t nb = (uint64_t)a.tiles_m * a.tiles_n;
@@ -1165,6 +1212,7 @@ __device__ __forceinline__ void skinny_store(const SkinnyArgs& a, uint64_t p, in
   if (e.aux) v = act_deriv_mul(e.aux_act, e.aux[m * e.ldaux + n], v);
   *cp = v;
   if (e.Cbf) e.Cbf[m * e.ldc + n] = f2bf(v);
+  if (e.Cbft) e.Cbft[n * e.ldt + m] = f2bf(v);
 }
 
 __device__ __forceinline__ float wave_sum64(float v) {
@@ -1412,7 +1460,8 @@ static void gemm_impl(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M
                       uint64_t K, float alpha, const float* A, uint64_t lda, const float* B,
                       uint64_t ldb, float beta, float* Cp, uint64_t ldc, const float* bias, int act,
                       uint16_t* c_bf16, const float* aux, uint64_t ldaux, int aux_act,
-                      const uint16_t* a_bf16 = nullptr) {
+                      const uint16_t* a_bf16 = nullptr, const uint16_t* b_bf16 = nullptr,
+                      uint16_t* c_bf16t = nullptr, uint64_t ldt = 0) {
   use(g);
   if (M == 0 || N == 0) return;
   VKH_REQUIRE(Cp, "gemm: null C");
@@ -1423,11 +1472,13 @@ static void gemm_impl(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M
   VKH_REQUIRE(act >= VKH_ACT_NONE && act <= VKH_ACT_SIGMOID, "gemm: unknown activation");
   VKH_REQUIRE(aux_act >= VKH_ACT_NONE && aux_act <= VKH_ACT_SIGMOID, "gemm: unknown aux activation");
   VKH_REQUIRE(!aux || ldaux >= N, "gemm: ldaux < N");
+  VKH_REQUIRE(!c_bf16t || ldt >= M, "gemm: ldt < M");
   GemmArgs a{};
   a.A = A; a.B = B; a.C = Cp; a.bias = bias; a.Cbf = c_bf16;
   a.M = M; a.N = N; a.K = K; a.lda = lda; a.ldb = ldb; a.ldc = ldc;
   a.alpha = alpha; a.beta = beta; a.act = act;
   a.aux = aux; a.ldaux = ldaux; a.aux_act = aux_act;
+  a.Cbft = c_bf16t; a.ldt = ldt;
   a.vec_c = ((uintptr_t)Cp & 15u) == 0 && ldc % 4 == 0 && (!bias || ((uintptr_t)bias & 15u) == 0) &&
             (!aux || (((uintptr_t)aux & 15u) == 0 && ldaux % 4 == 0)) &&
             (!c_bf16 || ((uintptr_t)c_bf16 & 7u) == 0);
@@ -1449,9 +1500,9 @@ static void gemm_impl(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M
     case VKH_GEMM_BF16:
     case VKH_GEMM_BF16X3: {
       auto it = g->gemm_cfg.find(cfg_key(mode, trans_a, trans_b, M, N, K));
-      const bool pre = a_bf16 && mode == VKH_GEMM_BF16 && !trans_a && K % PK_KALIGN == 0;
+      const bool pre = mode == VKH_GEMM_BF16 && K % PK_KALIGN == 0;
       gemm_packed(g, mode, trans_a, trans_b, a, it != g->gemm_cfg.end() ? it->second : default_cfg(mode, M, N, K),
-                  pre ? a_bf16 : nullptr);
+                  pre ? a_bf16 : nullptr, pre ? b_bf16 : nullptr);
       break;
     }
     default: fail(VKH_INVALID, "gemm: unknown mode");
@@ -1618,6 +1669,17 @@ int vkh_gemm_bf16a(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M, u
   return guarded([&] {
     gemm_impl(g, mode, trans_a, trans_b, M, N, K, alpha, A, lda, B, ldb, beta, Cp, ldc, bias, act,
               c_bf16, aux, ldaux, aux_act, a_bf16);
+  });
+}
+
+int vkh_gemm_bf16io(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M, uint64_t N, uint64_t K,
+                    float alpha, const float* A, uint64_t lda, const float* B, uint64_t ldb, float beta,
+                    float* Cp, uint64_t ldc, const float* bias, int act, uint16_t* c_bf16,
+                    const float* aux, uint64_t ldaux, int aux_act, const uint16_t* a_bf16,
+                    const uint16_t* b_bf16, uint16_t* c_bf16t, uint64_t ldt) {
+  return guarded([&] {
+    gemm_impl(g, mode, trans_a, trans_b, M, N, K, alpha, A, lda, B, ldb, beta, Cp, ldc, bias, act,
+              c_bf16, aux, ldaux, aux_act, a_bf16, b_bf16, c_bf16t, ldt);
   });
 }
 

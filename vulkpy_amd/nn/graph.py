@@ -68,11 +68,17 @@ def _units(layers) -> List[_Unit]:
     return units
 
 
+def _mfma(M: int, N: int, K: int) -> bool:
+    """Whether an M x N x K product runs on the matrix-core path (neither
+    the skinny nor the short-k vector path)."""
+    return min(M, N) > 16 and K > 16
+
+
 def _packs_a(M: int, N: int, K: int) -> bool:
-    """Whether an M x N x K product in BF16 mode packs its A operand in a
-    layout a BF16 copy of A can stand in for: the matrix-core path (neither
-    the skinny nor the short-k vector path) with K a multiple of 64."""
-    return min(M, N) > 16 and K > 16 and K % 64 == 0
+    """Whether an M x N x K product in BF16 mode packs its operands in a
+    layout a BF16 copy can stand in for: the matrix-core path with K a
+    multiple of 64."""
+    return _mfma(M, N, K) and K % 64 == 0
 
 
 class _Plan:
@@ -126,6 +132,14 @@ class _Plan:
         # with the same bits
         self.hs_bf: List[Optional[_DeviceBuffer]] = [None] * len(units)
         self.ds_bf: List[Optional[_DeviceBuffer]] = [None] * len(units)
+        # ... and their transposes hs[i]^T and ds[i]^T, written by the same
+        # products (c_bf16t, one 64-byte run of C^T per column of each 32-row
+        # tile) for the weight-gradient products dW = ds[i]^T x, which take
+        # them as packed op(A) and op(B)^T (vkh_gemm_bf16io) instead of
+        # packing with a transposing pass; only where the producer runs on
+        # the matrix cores, whose epilogue writes C^T coalesced
+        self.hs_bft: List[Optional[_DeviceBuffer]] = [None] * len(units)
+        self.ds_bft: List[Optional[_DeviceBuffer]] = [None] * len(units)
         if self.mode == GEMM_MODE["bf16"]:
             rows = -(-B // 256) * 256
             for i in range(len(units) - 1):   # consumer: forward product of unit i + 1
@@ -136,6 +150,17 @@ class _Plan:
                 k = units[i].dense.output_dim
                 if _packs_a(B, units[i].dense.input_dim, k):
                     self.ds_bf[i] = _DeviceBuffer(gpu, rows * k, np.uint16)
+            for i in range(len(units) - 1):
+                d, nxt = units[i].dense, units[i + 1].dense
+                trows = -(-d.output_dim // 256) * 256
+                # hs[i]^T: forward product of unit i -> op(B)^T of dW of unit i + 1
+                if (nxt.w.grad is not None and _mfma(B, d.output_dim, d.input_dim)
+                        and _packs_a(nxt.output_dim, d.output_dim, B)):
+                    self.hs_bft[i] = _DeviceBuffer(gpu, trows * B, np.uint16)
+                # ds[i]^T: dX product of unit i + 1 -> op(A) of dW of unit i
+                if (d.w.grad is not None and _mfma(B, d.output_dim, nxt.output_dim)
+                        and _packs_a(d.output_dim, d.input_dim, B)):
+                    self.ds_bft[i] = _DeviceBuffer(gpu, trows * B, np.uint16)
         # parameters, their optimizer states and device beta powers
         self.params: List[Parameter] = []
         for u in units:
@@ -165,10 +190,11 @@ class _Plan:
         src, src_bf = self.x, None
         for i, (u, hout) in enumerate(zip(units, self.hs)):
             d = u.dense
-            hb = self.hs_bf[i]
-            call("vkh_gemm_bf16a", h, mode, 0, 1, B, d.output_dim, d.input_dim, 1.0, src._ptr, d.input_dim,
-                 d.w.value._ptr, d.input_dim, 0.0, hout._ptr, d.output_dim, d.b.value._ptr,
-                 ACT[u.act], hb.ptr if hb else None, None, 0, 0, src_bf.ptr if src_bf else None)
+            hb, hbt = self.hs_bf[i], self.hs_bft[i]
+            call("vkh_gemm_bf16io", h, mode, 0, 1, B, d.output_dim, d.input_dim, 1.0, src._ptr,
+                 d.input_dim, d.w.value._ptr, d.input_dim, 0.0, hout._ptr, d.output_dim, d.b.value._ptr,
+                 ACT[u.act], hb.ptr if hb else None, None, 0, 0, src_bf.ptr if src_bf else None,
+                 None, hbt.ptr if hbt else None, B)
             src, src_bf = hout, hb
         out = self.hs[-1]
         O = units[-1].dense.output_dim
@@ -204,19 +230,21 @@ class _Plan:
                     self._enqueue_reg_grad(p)
                     beta = 1.0
                 if is_w:
-                    call("vkh_gemm", h, mode, 1, 0, d.output_dim, d.input_dim, B, 1.0, dy._ptr,
+                    dyt, xt = self.ds_bft[i], (self.hs_bft[i - 1] if i > 0 else None)
+                    call("vkh_gemm_bf16io", h, mode, 1, 0, d.output_dim, d.input_dim, B, 1.0, dy._ptr,
                          d.output_dim, xin._ptr, d.input_dim, beta, p.grad._ptr, d.input_dim,
-                         None, 0, None)
+                         None, 0, None, None, 0, 0, dyt.ptr if dyt else None, xt.ptr if xt else None,
+                         None, 0)
                 else:
                     call("vkh_bias_grad", h, dy._ptr, p.grad._ptr, B, d.output_dim, beta)
             if i > 0:
                 prev = units[i - 1]
-                out_bf, dy_bf = self.ds_bf[i - 1], self.ds_bf[i]
-                call("vkh_gemm_bf16a", h, mode, 0, 0, B, d.input_dim, d.output_dim, 1.0, dy._ptr,
+                out_bf, dy_bf, out_bft = self.ds_bf[i - 1], self.ds_bf[i], self.ds_bft[i - 1]
+                call("vkh_gemm_bf16io", h, mode, 0, 0, B, d.input_dim, d.output_dim, 1.0, dy._ptr,
                      d.output_dim, d.w.value._ptr, d.input_dim, 0.0, self.ds[i - 1]._ptr,
                      d.input_dim, None, 0, out_bf.ptr if out_bf else None,
                      self.hs[i - 1]._ptr if prev.act else None, d.input_dim, ACT[prev.act],
-                     dy_bf.ptr if dy_bf else None)
+                     dy_bf.ptr if dy_bf else None, None, out_bft.ptr if out_bft else None, B)
         # optimizer
         for p in self.params:
             st = p.opt_state
