@@ -164,6 +164,12 @@ def test_graph_uncaptured_matches_captured(gpu, precision):
         _, lu = unc.train(X, Y)
         gpu.wait()
         assert unc._plan.graph is None
+        if precision == "bf16":
+            # h0 and dY1 go to the next products as BF16 copies, and h0^T,
+            # dY0^T to the weight gradients as transposed BF16 copies
+            p = unc._plan
+            assert p.hs_bf[0] is not None and p.ds_bf[1] is not None
+            assert p.hs_bft[0] is not None and p.ds_bft[0] is not None
         assert np.asarray(lc).item() == np.asarray(lu).item(), step
         for a, b in zip(weights(cap_layers), weights(unc_layers)):
             np.testing.assert_array_equal(a, b)
