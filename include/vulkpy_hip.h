@@ -379,8 +379,10 @@ int vkh_gemm_bf16a(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
  * c_bf16t (optional) receives C^T rounded to BF16: c_bf16t[n * ldt + m],
  * ldt >= M -- with ldt = M = the next product's K it is that product's
  * packed op(A) or op(B)^T when C is its op(A)^T or op(B).  The FP32 result
- * is the one vkh_gemm_ex computes.  No counterpart in the reference: it
- * lets a captured training step hand activations and input gradients to
+ * is the one vkh_gemm_ex computes.  a_bf16 and b_bf16 must be 16-byte
+ * aligned, c_bf16t 2-byte aligned (VKH_INVALID otherwise; the same holds
+ * for vkh_gemm_bf16a's a_bf16).  No counterpart in the reference: it lets
+ * a captured training step hand activations and input gradients to
  * the weight-gradient products, which read them transposed, without a
  * transposing pack. */
 int vkh_gemm_bf16io(vkh_gpu* gpu, int mode, int trans_a, int trans_b,

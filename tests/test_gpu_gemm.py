@@ -358,3 +358,14 @@ def test_bf16io_transposed_output_and_packed_b(gpu, ta, tb):
         np.testing.assert_array_equal(run(a_pre=pre[0]), C_ref, err_msg=(M, N, K))
         np.testing.assert_array_equal(run(b_pre=pre[1]), C_ref, err_msg=(M, N, K))
         np.testing.assert_array_equal(run(a_pre=pre[0], b_pre=pre[1]), C_ref, err_msg=(M, N, K))
+    # misaligned hand-off buffers are refused on the host, before any launch
+    # (the offsets stay inside `big`, 256 KiB, whatever the product would touch)
+    big = vk.U32Array(gpu, shape=(256 * 256,))
+    for a_p, b_p, c_p in ((big._ptr + 2, None, None), (None, big._ptr + 8, None),
+                          (None, None, big._ptr + 1)):
+        C = vk.Array(gpu, shape=(M, N))
+        with pytest.raises(ValueError):
+            _lib.call("vkh_gemm_bf16io", gpu._handle, _lib.GEMM_MODE["bf16"], ta, tb, M, N, K, 1.0,
+                      gA._ptr, A.shape[1], gB._ptr, Bm.shape[1], 0.0, C._ptr, N, None, 0, None, None, 0,
+                      0, a_p, b_p, c_p, M)
+    gpu.wait()

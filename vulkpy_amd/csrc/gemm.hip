@@ -1473,6 +1473,9 @@ static void gemm_impl(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M
   VKH_REQUIRE(aux_act >= VKH_ACT_NONE && aux_act <= VKH_ACT_SIGMOID, "gemm: unknown aux activation");
   VKH_REQUIRE(!aux || ldaux >= N, "gemm: ldaux < N");
   VKH_REQUIRE(!c_bf16t || ldt >= M, "gemm: ldt < M");
+  VKH_REQUIRE(!c_bf16t || ((uintptr_t)c_bf16t & 1u) == 0, "gemm: c_bf16t not 2-byte aligned");
+  VKH_REQUIRE(!a_bf16 || ((uintptr_t)a_bf16 & 15u) == 0, "gemm: a_bf16 not 16-byte aligned");
+  VKH_REQUIRE(!b_bf16 || ((uintptr_t)b_bf16 & 15u) == 0, "gemm: b_bf16 not 16-byte aligned");
   GemmArgs a{};
   a.A = A; a.B = B; a.C = Cp; a.bias = bias; a.Cbf = c_bf16;
   a.M = M; a.N = N; a.K = K; a.lda = lda; a.ldb = ldb; a.ldc = ldc;
