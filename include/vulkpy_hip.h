@@ -118,11 +118,13 @@ int vkh_empty_cache(vkh_gpu* gpu);
 /* bytes held by the cache: live allocations and free cached blocks */
 int vkh_memory_stats(vkh_gpu* gpu, size_t* live_bytes, size_t* cached_bytes);
 /* host -> device copy enqueued on the stream; the host buffer must stay valid
- * and unchanged until an event recorded after this call completes.  Replaces
- * Buffer::set / the host-visible mapping (_vkarray.cc:85, 98-105). */
+ * and unchanged until an event recorded after this call completes.  VKH_INVALID
+ * while the context captures a graph.  Replaces Buffer::set / the host-visible
+ * mapping (_vkarray.cc:85, 98-105). */
 int vkh_upload(vkh_gpu* gpu, void* dst, const void* src, size_t nbytes);
 /* device -> host copy ordered after all work on the stream; returns once the
- * data is in host memory (replaces reading the mapped pointer, _vkarray.cc:107) */
+ * data is in host memory; VKH_INVALID while the context captures a graph
+ * (replaces reading the mapped pointer, _vkarray.cc:107) */
 int vkh_download(vkh_gpu* gpu, void* dst, const void* src, size_t nbytes);
 /* Page-locked host blocks for the host mirrors of arrays, so that vkh_upload /
  * vkh_download move them by DMA without the runtime's staging copies.  The

@@ -71,6 +71,31 @@ def test_graph_fp32_matches_eager(gpu, case):
         gpu.wait()
 
 
+def test_graph_returns_fresh_arrays(gpu):
+    """Each train call returns its own prediction and loss arrays (as the
+    reference's Sequence.train does), so results kept across steps are not
+    overwritten by later replays."""
+    dims, acts = [(16, 64), (64, 64), (64, 1)], ["relu", "relu", None]
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((64, 16)).astype(np.float32)
+    y = rng.standard_normal((64, 1)).astype(np.float32)
+    X, Y = vk.Array(gpu, data=x), vk.Array(gpu, data=y)
+    eager = nn.Sequence(make_net(gpu, dims, acts, 21, lambda: nn.Ridge(1e-4)), nn.HuberLoss(),
+                        regularize=True)
+    graph = nn.GraphSequence(make_net(gpu, dims, acts, 21, lambda: nn.Ridge(1e-4)), nn.HuberLoss(),
+                             precision="fp32")
+    kept, want = [], []
+    for _ in range(5):
+        pe, le = eager.train(X, Y)
+        want.append((np.asarray(pe).copy(), np.asarray(le).item()))
+        kept.append(graph.train(X, Y))
+    assert len({id(a) for pair in kept for a in pair}) == 10
+    for (pg, lg), (pe, le) in zip(kept, want):
+        np.testing.assert_array_equal(np.asarray(pg), pe)
+        assert np.asarray(lg).item() == le
+    assert want[0][1] != want[-1][1]
+
+
 def test_graph_bf16_tracks_fp32(gpu):
     dims, acts = [(32, 128), (128, 128), (128, 1)], ["relu", "relu", None]
     B = 512

@@ -81,3 +81,62 @@ def test_host_pool_errors():
         _lib.call("vkh_host_free", ctypes.c_void_p(0x1000))
     _lib.call("vkh_host_empty_cache")
     assert host_stats()[1] == 0
+
+
+def test_mirror_survives_inplace_then_delete(gpu):
+    """An upload from a pinned mirror still queued behind a long kernel, an
+    in-place op on the array, then the array deleted: its mirror goes back to
+    the pool only after the upload ran, so a new array of the same size class
+    (which takes that block and writes other bytes into it) cannot change
+    what the first array's consumer computes."""
+    n = 1 << 22                                            # 16 MiB: pinned, one chunk
+    rng = np.random.default_rng(42)
+    first = rng.standard_normal(n).astype(np.float32)
+    second = rng.standard_normal(n).astype(np.float32)
+    big = vk.Array(gpu, shape=(4096, 4096))
+    big[:] = 1.0
+    gpu.wait()
+    slow = big @ big                                       # long kernel ahead in the stream
+    x = vk.Array(gpu, data=first)                          # upload queued behind it
+    x *= 2.0                                               # in-place: new ticket on x
+    acc = x.sum()                                          # consumer of x's bytes
+    del x
+    gc.collect()
+    y = vk.Array(gpu, data=second)                         # same size class: may reuse the block
+    got = float(np.asarray(acc))
+    want = float((first.astype(np.float64) * 2).sum())
+    assert abs(got - want) <= 1e-6 * np.abs(first).sum() * 2, (got, want)
+    np.testing.assert_array_equal(np.asarray(y), second)
+    del slow, big
+
+
+def test_upload_download_refused_during_capture(gpu):
+    """Copies to or from host memory are not graph nodes: a captured upload
+    would read its host buffer again at every replay."""
+    h = gpu._handle
+    dev = vk.Array(gpu, shape=(16,))
+    host = np.zeros(16, dtype=np.float32)
+    gpu.wait()
+    _lib.call("vkh_graph_begin", h)
+    try:
+        assert _lib.lib.vkh_upload(h, ctypes.c_void_p(dev._ptr), host.ctypes.data_as(ctypes.c_void_p),
+                                   ctypes.c_size_t(64)) == _lib.VKH_INVALID
+        assert "graph capture" in _lib.last_error()
+        assert _lib.lib.vkh_download(h, host.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(dev._ptr),
+                                     ctypes.c_size_t(64)) == _lib.VKH_INVALID
+        _lib.call("vkh_fill_f32", h, dev._ptr, 1.0, 16)    # the capture itself goes on
+    finally:
+        ex = ctypes.c_void_p()
+        _lib.call("vkh_graph_end", h, ctypes.byref(ex))
+    dev[:] = 3.0
+    np.testing.assert_array_equal(np.asarray(dev), np.full(16, 3.0, np.float32))
+    _lib.call("vkh_graph_launch", h, ex.value)
+    np.testing.assert_array_equal(_download(gpu, dev),
+                                  np.ones(16, np.float32))
+    _lib.call("vkh_graph_destroy", ex.value)
+
+
+def _download(gpu, arr):
+    out = np.empty(arr.size, dtype=np.float32)
+    _lib.call("vkh_download", gpu._handle, out.ctypes.data, arr._ptr, out.nbytes)
+    return out

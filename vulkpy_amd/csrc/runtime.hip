@@ -310,6 +310,9 @@ int vkh_memory_stats(vkh_gpu* g, size_t* live, size_t* cached) {
 int vkh_upload(vkh_gpu* g, void* dst, const void* src, size_t nbytes) {
   return guarded([&] {
     use(g);
+    // A captured copy would read the host buffer again at every replay, long
+    // after the caller may have reused it.
+    VKH_REQUIRE(!g->capturing, "upload: not allowed during graph capture");
     if (nbytes == 0) return;
     VKH_REQUIRE(dst && src, "upload: null pointer");
     VKH_HIP(hipMemcpyAsync(dst, src, nbytes, hipMemcpyHostToDevice, g->stream));
@@ -319,6 +322,8 @@ int vkh_upload(vkh_gpu* g, void* dst, const void* src, size_t nbytes) {
 int vkh_download(vkh_gpu* g, void* dst, const void* src, size_t nbytes) {
   return guarded([&] {
     use(g);
+    // the copy is waited for on the host, which a capturing stream cannot do
+    VKH_REQUIRE(!g->capturing, "download: not allowed during graph capture");
     if (nbytes == 0) return;
     VKH_REQUIRE(dst && src, "download: null pointer");
     VKH_HIP(hipMemcpyAsync(dst, src, nbytes, hipMemcpyDeviceToHost, g->stream));

@@ -22,8 +22,9 @@ In ``precision="fp32"`` every operation is the one ``Sequence.train`` performs
 so both give identical weights.  ``"bf16"`` runs the GEMMs on BF16 matrix
 cores with FP32 accumulation and FP32 master weights.
 
-The arrays returned by ``train`` are the graph's own output buffers and are
-overwritten by the next call.
+``train`` returns fresh arrays holding this step's prediction and loss (two
+small device copies out of the graph's buffers, which the next replay
+overwrites).
 """
 from __future__ import annotations
 
@@ -355,9 +356,19 @@ class _Plan:
                 arr = getattr(st, name, None)
                 if isinstance(arr, Array):
                     arr._set_job()
-        for a in self.hs + [self.loss, self.L_rows]:
-            a._set_job()
-        return self.hs[-1], self.loss
+        # every array the step writes gets the new ticket (and a stale host
+        # mirror is dropped)
+        for a in self.hs + self.ds + [self.loss, self.L_rows, self.sm, self.g_out]:
+            if a is not None:
+                a._set_job()
+        # fresh arrays, like the reference's train (nn/models.py:55-78): the
+        # next replay overwrites the graph's own output buffers
+        pred, loss = Array(self.gpu, shape=self.hs[-1].shape), Array(self.gpu, shape=())
+        call("vkh_copy", h, pred._ptr, self.hs[-1]._ptr, pred.buffer.nbytes)
+        call("vkh_copy", h, loss._ptr, self.loss._ptr, loss.buffer.nbytes)
+        pred._set_job([self.hs[-1]])
+        loss._set_job([self.loss])
+        return pred, loss
 
     def __del__(self):
         try:
@@ -438,8 +449,7 @@ class GraphSequence:
 
     def train(self, x: Array, y: Array) -> Tuple[Array, Array]:
         """
-        One training step; returns (prediction, loss) held in the graph's
-        output buffers.
+        One training step; returns (prediction, loss) as fresh arrays.
         """
         key = (tuple(x.shape), tuple(y.shape))
         if self._plan is None or self._plan.key != key:

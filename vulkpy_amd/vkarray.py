@@ -408,10 +408,13 @@ class _GPUArray(Resource):
         self._upload_pending = False
 
     def _set_job(self, keep: Iterable[Resource] = ()):
+        # _upload_pending stays as it is: an upload from the host mirror may
+        # still be queued, and the new ticket (later on the same in-order
+        # stream) is what __del__ must wait for before the mirror's pinned
+        # block can go back to the pool.
         self.job = self._gpu._job()
         self._keep = list(keep)
         self._host_valid = False
-        self._upload_pending = False
 
     def flush(self):
         """
