@@ -200,6 +200,19 @@ int vkh_broadcast_to(vkh_gpu* gpu, const float* a, float* out, int ndim,
  * Uses workspace from the context's allocator. */
 int vkh_reduce_all(vkh_gpu* gpu, int op, const float* a, uint64_t n,
                    float* out, int mean);
+/* vkh_reduce_all without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * FP64 block partials go to a file-scope device array of 8192 doubles (64 KiB
+ * per device, held until the process ends) instead of a workspace, so the
+ * call allocates nothing and its results equal vkh_reduce_all's bit for bit
+ * (same kernels, same configuration, same partition).  Returns VKH_INVALID if
+ * the launch would need more than 8192 partials (device CUs x blocks per CU of
+ * the chosen configuration; at most 4096 on a 256-CU MI355X).  All calls on
+ * one GPU share that array: calls that overlap on one GPU (another stream or
+ * another context of the same device) race and give undefined results. */
+int vkh_reduce_all_noalloc(vkh_gpu* gpu, int op, const float* a, uint64_t n,
+                           float* out, int mean);
 /* F13/F14 axis reduction over a viewed as [prev, axis, post]:
  * out[p, j] = op_k a[p, k, j]  (rebroadcast == 0, out is [prev, post]) or
  * out[p, k, j] = op_k' a[p, k', j] (rebroadcast != 0, out is [prev, axis, post]).

@@ -89,6 +89,7 @@ _SIGNATURES = {
     "vkh_broadcast_to": (c_int, [P, P, P, c_int, _u64p, _u64p]),
     "vkh_onehot": (c_int, [P, P, U64, c_uint32, P]),
     "vkh_reduce_all": (c_int, [P, c_int, P, U64, P, c_int]),
+    "vkh_reduce_all_noalloc": (c_int, [P, c_int, P, U64, P, c_int]),
     "vkh_reduce_axis": (c_int, [P, c_int, P, P, U64, U64, U64, c_int, c_float, c_float]),
     "vkh_reduce_family": (c_int, [c_int, U64, U64, U64, POINTER(c_int)]),
     "vkh_reduce_configs": (c_int, [c_int, POINTER(c_int)]),

@@ -329,12 +329,15 @@ static RedCfg red_cfg(vkh_gpu* g, int fam, uint64_t prev, uint64_t K, uint64_t p
   return c[index];
 }
 
+// blocks of a full reduction of n elements: one FP64 partial each
+static unsigned all_blocks(vkh_gpu* g, uint64_t n, const RedCfg& c) {
+  return stream_grid(g, ceil_div(n, 4 * 4), ALL_BLOCK, (unsigned)c.par);
+}
+
+// partial pass into part[0, nb), then the final combine into out[0]
 template <int OP>
-static void reduce_all(vkh_gpu* g, const float* a, uint64_t n, float* out, int mean, int index = -1) {
-  const RedCfg c = red_cfg(g, VKH_RFAM_ALL, 1, n, 1, index);
-  unsigned nb = stream_grid(g, ceil_div(n, 4 * 4), ALL_BLOCK, (unsigned)c.par);
-  Workspace ws(g, sizeof(double) * nb);
-  double* part = (double*)ws.get();
+static void launch_all(vkh_gpu* g, const float* a, uint64_t n, float* out, int mean, const RedCfg& c,
+                       unsigned nb, double* part) {
   if (c.unroll == 8)
     hipLaunchKernelGGL((k_all_partial<OP, 8>), dim3(nb), dim3(ALL_BLOCK), 0, g->stream, a, n, part);
   else
@@ -342,6 +345,52 @@ static void reduce_all(vkh_gpu* g, const float* a, uint64_t n, float* out, int m
   check_launch("reduce_all partial");
   hipLaunchKernelGGL(k_all_final<OP>, dim3(1), dim3(ALL_BLOCK), 0, g->stream, part, (uint64_t)nb, out, n, mean);
   check_launch("reduce_all final");
+}
+
+template <int OP>
+static void reduce_all(vkh_gpu* g, const float* a, uint64_t n, float* out, int mean, int index = -1) {
+  const RedCfg c = red_cfg(g, VKH_RFAM_ALL, 1, n, 1, index);
+  const unsigned nb = all_blocks(g, n, c);
+  Workspace ws(g, sizeof(double) * nb);
+  launch_all<OP>(g, a, n, out, mean, c, nb, (double*)ws.get());
+}
+
+// ---- allocation-free full reduction (vkh_reduce_all_noalloc) ------------------
+// The FP64 block partials go to this file-scope device array instead of a
+// workspace.  Each device holds its own copy at its own address, looked up
+// once per device with hipGetSymbolAddress; host code passes that address,
+// never the array's name (on the host the name is a stand-in, not device
+// memory).  Every block writes its own partial before the final kernel reads
+// it, so no call depends on what an earlier call left there.
+constexpr unsigned NOALLOC_PARTIALS = 8192;  // 64 KiB: 256 CUs x 16 blocks per CU, twice over
+static __device__ __align__(256) double scratch[NOALLOC_PARTIALS];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[MAX_DEVICES] = {};
+
+// address of this device's scratch; the device must be current
+static double* scratch_address(vkh_gpu* g) {
+  VKH_REQUIRE(g->device >= 0 && g->device < MAX_DEVICES, "reduce_all_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  double*& p = g_scratch_ptr[g->device];
+  if (!p) {
+    void* addr = nullptr;
+    VKH_HIP(hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)));
+    VKH_REQUIRE(addr != nullptr, "reduce_all_noalloc: no device address for the scratch array");
+    p = (double*)addr;
+  }
+  return p;
+}
+
+template <int OP>
+static void reduce_all_noalloc(vkh_gpu* g, const float* a, uint64_t n, float* out, int mean) {
+  const RedCfg c = red_cfg(g, VKH_RFAM_ALL, 1, n, 1, -1);
+  const unsigned nb = all_blocks(g, n, c);
+  if (nb > NOALLOC_PARTIALS)
+    fail(VKH_INVALID, "reduce_all_noalloc: " + std::to_string(nb) + " block partials exceed the " +
+                          std::to_string(NOALLOC_PARTIALS) + " the scratch array holds");
+  launch_all<OP>(g, a, n, out, mean, c, nb, scratch_address(g));
 }
 
 template <int OP>
@@ -450,6 +499,25 @@ int vkh_reduce_all(vkh_gpu* g, int op, const float* a, uint64_t n, float* out, i
       case VKH_RPROD: reduce_all<VKH_RPROD>(g, a, n, out, mean); break;
       case VKH_RMAX: reduce_all<VKH_RMAX>(g, a, n, out, mean); break;
       case VKH_RMIN: reduce_all<VKH_RMIN>(g, a, n, out, mean); break;
+      default: fail(VKH_INVALID, "unknown reduction op");
+    }
+  });
+}
+
+int vkh_reduce_all_noalloc(vkh_gpu* g, int op, const float* a, uint64_t n, float* out, int mean) {
+  return guarded([&] {
+    use(g);
+    VKH_REQUIRE(out, "reduce_all_noalloc: null output");
+    VKH_REQUIRE(n == 0 || a, "reduce_all_noalloc: null input");
+    if (n == 0) {
+      VKH_REQUIRE(op == VKH_RSUM || op == VKH_RPROD,
+                  "zero-size array to reduction operation which has no identity");
+    }
+    switch (op) {
+      case VKH_RSUM: reduce_all_noalloc<VKH_RSUM>(g, a, n, out, mean); break;
+      case VKH_RPROD: reduce_all_noalloc<VKH_RPROD>(g, a, n, out, mean); break;
+      case VKH_RMAX: reduce_all_noalloc<VKH_RMAX>(g, a, n, out, mean); break;
+      case VKH_RMIN: reduce_all_noalloc<VKH_RMIN>(g, a, n, out, mean); break;
       default: fail(VKH_INVALID, "unknown reduction op");
     }
   });
