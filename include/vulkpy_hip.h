@@ -467,14 +467,29 @@ int vkh_adam(vkh_gpu* gpu, float* w, const float* g, float* m, float* v,
 int vkh_adam_pow_advance(vkh_gpu* gpu, double* bpow, double beta1,
                          double beta2);
 /* Adam update reading the beta powers from device memory bpow[0..1];
- * w_bf16 (may be NULL) also receives the updated weights as BF16 */
-int vkh_adam_device(vkh_gpu* gpu, float* w, const float* g, float* m, float* v,
+ * w_bf16 (may be NULL) also receives the updated weights as BF16.
+ * reg_kind != 0 first adds the parameter's regularizer gradient onto g and
+ * writes the sum back to g (Parameter.regular_grad, nn/parameters.py:108-116),
+ * then updates with it; reg_kind / reg_l1 / reg_s2 as in
+ * vkh_regularizer_grad (0 = none).  Used by captured training steps. */
+int vkh_adam_device(vkh_gpu* gpu, float* w, float* g, float* m, float* v,
                     uint64_t n, float lr, double beta1, double beta2, float eps,
-                    const double* bpow, uint16_t* w_bf16);
-/* SGD (nn/optimizers.py:53) and AdaGrad (nn/optimizers.py:134-140) */
-int vkh_sgd(vkh_gpu* gpu, float* w, const float* g, uint64_t n, float lr);
-int vkh_adagrad(vkh_gpu* gpu, float* w, const float* g, float* h, uint64_t n,
-                float lr, float eps);
+                    const double* bpow, uint16_t* w_bf16, int reg_kind,
+                    float reg_l1, float reg_s2);
+/* SGD (nn/optimizers.py:53) and AdaGrad (nn/optimizers.py:134-140), with the
+ * optional regularizer term of vkh_adam_device (reg_kind 0 = none) */
+int vkh_sgd(vkh_gpu* gpu, float* w, float* g, uint64_t n, float lr,
+            int reg_kind, float reg_l1, float reg_s2);
+int vkh_adagrad(vkh_gpu* gpu, float* w, float* g, float* h, uint64_t n,
+                float lr, float eps, int reg_kind, float reg_l1, float reg_s2);
+/* Regularizer gradient of the parameter w in one pass (nn/regularizers.py):
+ *   kind 1 Lasso.grad    r = l1 * sign(w)                        (56-69)
+ *   kind 2 Ridge.grad    r = s2 * w, s2 = 2 * coeff rounded to FP32 (113-127)
+ *   kind 3 Elastic.grad  r = fl(l1 * sign(w)) + fl(s2 * w)         (177-192)
+ * accumulate != 0: grad = grad + r (Parameter.regular_grad's add_grad),
+ * else grad = r.  Replaces the 1 / 2 / 4 array ops of the composition. */
+int vkh_regularizer_grad(vkh_gpu* gpu, int kind, const float* w, float* grad,
+                         uint64_t n, float l1, float s2, int accumulate);
 /* column sums of dy[rows, cols] accumulated into db (db += sum_r dy[r, :]):
  * Dense bias gradient (nn/layers.py:126-127) */
 int vkh_bias_grad(vkh_gpu* gpu, const float* dy, float* db, uint64_t rows,

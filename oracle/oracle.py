@@ -436,6 +436,20 @@ def adam(w, g, m, v, lr, beta1, beta2, eps, b1t, b2t):
     return (w + mhat).astype(np.float32), m, v
 
 
+def regularizer_grad(kind: str, w, l1: float = 0.0, l2: float = 0.0):
+    """Regularizer.grad of nn/regularizers.py in FP32 per op: "lasso"
+    coeff * sign(w) (56-69), "ridge" (2 coeff) * w with the scalar rounded
+    once (113-127), "elastic" L1.grad + L2.grad (177-192)"""
+    w = _f32(w)
+    t1 = (sign(w) * f32(l1)).astype(np.float32)
+    t2 = (w * f32(2 * l2)).astype(np.float32)
+    if kind == "lasso":
+        return t1
+    if kind == "ridge":
+        return t2
+    return (t1 + t2).astype(np.float32)
+
+
 def sgd(w, g, lr):
     """nn/optimizers.py:53"""
     return (_f32(w) + (_f32(g) * f32(-lr)).astype(np.float32)).astype(np.float32)

@@ -18,10 +18,13 @@ def make_net(gpu, dims, acts, seed, reg=None, opt="adam", precision=None):
     for k, (i, o) in enumerate(dims):
         if opt == "adam":
             wo, bo = nn.Adam(gpu, lr=1e-3), nn.Adam(gpu, lr=1e-3)
+        elif opt == "adagrad":
+            wo, bo = nn.AdaGrad(gpu, lr=1e-2, tau=0.1), nn.AdaGrad(gpu, lr=1e-2, tau=0.1)
         else:
             wo, bo = nn.SGD(0.05), nn.SGD(0.05)
         layers.append(nn.Dense(gpu, i, o, w_init=nn.HeNormal(gpu, i, seed=seed + k),
                                w_opt=wo, b_opt=bo, w_reg=reg() if reg else None,
+                               b_reg=reg() if reg and opt != "sgd" else None,
                                precision=precision))
         if acts[k] == "relu":
             layers.append(nn.ReLU())
@@ -39,6 +42,8 @@ CASES = [
     ([(20, 32), (32, 24), (24, 10)], ["relu", "sigmoid", None], "sce", None, "adam"),
     ([(16, 64), (64, 64), (64, 1)], ["relu", "relu", None], "huber", lambda: nn.Ridge(1e-4), "adam"),
     ([(8, 16), (16, 3)], ["sigmoid", None], "mse", lambda: nn.Lasso(1e-3), "sgd"),
+    ([(12, 40), (40, 4)], ["relu", None], "sce", lambda: nn.Elastic(1e-3, 2e-3), "adagrad"),
+    ([(12, 40), (40, 40), (40, 2)], ["relu", "relu", None], "huber", lambda: nn.Elastic(5e-4, 1e-3), "adam"),
 ]
 
 

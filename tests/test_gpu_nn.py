@@ -150,6 +150,31 @@ def test_regularizers(gpu):
         np.testing.assert_allclose(nn.Elastic(1.0, 1.0).grad(p.value), np.asarray((2 * v + np.sign(v),)))
 
 
+@pytest.mark.parametrize("kind", ["lasso", "ridge", "elastic"])
+def test_regularizer_grad_fused(gpu, kind):
+    """Regularizer.grad is one kernel, bit for bit the oracle's FP32 restatement
+    of the reference's composition; Parameter.regular_grad adds it onto the
+    gradient in place, equal to grad + R.grad(w)."""
+    rng = np.random.default_rng(12)
+    w = rng.standard_normal(100003).astype(f32)
+    w[:4] = [0.0, -0.0, 1e-41, np.inf]
+    g0 = rng.standard_normal(w.shape).astype(f32)
+    c1, c2 = 3e-3, 5e-4
+    R = {"lasso": nn.Lasso(c1), "ridge": nn.Ridge(c2), "elastic": nn.Elastic(c1, c2)}[kind]
+    want = O.regularizer_grad(kind, w, l1=c1, l2=c2)
+    bits_equal(R.grad(A(gpu, w)), want)
+    p = Parameter(gpu, w.shape, initializer=nn.Constant(0.0), regularizer=R)
+    p.value[:] = w
+    p.grad[:] = g0
+    p.regular_grad()
+    bits_equal(p.grad, (g0 + want).astype(f32))
+    # composition with array ops, as the reference writes it
+    W = A(gpu, w)
+    comp = {"lasso": lambda: c1 * W.sign(), "ridge": lambda: (2 * c2) * W,
+            "elastic": lambda: c1 * W.sign() + (2 * c2) * W}[kind]()
+    bits_equal(R.grad(W), comp)
+
+
 # ---- fused kernels against the oracle -------------------------------------------------------
 def test_activation_kernels_bit_exact(gpu):
     rng = np.random.default_rng(0)

@@ -339,6 +339,28 @@ def test_loss_kats():
     np.testing.assert_allclose(O.loss_grad("huber", _x, _y), [[-1.0, -0.8], [1.0, 0.2]], rtol=1e-6)
 
 
+def test_regularizer_grad_kats_and_closed_form():
+    # test/test_nn.py:487-549: W = 0, 3.5, -3.5 with coefficients 1
+    for v, ridge, lasso, elastic in ((0.0, 0.0, 0.0, 0.0), (3.5, 7.0, 1.0, 8.0), (-3.5, -7.0, -1.0, -8.0)):
+        w = np.float32([v])
+        assert O.regularizer_grad("ridge", w, l2=1.0)[0] == ridge
+        assert O.regularizer_grad("lasso", w, l1=1.0)[0] == lasso
+        assert O.regularizer_grad("elastic", w, l1=1.0, l2=1.0)[0] == elastic
+    # closed form in FP64 for other coefficients: each term is one rounding,
+    # so a dropped factor 2 or a swapped coefficient shows at once
+    rng = np.random.default_rng(5)
+    w = rng.standard_normal(10000).astype(np.float32)
+    w[:3] = [0.0, -0.0, 1e-42]
+    c1, c2 = 3e-4, 7e-3
+    r = O.regularizer_grad("ridge", w, l2=c2).astype(np.float64)
+    exact = 2 * c2 * w.astype(np.float64)
+    assert np.all(np.abs(r - exact) <= 2 * np.spacing(np.abs(exact).astype(np.float32)) + 1e-45)
+    assert np.array_equal(O.regularizer_grad("lasso", w, l1=c1), np.float32(c1) * np.sign(w))
+    e = O.regularizer_grad("elastic", w, l1=c1, l2=c2).astype(np.float64)
+    ex = c1 * np.sign(w) + 2 * c2 * w.astype(np.float64)
+    assert np.all(np.abs(e - ex) <= 4 * np.spacing(np.abs(ex).astype(np.float32)) + 1e-45)
+
+
 def test_optimizers_against_textbook_formulas():
     rng = np.random.default_rng(11)
     w = rng.standard_normal(100).astype(f32)

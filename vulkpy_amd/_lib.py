@@ -138,9 +138,11 @@ _SIGNATURES = {
     "vkh_loss_grad": (c_int, [P, c_int, P, P, P, U64, c_float]),
     "vkh_adam": (c_int, [P, P, P, P, P, U64, c_float, c_double, c_double, c_float, c_double, c_double]),
     "vkh_adam_pow_advance": (c_int, [P, P, c_double, c_double]),
-    "vkh_adam_device": (c_int, [P, P, P, P, P, U64, c_float, c_double, c_double, c_float, P, P]),
-    "vkh_sgd": (c_int, [P, P, P, U64, c_float]),
-    "vkh_adagrad": (c_int, [P, P, P, P, U64, c_float, c_float]),
+    "vkh_adam_device": (c_int, [P, P, P, P, P, U64, c_float, c_double, c_double, c_float, P, P,
+                                c_int, c_float, c_float]),
+    "vkh_sgd": (c_int, [P, P, P, U64, c_float, c_int, c_float, c_float]),
+    "vkh_regularizer_grad": (c_int, [P, c_int, P, P, U64, c_float, c_float, c_int]),
+    "vkh_adagrad": (c_int, [P, P, P, P, U64, c_float, c_float, c_int, c_float, c_float]),
     "vkh_bias_grad": (c_int, [P, P, P, U64, U64, c_float]),
     "vkh_to_bf16": (c_int, [P, P, P, U64]),
     "vkh_graph_begin": (c_int, [P]),

@@ -8,6 +8,8 @@
 // over short rows are sequential like vulkpy/shader/sum_axis.comp:28-30.
 #include "common.h"
 
+#include <initializer_list>
+
 namespace vkh {
 
 __device__ __forceinline__ float sign_np(float x) {
@@ -243,6 +245,54 @@ k_xent_bwd(const float* x, const float* y, float* dx, uint64_t n, float scale) {
   }
 }
 
+// ---- regularizer gradients (nn/regularizers.py:23-192) ------------------------------
+// kind 1 Lasso.grad  coeff * sign(w)
+// kind 2 Ridge.grad  (2 coeff) * w, the scalar 2 coeff formed in FP64 and rounded
+// kind 3 Elastic.grad  L1.grad + L2.grad, each rounded before the sum
+// Parameter.regular_grad then adds the term onto the gradient: g + r.
+struct RegTerms {
+  int kind;  // 0: no regularizer
+  float l1, s2;
+};
+
+__device__ __forceinline__ float reg_term(const RegTerms& r, float w) {
+  const float t1 = sign_np(w) * r.l1;
+  const float t2 = w * r.s2;
+  return r.kind == 1 ? t1 : (r.kind == 2 ? t2 : t1 + t2);
+}
+
+__device__ __forceinline__ float with_reg(const RegTerms& r, float g, float w) {
+  return r.kind ? g + reg_term(r, w) : g;
+}
+
+// Element loops of the parameter kernels: 16-byte accesses over the first
+// 4 * n4 elements (n4 = 0 when a pointer is not 16-byte aligned), then the
+// scalar tail.
+template <typename F4, typename F1>
+__device__ __forceinline__ void vec_loop(uint64_t n4, uint64_t n, F4&& body4, F1&& body1) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  for (uint64_t q = t; q < n4; q += stride) body4(q);
+  for (uint64_t i = 4 * n4 + t; i < n; i += stride) body1(i);
+}
+
+__global__ void __launch_bounds__(256)
+k_reg_grad(RegTerms r, const float* __restrict__ w, float* __restrict__ g, uint64_t n, uint64_t n4,
+           int accumulate) {
+  vec_loop(n4, n, [&](uint64_t q) {
+    const float4 wv = reinterpret_cast<const float4*>(w)[q];
+    float4 gv = accumulate ? reinterpret_cast<float4*>(g)[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+    gv.x = accumulate ? gv.x + reg_term(r, wv.x) : reg_term(r, wv.x);
+    gv.y = accumulate ? gv.y + reg_term(r, wv.y) : reg_term(r, wv.y);
+    gv.z = accumulate ? gv.z + reg_term(r, wv.z) : reg_term(r, wv.z);
+    gv.w = accumulate ? gv.w + reg_term(r, wv.w) : reg_term(r, wv.w);
+    reinterpret_cast<float4*>(g)[q] = gv;
+  }, [&](uint64_t i) {
+    const float t = reg_term(r, w[i]);
+    g[i] = accumulate ? g[i] + t : t;
+  });
+}
+
 // ---- optimizers ----------------------------------------------------------------------
 struct AdamScalars {
   float beta1, one_m_b1, beta2, one_m_b2, eps, neg_lr, c1, c2;
@@ -263,21 +313,51 @@ __device__ __forceinline__ void adam_elem(float& w, float g, float& m, float& v,
   w = w + mhat;
 }
 
-__global__ void __launch_bounds__(256)
-k_adam(float* w, const float* g, float* m, float* v, uint64_t n, AdamScalars s) {
-  const uint64_t stride = (uint64_t)gridDim.x * 256;
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
-    float wi = w[i], mi = m[i], vi = v[i];
-    adam_elem(wi, g[i], mi, vi, s);
-    w[i] = wi; m[i] = mi; v[i] = vi;
-  }
-}
-
 __device__ __forceinline__ uint16_t f2bf_nn(float x) {
   uint32_t u = __float_as_uint(x);
   if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
   u += 0x7fffu + ((u >> 16) & 1u);
   return (uint16_t)(u >> 16);
+}
+
+// One Adam step of parameter element(s): the regularizer term (if any) is
+// added to the gradient first and the sum written back, as
+// Parameter.regular_grad leaves it, then the update reads it.
+__device__ __forceinline__ void adam4(float4& w, float4& g, float4& m, float4& v, const AdamScalars& s,
+                                      const RegTerms& r) {
+  g.x = with_reg(r, g.x, w.x); g.y = with_reg(r, g.y, w.y);
+  g.z = with_reg(r, g.z, w.z); g.w = with_reg(r, g.w, w.w);
+  adam_elem(w.x, g.x, m.x, v.x, s); adam_elem(w.y, g.y, m.y, v.y, s);
+  adam_elem(w.z, g.z, m.z, v.z, s); adam_elem(w.w, g.w, m.w, v.w, s);
+}
+
+__global__ void __launch_bounds__(256)
+k_adam(float* __restrict__ w, float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+       uint64_t n, uint64_t n4, AdamScalars s, const double* bpow, RegTerms r, uint16_t* wbf) {
+  if (bpow) {  // beta powers kept on the device (captured steps)
+    s.c1 = (float)(1.0 - bpow[0]);
+    s.c2 = (float)(1.0 - bpow[1]);
+  }
+  vec_loop(n4, n, [&](uint64_t q) {
+    float4 wv = reinterpret_cast<float4*>(w)[q], gv = reinterpret_cast<const float4*>(g)[q];
+    float4 mv = reinterpret_cast<float4*>(m)[q], vv = reinterpret_cast<float4*>(v)[q];
+    adam4(wv, gv, mv, vv, s, r);
+    reinterpret_cast<float4*>(w)[q] = wv;
+    reinterpret_cast<float4*>(m)[q] = mv;
+    reinterpret_cast<float4*>(v)[q] = vv;
+    if (r.kind) reinterpret_cast<float4*>(g)[q] = gv;
+    if (wbf) {
+      ushort4 b;
+      b.x = f2bf_nn(wv.x); b.y = f2bf_nn(wv.y); b.z = f2bf_nn(wv.z); b.w = f2bf_nn(wv.w);
+      reinterpret_cast<ushort4*>(wbf)[q] = b;
+    }
+  }, [&](uint64_t i) {
+    float wi = w[i], gi = with_reg(r, g[i], w[i]), mi = m[i], vi = v[i];
+    adam_elem(wi, gi, mi, vi, s);
+    w[i] = wi; m[i] = mi; v[i] = vi;
+    if (r.kind) g[i] = gi;
+    if (wbf) wbf[i] = f2bf_nn(wi);
+  });
 }
 
 // Beta powers live in device memory: bpow[0] = beta1^t, bpow[1] = beta2^t
@@ -289,40 +369,53 @@ __global__ void k_adam_pow(double* bpow, double beta1, double beta2) {
   }
 }
 
+// nn/optimizers.py:53: w += (-lr) * g (g with the regularizer term added first)
 __global__ void __launch_bounds__(256)
-k_adam_dev(float* w, const float* g, float* m, float* v, uint64_t n, AdamScalars s,
-           const double* bpow, uint16_t* wbf) {
-  s.c1 = (float)(1.0 - bpow[0]);
-  s.c2 = (float)(1.0 - bpow[1]);
-  const uint64_t stride = (uint64_t)gridDim.x * 256;
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
-    float wi = w[i], mi = m[i], vi = v[i];
-    adam_elem(wi, g[i], mi, vi, s);
-    w[i] = wi; m[i] = mi; v[i] = vi;
-    if (wbf) wbf[i] = f2bf_nn(wi);
-  }
-}
-
-// nn/optimizers.py:53: w += (-lr) * g
-__global__ void __launch_bounds__(256) k_sgd(float* w, const float* g, uint64_t n, float neg_lr) {
-  const uint64_t stride = (uint64_t)gridDim.x * 256;
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) w[i] = w[i] + g[i] * neg_lr;
+k_sgd(float* __restrict__ w, float* __restrict__ g, uint64_t n, uint64_t n4, float neg_lr, RegTerms r) {
+  vec_loop(n4, n, [&](uint64_t q) {
+    float4 wv = reinterpret_cast<float4*>(w)[q], gv = reinterpret_cast<const float4*>(g)[q];
+    gv.x = with_reg(r, gv.x, wv.x); gv.y = with_reg(r, gv.y, wv.y);
+    gv.z = with_reg(r, gv.z, wv.z); gv.w = with_reg(r, gv.w, wv.w);
+    wv.x = wv.x + gv.x * neg_lr; wv.y = wv.y + gv.y * neg_lr;
+    wv.z = wv.z + gv.z * neg_lr; wv.w = wv.w + gv.w * neg_lr;
+    reinterpret_cast<float4*>(w)[q] = wv;
+    if (r.kind) reinterpret_cast<float4*>(g)[q] = gv;
+  }, [&](uint64_t i) {
+    const float gi = with_reg(r, g[i], w[i]);
+    w[i] = w[i] + gi * neg_lr;
+    if (r.kind) g[i] = gi;
+  });
 }
 
 // nn/optimizers.py:134-140
+__device__ __forceinline__ void adagrad_elem(float& w, float g, float& h, float neg_lr, float eps) {
+  h = h + g * g;
+  float s = sqrtf(h);
+  s = s + eps;
+  float q = g / s;
+  q = q * neg_lr;
+  w = w + q;
+}
+
 __global__ void __launch_bounds__(256)
-k_adagrad(float* w, const float* g, float* h, uint64_t n, float neg_lr, float eps) {
-  const uint64_t stride = (uint64_t)gridDim.x * 256;
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
-    float gi = g[i];
-    float hi = h[i] + gi * gi;
-    h[i] = hi;
-    float s = sqrtf(hi);
-    s = s + eps;
-    float r = gi / s;
-    r = r * neg_lr;
-    w[i] = w[i] + r;
-  }
+k_adagrad(float* __restrict__ w, float* __restrict__ g, float* __restrict__ h, uint64_t n, uint64_t n4,
+          float neg_lr, float eps, RegTerms r) {
+  vec_loop(n4, n, [&](uint64_t q) {
+    float4 wv = reinterpret_cast<float4*>(w)[q], gv = reinterpret_cast<const float4*>(g)[q];
+    float4 hv = reinterpret_cast<float4*>(h)[q];
+    gv.x = with_reg(r, gv.x, wv.x); gv.y = with_reg(r, gv.y, wv.y);
+    gv.z = with_reg(r, gv.z, wv.z); gv.w = with_reg(r, gv.w, wv.w);
+    adagrad_elem(wv.x, gv.x, hv.x, neg_lr, eps); adagrad_elem(wv.y, gv.y, hv.y, neg_lr, eps);
+    adagrad_elem(wv.z, gv.z, hv.z, neg_lr, eps); adagrad_elem(wv.w, gv.w, hv.w, neg_lr, eps);
+    reinterpret_cast<float4*>(w)[q] = wv;
+    reinterpret_cast<float4*>(h)[q] = hv;
+    if (r.kind) reinterpret_cast<float4*>(g)[q] = gv;
+  }, [&](uint64_t i) {
+    float wi = w[i], gi = with_reg(r, g[i], w[i]), hi = h[i];
+    adagrad_elem(wi, gi, hi, neg_lr, eps);
+    w[i] = wi; h[i] = hi;
+    if (r.kind) g[i] = gi;
+  });
 }
 
 __global__ void __launch_bounds__(256)
@@ -466,6 +559,32 @@ int vkh_loss_grad(vkh_gpu* g, int kind, const float* x, const float* y, float* d
   });
 }
 
+// 16-byte element loops when every pointer is 16-byte aligned
+static uint64_t vec4_count(uint64_t n, std::initializer_list<const void*> ptrs) {
+  for (const void* p : ptrs)
+    if (p && ((uintptr_t)p & 15u)) return 0;
+  return n / 4;
+}
+
+static RegTerms reg_terms(int kind, float l1, float s2) {
+  VKH_REQUIRE(kind >= 0 && kind <= 3, "unknown regularizer kind");
+  return RegTerms{kind, l1, s2};
+}
+
+int vkh_regularizer_grad(vkh_gpu* g, int kind, const float* w, float* grad, uint64_t n, float l1, float s2,
+                         int accumulate) {
+  return guarded([&] {
+    use(g);
+    VKH_REQUIRE(kind >= 1 && kind <= 3, "regularizer_grad: unknown regularizer kind");
+    if (n == 0) return;
+    VKH_REQUIRE(w && grad, "regularizer_grad: null pointer");
+    const uint64_t n4 = vec4_count(n, {w, grad});
+    hipLaunchKernelGGL(k_reg_grad, dim3(grid_for(g, n4 ? n4 : n)), dim3(256), 0, g->stream,
+                       RegTerms{kind, l1, s2}, w, grad, n, n4, accumulate);
+    check_launch("regularizer_grad");
+  });
+}
+
 int vkh_adam(vkh_gpu* g, float* w, const float* gr, float* m, float* v, uint64_t n, float lr,
              double beta1, double beta2, float eps, double b1t, double b2t) {
   return guarded([&] {
@@ -476,7 +595,10 @@ int vkh_adam(vkh_gpu* g, float* w, const float* gr, float* m, float* v, uint64_t
     // reference passes to the FP32 scalar kernels
     AdamScalars s{(float)beta1, (float)(1.0 - beta1), (float)beta2, (float)(1.0 - beta2),
                   eps, -lr, (float)(1.0 - b1t), (float)(1.0 - b2t)};
-    hipLaunchKernelGGL(k_adam, dim3(grid_for(g, n)), dim3(256), 0, g->stream, w, gr, m, v, n, s);
+    const uint64_t n4 = vec4_count(n, {w, gr, m, v});
+    hipLaunchKernelGGL(k_adam, dim3(grid_for(g, n4 ? n4 : n)), dim3(256), 0, g->stream, w,
+                       const_cast<float*>(gr), m, v, n, n4, s, (const double*)nullptr, RegTerms{0, 0.f, 0.f},
+                       (uint16_t*)nullptr);
     check_launch("adam");
   });
 }
@@ -490,36 +612,46 @@ int vkh_adam_pow_advance(vkh_gpu* g, double* bpow, double beta1, double beta2) {
   });
 }
 
-int vkh_adam_device(vkh_gpu* g, float* w, const float* gr, float* m, float* v, uint64_t n, float lr,
-                    double beta1, double beta2, float eps, const double* bpow, uint16_t* w_bf16) {
+int vkh_adam_device(vkh_gpu* g, float* w, float* gr, float* m, float* v, uint64_t n, float lr,
+                    double beta1, double beta2, float eps, const double* bpow, uint16_t* w_bf16,
+                    int reg_kind, float reg_l1, float reg_s2) {
   return guarded([&] {
     use(g);
     VKH_REQUIRE(w && gr && m && v && bpow, "adam_device: null pointer");
+    const RegTerms r = reg_terms(reg_kind, reg_l1, reg_s2);
     if (n == 0) return;
     AdamScalars s{(float)beta1, (float)(1.0 - beta1), (float)beta2, (float)(1.0 - beta2),
                   eps, -lr, 1.0f, 1.0f};
-    hipLaunchKernelGGL(k_adam_dev, dim3(grid_for(g, n)), dim3(256), 0, g->stream, w, gr, m, v, n, s,
-                       bpow, w_bf16);
+    const uint64_t n4 = (w_bf16 && ((uintptr_t)w_bf16 & 7u)) ? 0 : vec4_count(n, {w, gr, m, v});
+    hipLaunchKernelGGL(k_adam, dim3(grid_for(g, n4 ? n4 : n)), dim3(256), 0, g->stream, w, gr, m, v, n, n4,
+                       s, bpow, r, w_bf16);
     check_launch("adam_device");
   });
 }
 
-int vkh_sgd(vkh_gpu* g, float* w, const float* gr, uint64_t n, float lr) {
+int vkh_sgd(vkh_gpu* g, float* w, float* gr, uint64_t n, float lr, int reg_kind, float reg_l1,
+            float reg_s2) {
   return guarded([&] {
     use(g);
+    const RegTerms r = reg_terms(reg_kind, reg_l1, reg_s2);
     if (n == 0) return;
     VKH_REQUIRE(w && gr, "sgd: null pointer");
-    hipLaunchKernelGGL(k_sgd, dim3(grid_for(g, n)), dim3(256), 0, g->stream, w, gr, n, -lr);
+    const uint64_t n4 = vec4_count(n, {w, gr});
+    hipLaunchKernelGGL(k_sgd, dim3(grid_for(g, n4 ? n4 : n)), dim3(256), 0, g->stream, w, gr, n, n4, -lr, r);
     check_launch("sgd");
   });
 }
 
-int vkh_adagrad(vkh_gpu* g, float* w, const float* gr, float* h, uint64_t n, float lr, float eps) {
+int vkh_adagrad(vkh_gpu* g, float* w, float* gr, float* h, uint64_t n, float lr, float eps, int reg_kind,
+                float reg_l1, float reg_s2) {
   return guarded([&] {
     use(g);
+    const RegTerms r = reg_terms(reg_kind, reg_l1, reg_s2);
     if (n == 0) return;
     VKH_REQUIRE(w && gr && h, "adagrad: null pointer");
-    hipLaunchKernelGGL(k_adagrad, dim3(grid_for(g, n)), dim3(256), 0, g->stream, w, gr, h, n, -lr, eps);
+    const uint64_t n4 = vec4_count(n, {w, gr, h});
+    hipLaunchKernelGGL(k_adagrad, dim3(grid_for(g, n4 ? n4 : n)), dim3(256), 0, g->stream, w, gr, h, n, n4,
+                       -lr, eps, r);
     check_launch("adagrad");
   });
 }

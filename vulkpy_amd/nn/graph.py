@@ -10,12 +10,15 @@ over preallocated buffers:
   its epilogue;
 * loss: one fused row kernel plus the batch reduction; the loss gradient is
   one kernel;
-* backward: per Dense, ``dW = dY^T X`` accumulated (GEMM beta = 1) onto the
-  regularizer gradient, ``db`` by a column sum, and ``dX = dY W`` with the
-  previous activation's derivative in the GEMM epilogue;
-* update: one kernel per parameter; Adam's beta powers live in device memory
-  and are advanced inside the graph (nn/optimizers.py:241-242), so no host
-  scalar changes between replays.
+* backward: per Dense, ``dW = dY^T X``, ``db`` by a column sum, and
+  ``dX = dY W`` with the previous activation's derivative in the GEMM
+  epilogue;
+* update: one kernel per parameter, which first adds the parameter's
+  regularizer gradient onto its gradient (writing the sum back, as
+  ``Parameter.regular_grad`` leaves it) and then applies the optimizer;
+  Adam's beta powers live in device memory and are advanced inside the graph
+  (nn/optimizers.py:241-242), once per group of parameters whose powers are
+  equal, so no host scalar changes between replays.
 
 In ``precision="fp32"`` every operation is the one ``Sequence.train`` performs
 (vulkpy/nn/models.py:55-78 with ``regularize=True``), in the same FP32 order,
@@ -33,14 +36,14 @@ from typing import Iterable, List, Optional, Tuple
 
 import numpy as np
 
-from vulkpy_amd._lib import ACT, BINOP, GEMM_MODE, UNOP, call
+from vulkpy_amd._lib import ACT, GEMM_MODE, call
 from vulkpy_amd.vkarray import Array, _DeviceBuffer
 from .core import Loss, Module
 from .layers import Dense, ReLU, Sigmoid
 from .losses import CrossEntropyLoss, HuberLoss, MSELoss, SoftmaxCrossEntropyLoss
 from .optimizers import AdaGradState, AdamState, SGDState
 from .parameters import Parameter, _has_fused_update
-from .regularizers import Elastic, Lasso, Ridge
+from .regularizers import _fused_penalty
 
 __all__ = ["GraphSequence"]
 
@@ -162,8 +165,10 @@ class _Plan:
                 if (d.w.grad is not None and _mfma(B, d.output_dim, nxt.output_dim)
                         and _packs_a(d.output_dim, d.input_dim, B)):
                     self.ds_bft[i] = _DeviceBuffer(gpu, trows * B, np.uint16)
-        # parameters, their optimizer states and device beta powers
+        # parameters, their optimizer states, regularizer terms and device
+        # beta powers
         self.params: List[Parameter] = []
+        self.reg = {}
         for u in units:
             for p in (u.dense.w, u.dense.b):
                 if p.grad is None:
@@ -172,13 +177,26 @@ class _Plan:
                 if not _has_fused_update(st) or not isinstance(st, (AdamState, SGDState, AdaGradState)):
                     raise ValueError(f"GraphSequence does not support optimizer state {type(st).__name__}")
                 self.params.append(p)
+                terms = (0, 0.0, 0.0)
+                if seq.regularize and p.R is not None:
+                    if not _fused_penalty(p.R):
+                        raise ValueError(f"GraphSequence does not support regularizer {type(p.R).__name__}")
+                    terms = p.R.terms()
+                self.reg[id(p)] = terms
+        # Adam states whose beta powers evolve together (same betas, same
+        # powers so far) share one FP64 device pair, advanced once per step
         self.bpow = {}
+        self.bpow_groups: List[Tuple[_DeviceDoubles, float, float]] = []
+        groups = {}
         for p in self.params:
             st = p.opt_state
             if isinstance(st, AdamState):
-                bp = np.array([st.beta1t, st.beta2t], dtype=np.float64)
-                buf = _DeviceDoubles(gpu, bp)
-                self.bpow[id(st)] = buf
+                key = (float(st.opt.beta1), float(st.opt.beta2), float(st.beta1t), float(st.beta2t))
+                if key not in groups:
+                    buf = _DeviceDoubles(gpu, np.array([st.beta1t, st.beta2t], dtype=np.float64))
+                    groups[key] = buf
+                    self.bpow_groups.append((buf, key[0], key[1]))
+                self.bpow[id(st)] = groups[key]
         gpu.wait()
         self.graph = None
         self._capture()
@@ -226,10 +244,8 @@ class _Plan:
             for p, is_w in ((d.w, True), (d.b, False)):
                 if p.grad is None:
                     continue
+                # the regularizer term is added by the optimizer kernel
                 beta = 0.0
-                if self.seq.regularize and p.R is not None:
-                    self._enqueue_reg_grad(p)
-                    beta = 1.0
                 if is_w:
                     dyt, xt = self.ds_bft[i], (self.hs_bft[i - 1] if i > 0 else None)
                     call("vkh_gemm_bf16io", h, mode, 1, 0, d.output_dim, d.input_dim, B, 1.0, dy._ptr,
@@ -246,21 +262,24 @@ class _Plan:
                      d.input_dim, None, 0, out_bf.ptr if out_bf else None,
                      self.hs[i - 1]._ptr if prev.act else None, d.input_dim, ACT[prev.act],
                      dy_bf.ptr if dy_bf else None, None, out_bft.ptr if out_bft else None, B)
-        # optimizer
+        # optimizer: beta powers once per group, then one kernel per
+        # parameter (regularizer term, gradient write-back and update)
+        for buf, beta1, beta2 in self.bpow_groups:
+            call("vkh_adam_pow_advance", h, buf.ptr, beta1, beta2)
         for p in self.params:
             st = p.opt_state
             n = p.value.size
+            kind, l1, s2 = self.reg[id(p)]
             if isinstance(st, AdamState):
-                bp = self.bpow[id(st)]
                 o = st.opt
-                call("vkh_adam_pow_advance", h, bp.ptr, float(o.beta1), float(o.beta2))
                 call("vkh_adam_device", h, p.value._ptr, p.grad._ptr, st.m._ptr, st.v._ptr, n,
-                     float(o.lr), float(o.beta1), float(o.beta2), float(o.eps), bp.ptr, None)
+                     float(o.lr), float(o.beta1), float(o.beta2), float(o.eps), self.bpow[id(st)].ptr,
+                     None, kind, l1, s2)
             elif isinstance(st, SGDState):
-                call("vkh_sgd", h, p.value._ptr, p.grad._ptr, n, float(st.opt.lr))
+                call("vkh_sgd", h, p.value._ptr, p.grad._ptr, n, float(st.opt.lr), kind, l1, s2)
             else:
                 call("vkh_adagrad", h, p.value._ptr, p.grad._ptr, st.h._ptr, n,
-                     float(st.opt.lr), float(st.opt.eps))
+                     float(st.opt.lr), float(st.opt.eps), kind, l1, s2)
 
     def _gemm_shapes(self):
         """(trans_a, trans_b, M, N, K) of every product _enqueue issues"""
@@ -279,36 +298,7 @@ class _Plan:
         for d in sorted({u.dense.output_dim for u in self.units if u.dense.b.grad is not None}):
             yield 1, B, d                                 # bias gradient (vkh_bias_grad)
 
-    def _enqueue_reg_grad(self, p: Parameter):
-        """grad = R.grad(value) (regularizers.py:23-192), before dW / db are added"""
-        h, n, R = self.h, p.value.size, p.R
-        if isinstance(R, Ridge):
-            call("vkh_binary_scalar", h, BINOP["mul"], p.value._ptr, float(2 * R.coeff), p.grad._ptr, n)
-        elif isinstance(R, Lasso):
-            call("vkh_unary", h, UNOP["sign"], p.value._ptr, p.grad._ptr, n)
-            call("vkh_binary_scalar", h, BINOP["mul"], p.grad._ptr, float(R.coeff), p.grad._ptr, n)
-        elif isinstance(R, Elastic):
-            tmp = self._scratch(p)
-            call("vkh_unary", h, UNOP["sign"], p.value._ptr, p.grad._ptr, n)
-            call("vkh_binary_scalar", h, BINOP["mul"], p.grad._ptr, float(R.L1.coeff), p.grad._ptr, n)
-            call("vkh_binary_scalar", h, BINOP["mul"], p.value._ptr, float(2 * R.L2.coeff), tmp._ptr, n)
-            call("vkh_binary", h, BINOP["add"], p.grad._ptr, tmp._ptr, p.grad._ptr, n)
-        else:
-            raise ValueError(f"GraphSequence does not support regularizer {type(R).__name__}")
-
-    def _scratch(self, p: Parameter) -> Array:
-        if not hasattr(self, "_scr"):
-            self._scr = {}
-        if id(p) not in self._scr:
-            self._scr[id(p)] = Array(self.gpu, shape=p.value.shape)
-        return self._scr[id(p)]
-
     def _capture(self):
-        # buffers the step may allocate are created before the capture begins
-        for u in self.units:
-            for p in (u.dense.w, u.dense.b):
-                if p.grad is not None and self.seq.regularize and isinstance(p.R, Elastic):
-                    self._scratch(p)
         # kernel configuration (when tuning) and the graph's GEMM packing
         # region for every product of the step
         for ta, tb, M, N, K in self._gemm_shapes():

@@ -34,38 +34,58 @@ __all__ = [
 _SCE, _MSE, _HUBER = 0, 1, 2
 
 
-F = Callable[[Array], Array]
+def _inverse_batch(dx: Array) -> float:
+    """gradient scale of a batch mean: 1 / B (vulkpy/nn/losses.py:43)"""
+    return 1 / dx.shape[0]
+
+
 class ReduceLoss(Loss):
+    """
+    A loss evaluated per sample by ``forward`` and reduced over the batch
+    (axis 0) by its mean or its sum (vulkpy/nn/losses.py:40-86).
+
+    Parameters
+    ----------
+    reduce : {"mean", "sum"}, optional
+        Batch reduction (default ``"mean"``).  The gradient of the mean is
+        the per-sample gradient scaled by 1 / B.
+    """
     def __init__(self, reduce: Literal["mean", "sum"] = "mean"):
-        tmp: Tuple[F, Optional[F]] = {
-            "mean": (lambda _L: _L.mean(axis=0), lambda _dx: 1/_dx.shape[0]),
-            "sum": (lambda _L: _L.sum(axis=0), None),
-        }[reduce]
-        self.reduce, self.scale_backward = tmp
+        if reduce not in ("mean", "sum"):
+            raise KeyError(reduce)
+        self._batch_reduction = reduce
+
+    def reduce(self, L: Array) -> Array:
+        """batch reduction of the per-sample losses ``L``"""
+        if self._batch_reduction == "mean":
+            return L.mean(axis=0)
+        return L.sum(axis=0)
+
+    @property
+    def scale_backward(self) -> Optional[Callable[[Array], float]]:
+        """scale of the gradient as a function of the gradient array
+        (1 / B for the mean), or ``None`` (the sum)"""
+        return _inverse_batch if self._batch_reduction == "mean" else None
 
     def __call__(self, x: Array, y: Array) -> Array:
         """
-        Compute Loss (per-sample ``forward`` reduced over the batch)
+        Loss of the batch ``(x, y)``; ``grad()`` then differentiates it
         """
         self._x = x
         self._y = y
-        L = self.forward(x, y)
-        return self.reduce(L)
+        return self.reduce(self.forward(x, y))
 
     def grad(self) -> Array:
         """
-        Compute Gradients of the last ``__call__(x, y)`` (scaled by 1/B for
-        ``reduce="mean"``, vulkpy/nn/losses.py:70-86)
+        Gradient of the last ``__call__(x, y)`` with respect to ``x``
         """
-        fused = getattr(type(self), "_backward_scaled", None)
-        if fused is not None and _backward_not_overridden(self):
-            scale = 1.0
-            if self.scale_backward is not None:
-                scale = self.scale_backward(self._x)
-            return self._backward_scaled(float(scale))
+        scale = self.scale_backward
+        if getattr(type(self), "_backward_scaled", None) is not None and _backward_not_overridden(self):
+            # one kernel: the per-sample gradient with the batch scale applied
+            return self._backward_scaled(1.0 if scale is None else float(scale(self._x)))
         dx = self.backward()
-        if self.scale_backward is not None:
-            dx *= self.scale_backward(dx)
+        if scale is not None:
+            dx *= scale(dx)
         return dx
 
     def forward(self, x: Array, y: Array) -> Array:
@@ -243,35 +263,39 @@ class HuberLoss(ReduceLoss):
 
 class MixLoss(Loss):
     """
-    Mixing Loss class
+    Weighted sum of losses, sum_k c_k L_k (vulkpy/nn/losses.py:396-458); its
+    gradient is the same weighted sum of the losses' gradients.  Terms are
+    scaled and accumulated left to right in FP32.
+
+    Parameters
+    ----------
+    losses : iterable of (float, vulkpy_amd.nn.Loss)
+        Coefficient and loss of every term.
+
+    Raises
+    ------
+    ValueError
+        When ``losses`` is empty
     """
     def __init__(self, losses: Iterable[Tuple[float, Loss]]):
-        """
-        Parameters
-        ----------
-        losses : iterable of tuple of float and vulkpy_amd.nn.Loss
-            Sets of coefficient and loss.
-
-        Raises
-        ------
-        ValueError
-            When losses is empty
-        """
         self.L: Tuple[Tuple[float, Loss], ...] = tuple(losses)
-        if len(self.L) < 1:
+        if not self.L:
             raise ValueError(f"losses should not empty")
 
     def __call__(self, x: Array, y: Array) -> Array:
-        return self._sum(lambda _L: _L(x, y))
+        """weighted sum of every loss of the batch ``(x, y)``"""
+        return self._weighted(lambda loss: loss(x, y))
 
     def grad(self) -> Array:
-        return self._sum(lambda _L: _L.grad())
+        """weighted sum of every loss's gradient for the last call"""
+        return self._weighted(lambda loss: loss.grad())
 
-    def _sum(self, F: Callable[[Loss], Array]) -> Array:
-        coeff, _L = self.L[0]
-        s = coeff * F(_L)
-
-        for coeff, _L in self.L[1:]:
-            s += coeff * F(_L)
-
-        return s
+    def _weighted(self, term: Callable[[Loss], Array]) -> Array:
+        total: Optional[Array] = None
+        for coeff, loss in self.L:
+            scaled = coeff * term(loss)
+            if total is None:
+                total = scaled
+            else:
+                total += scaled
+        return total

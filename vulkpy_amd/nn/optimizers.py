@@ -2,10 +2,16 @@
 Neural Network Optimizer Module (:mod:`vulkpy_amd.nn.optimizers`)
 =================================================================
 
-Same API as vulkpy/nn/optimizers.py:24-350.  ``grad2diff`` keeps the
-reference's Array-op composition; ``Parameter.update`` uses the fused
-``_update`` kernels, which apply the same FP32 operations in the same order
-(so both paths give identical weights).
+SGD, AdaGrad and Adam with the interface of vulkpy/nn/optimizers.py:24-350.
+
+Each state has two ways to step a parameter:
+
+* ``grad2diff(grad)`` returns the update as an Array built from array ops,
+  in the FP32 operation order of the reference's methods (cited on each), for
+  callers that apply the difference themselves;
+* ``_update(value, grad)``, used by ``Parameter.update``, is one fused kernel
+  that performs those same FP32 operations in the same order and adds the
+  result to ``value``, so both ways leave identical weights.
 """
 from __future__ import annotations
 import logging
@@ -24,26 +30,30 @@ __all__ = [
 
 logger = logging.getLogger("vulkpy")
 
+_NO_REG = (0, 0.0, 0.0)   # regularizer terms of the update kernels: none
+
 
 class SGDState(OptimizerState):
     """
-    Optimizer State for SGD
+    State of SGD for one parameter (SGD keeps nothing but its optimizer)
     """
     def __init__(self, opt: SGD):
         self.opt: SGD = opt
 
     def grad2diff(self, grad: Array) -> Array:
-        """diff = (-lr) * grad (vulkpy/nn/optimizers.py:53)"""
-        return (-self.opt.lr) * grad
+        """(-lr) * grad (vulkpy/nn/optimizers.py:53)"""
+        step = (-self.opt.lr) * grad
+        return step
 
     def _update(self, value: Array, grad: Array):
-        call("vkh_sgd", value._gpu._handle, value._ptr, grad._ptr, value.size, float(self.opt.lr))
+        call("vkh_sgd", value._gpu._handle, value._ptr, grad._ptr, value.size, float(self.opt.lr),
+             *_NO_REG)
         value._set_job([grad])
 
 
 class SGD(Optimizer):
     """
-    SGD Optimizer (constant learning rate)
+    Stochastic gradient descent with a constant learning rate
     """
     def __init__(self, lr: float):
         self.lr: float = lr
@@ -55,7 +65,8 @@ class SGD(Optimizer):
 
 class AdaGradState(OptimizerState):
     """
-    Optimizer State for AdaGrad
+    State of AdaGrad for one parameter: the running sum ``h`` of squared
+    gradients, starting at ``tau``
     """
     def __init__(self, opt: AdaGrad, shape: Iterable[int], tau: float):
         self.opt: AdaGrad = opt
@@ -63,25 +74,25 @@ class AdaGradState(OptimizerState):
         self.h[:] = tau
 
     def grad2diff(self, grad: Array) -> Array:
-        """vulkpy/nn/optimizers.py:134-140"""
+        """h += g^2; -lr * g / (sqrt(h) + eps), in the order of
+        vulkpy/nn/optimizers.py:134-140"""
         self.h += (grad ** 2)
-
-        sqrt = self.h.sqrt()  #               sqrt(sum)
-        sqrt += self.opt.eps  #               sqrt(sum) + eps
-        ret = grad / sqrt     #       grad / (sqrt(sum) + eps)
-        ret *= (-self.opt.lr) # -lr * grad / (sqrt(sum) + eps)
-        return ret
+        denom = self.h.sqrt()
+        denom += self.opt.eps
+        step = grad / denom
+        step *= (-self.opt.lr)
+        return step
 
     def _update(self, value: Array, grad: Array):
         call("vkh_adagrad", value._gpu._handle, value._ptr, grad._ptr, self.h._ptr, value.size,
-             float(self.opt.lr), float(self.opt.eps))
+             float(self.opt.lr), float(self.opt.eps), *_NO_REG)
         value._set_job([grad])
         self.h._set_job()
 
 
 class AdaGrad(Optimizer):
     """
-    AdaGrad Optimizer
+    AdaGrad Optimizer (per-element learning rate lr / (sqrt(sum g^2) + eps))
     """
     def __init__(self,
                  gpu: GPU, *,
@@ -102,7 +113,9 @@ class AdaGrad(Optimizer):
 
 class AdamState(OptimizerState):
     """
-    Optimizer State for Adam
+    State of Adam for one parameter: first and second moments ``m``, ``v``
+    and the powers ``beta1t``, ``beta2t`` of the decay rates (Python floats,
+    like the reference's)
     """
     def __init__(self, opt: Adam, shape: Iterable[int]):
         self.opt: Adam = opt
@@ -111,55 +124,50 @@ class AdamState(OptimizerState):
         self.beta1t: float = 1.0
         self.beta2t: float = 1.0
 
-    def grad2diff(self, grad: Array) -> Array:
-        """vulkpy/nn/optimizers.py:235-253"""
-        self.m *= self.opt.beta1
-        self.m += (1 - self.opt.beta1) * grad        # Allocate
-
-        self.v *= self.opt.beta2
-        self.v += (1 - self.opt.beta2) * (grad ** 2) # Allocate
-
+    def _advance_powers(self):
         self.beta1t *= self.opt.beta1
         self.beta2t *= self.opt.beta2
 
-        mhat = self.m / (1 - self.beta1t) # Allocate
-        vhat = self.v / (1 - self.beta2t) # Allocate
-
-        vhat.sqrt(inplace=True) # sqrt(vhat)
-        vhat += self.opt.eps    # sqrt(vhat) + eps
-
-        mhat *= (-self.opt.lr)  # -lr * mhat
-        mhat /= vhat            # -lr * mhat / (sqrt(vhat) + eps)
-
-        return mhat
+    def grad2diff(self, grad: Array) -> Array:
+        """The bias-corrected Adam step, with the array ops of
+        vulkpy/nn/optimizers.py:235-253 in their order"""
+        o = self.opt
+        # moments: m = b1 m + (1 - b1) g ; v = b2 v + (1 - b2) g^2
+        self.m *= o.beta1
+        self.m += (1 - o.beta1) * grad
+        self.v *= o.beta2
+        self.v += (1 - o.beta2) * (grad ** 2)
+        self._advance_powers()
+        # -lr * (m / (1 - b1^t)) / (sqrt(v / (1 - b2^t)) + eps)
+        m_hat = self.m / (1 - self.beta1t)
+        v_hat = self.v / (1 - self.beta2t)
+        v_hat.sqrt(inplace=True)
+        v_hat += o.eps
+        m_hat *= (-o.lr)
+        m_hat /= v_hat
+        return m_hat
 
     def _update(self, value: Array, grad: Array):
-        """Fused grad2diff + ``value += diff`` (one kernel)."""
-        self.beta1t *= self.opt.beta1
-        self.beta2t *= self.opt.beta2
+        """grad2diff + ``value += diff`` as one kernel"""
+        self._advance_powers()
+        o = self.opt
         call("vkh_adam", value._gpu._handle, value._ptr, grad._ptr, self.m._ptr, self.v._ptr,
-             value.size, float(self.opt.lr), float(self.opt.beta1), float(self.opt.beta2),
-             float(self.opt.eps), float(self.beta1t), float(self.beta2t))
+             value.size, float(o.lr), float(o.beta1), float(o.beta2), float(o.eps),
+             float(self.beta1t), float(self.beta2t))
         value._set_job([grad])
         self.m._set_job()
         self.v._set_job()
 
 
 class Adam(Optimizer):
-    r"""
-    Adam Optimizer
+    """
+    Adam Optimizer (Kingma and Ba, ICLR 2015)
 
-    .. math::
-
-         m_t = \beta _1 m_{t-1} + (1 - \beta _1) g_t\\
-         v_t = \beta _2 v_{t-1} + (1 - \beta _2) g_t ^2\\
-         \theta _t = \theta _{t-1} - \text{lr} \times
-         \hat{m}_t/(\sqrt{\hat{v}_t} + \epsilon)
-
-    References
-    ----------
-    .. [adam1] D. Kingma and J. Ba, "Adam: A Method for Stochastic Optimization",
-       ICLR (Poster) 2015
+    Per element, with gradient g and step count t: exponential moving
+    averages m <- beta1 m + (1 - beta1) g and v <- beta2 v + (1 - beta2) g^2,
+    then the parameter moves by -lr * m' / (sqrt(v') + eps), where
+    m' = m / (1 - beta1^t) and v' = v / (1 - beta2^t) undo the averages'
+    bias toward their zero start.
     """
     def __init__(self,
                  gpu: GPU, *,

@@ -2,8 +2,10 @@
 Random Module (:mod:`vulkpy_amd.random`)
 ========================================
 
-GPU-based Pseudo Random Number Generator (PRNG); same API as
-vulkpy/random.py.
+Device-side pseudo random numbers with the interface of vulkpy/random.py:
+one generator object holds ``size`` xoshiro128++ states in device memory, and
+every draw is a single kernel over them (the state of each stream stays in
+registers for all of its outputs).
 
 Examples
 --------
@@ -11,14 +13,12 @@ Examples
 >>> gpu = vk.GPU()
 >>> r = vk.random.Xoshiro128pp(gpu, seed=0)
 
-[0, 1) uniform random numbers can be generated by
-``random(shape=None, buffer=None)``.
+Uniform floats in [0, 1) (the reference's own example values for seed 0):
 
 >>> print(r.random(shape=(3,)))
 [0.42977667 0.8235899  0.90622926]
 
-Gaussian random numbers can be generated by
-``normal(shape=None, buffer=None, mean=0.0, stddev=1.0)``.
+Standard normal floats, continuing the same streams:
 
 >>> print(r.normal(shape=(3,)))
 [-2.3403292  0.7247794  0.7118352]
@@ -40,8 +40,9 @@ __all__ = ["Xoshiro128pp"]
 
 
 class PRNG(vk.Resource):
-    """Base class (vulkpy/random.py:40-189): normal() and randrange() on top
-    of the subclass's random()."""
+    """Generator interface (vulkpy/random.py:40-189).  A subclass supplies
+    ``random`` and ``randint``; ``normal`` and ``randrange`` here are built on
+    ``random`` (the Box-Muller and scaling kernels)."""
 
     _2p32 = int(2 ** 32)
 
@@ -64,7 +65,7 @@ class PRNG(vk.Resource):
             if shape is None:
                 raise ValueError("One of `shape` and  `buffer` must be specified.")
             return cls(gpu, shape=shape)
-        # For safety, we wait output buffer job.
+        # earlier work may still write the output buffer
         buffer.wait()
         return buffer
 
@@ -74,24 +75,24 @@ class PRNG(vk.Resource):
                mean: float = 0.0,
                stddev: float = 1.0) -> vk.Array:
         """
-        Generate Normal Distributing numbers
+        Normal draws with ``mean`` and ``stddev``
 
         Parameters
         ----------
         shape : iterable of ints, optional
-            If specified, new ``vulkpy_amd.Array`` with ``shape`` will be returned.
-        buffer : vulkpy_amd.Array
-            If specified, generated numbers will be stored.
+            Shape of a new output array.
+        buffer : vulkpy_amd.Array, optional
+            Existing output array (used instead of ``shape``).
 
         Returns
         -------
         vulkpy_amd.Array
-            Array which will get random numbers.
+            The output, filled when its job completes.
 
         Raises
         ------
         ValueError
-            If neither ``shape`` or ``buffer`` are specified
+            If neither ``shape`` nor ``buffer`` is given
 
         Notes
         -----
@@ -115,14 +116,13 @@ class PRNG(vk.Resource):
                   low: int = 0,
                   high: int = int(2 ** 32)) -> vk.U32Array:
         """
-        Generate [low, high) random numbers
+        Integers drawn from [low, high)
 
         Raises
         ------
         ValueError
-            If neither ``shape`` or ``buffer`` are specified.
-        ValueError
-            If not 0 <= low < high <= 2^32.
+            If neither ``shape`` nor ``buffer`` is given, or unless
+            0 <= low < high <= 2^32.
         """
         self._check_range(low, high)
         if (low == 0) and (high == self._2p32):
@@ -143,33 +143,28 @@ class PRNG(vk.Resource):
 
 class Xoshiro128pp(PRNG):
     """
-    xoshiro128++: Pseudo Random Number Generator
+    ``size`` parallel xoshiro128++ streams (Blackman and Vigna's 32-bit
+    generator with the ++ scrambler) in device memory
 
-    Notes
-    -----
-    This class implements xoshiro128++ [1]_. Initial internal states are
-    generated on the CPU exactly as vulkpy does (_vkarray.cc:643-675) and are
-    spaced by one application of its jump.  Each call runs one kernel in which
-    state i produces outputs i, i + size, i + 2 size, ... (_vkarray.cc:697-717).
-
-    References
-    ----------
-    .. [1] S. Vigna "xoshiro / xoroshiro generators and the PRNG shootout",
-       https://prng.di.unimi.it/
+    The initial states are those vulkpy computes (_vkarray.cc:643-675): a
+    splitmix64 chain from the seed gives stream 0, and stream i is stream
+    i - 1 advanced by the reference's jump as written.  Here they are formed
+    in parallel from GF(2) matrix powers of that jump, with identical bits.
+    A draw of n values assigns output k to stream ``k % size``, step
+    ``k // size`` (_vkarray.cc:697-717), in one kernel.
     """
 
     def __init__(self, gpu: vk.GPU, size: int = 64, *, seed: Optional[int] = None):
         """
-        Initialize Xoshiro128pp
-
         Parameters
         ----------
         gpu : vulkpy_amd.GPU
-            GPU where PRNG allocates
+            Device holding the streams' states.
         size : int
-            Number of internal states. These states generate random number parallelly.
+            Number of streams (default 64).
         seed : int, optional
-            Random seed. If ``None`` (default), use hardware random instead.
+            Seed of the splitmix64 chain; ``None`` (default) takes 32 bits
+            from the operating system's entropy source.
         """
         super().__init__(gpu)
         if seed is None:
@@ -196,12 +191,12 @@ class Xoshiro128pp(PRNG):
                shape: Optional[Iterable[int]] = None,
                buffer: Optional[vk.Array] = None) -> vk.Array:
         """
-        Generate [0, 1) floating numbers
+        Uniform floats in [0, 1): the top 23 bits of each draw as a mantissa
 
         Raises
         ------
         ValueError
-            If neither ``shape`` or ``buffer`` are specified.
+            If neither ``shape`` nor ``buffer`` is given.
         """
         buffer = cast(vk.Array, self._out(self._gpu, shape, buffer, vk.Array))
         call("vkh_xoshiro_float", self._rng, buffer._ptr, buffer.size)
@@ -212,12 +207,12 @@ class Xoshiro128pp(PRNG):
                 shape: Optional[Iterable[int]] = None,
                 buffer: Optional[vk.U32Array] = None) -> vk.U32Array:
         """
-        Generate [0, 2^32) unsigned integer numbers
+        Raw 32-bit draws (uint32 in [0, 2^32))
 
         Raises
         ------
         ValueError
-            If neither ``shape`` or ``buffer`` are specified.
+            If neither ``shape`` nor ``buffer`` is given.
         """
         buffer = cast(vk.U32Array, self._out(self._gpu, shape, buffer, vk.U32Array))
         call("vkh_xoshiro_uint32", self._rng, buffer._ptr, buffer.size)
@@ -230,8 +225,8 @@ class Xoshiro128pp(PRNG):
                mean: float = 0.0,
                stddev: float = 1.0) -> vk.Array:
         """
-        Generate Normal Distributing numbers (uniforms and Box-Muller fused
-        in one kernel; same values and state advance as PRNG.normal).
+        Normal draws: the uniforms and their Box-Muller transform in one
+        kernel, with the values and state advance of ``PRNG.normal``.
         """
         buffer = cast(vk.Array, self._out(self._gpu, shape, buffer, vk.Array))
         call("vkh_xoshiro_normal", self._rng, buffer._ptr, buffer.size,
@@ -245,15 +240,14 @@ class Xoshiro128pp(PRNG):
                   low: int = 0,
                   high: int = int(2 ** 32)) -> vk.U32Array:
         """
-        Generate [low, high) random numbers (``low + uint(range * u)`` with a
-        fused uniform draw, prng_randrange.comp:24-26)
+        Integers from [low, high) as ``low + uint(range * u)`` of a uniform
+        drawn in the same kernel (prng_randrange.comp:24-26)
 
         Raises
         ------
         ValueError
-            If neither ``shape`` or ``buffer`` are specified.
-        ValueError
-            If not 0 <= low < high <= 2^32.
+            If neither ``shape`` nor ``buffer`` is given, or unless
+            0 <= low < high <= 2^32.
         """
         self._check_range(low, high)
         if (low == 0) and (high == self._2p32):
