@@ -6,13 +6,25 @@ Workload (one "step"): the 8192 x 16384 float32 suite -- A ~ N(0, 1) from
 numpy.random.default_rng(0), broadcast operands col (8192, 1) and row
 (1, 16384), a second full operand B, and P ~ U[0.999, 1.001] for products:
 
-  * same-shape binary ops A (+ - * /) B                       12 N bytes each
-  * broadcast ops A (+ - * / max min) col and ... row         8 N + |operand|
-  * in-place broadcast ops A (+= -= *= /=) col / row          8 N + |operand|
-  * scalar op A * 2                                           8 N
+  * same-shape binary ops A (+ - * / max min) B, Q ** B        12 N bytes each
+    (Q = |A| + 0.1, the positive base of the powers)
+  * broadcast ops A (+ - * / max min) col and row, Q ** col,
+    Q ** row                                                   8 N + |operand|
+  * scalar ops A * 2, A + 1.5, Q ** 1.5, 1.5 ** A              8 N
+  * in-place pairs: A += B, -= B; A += col, -= col;
+    A *= row, /= row; A *= 2, /= 2; and Q **= 1                the ops' bytes
+  * clamp(A, -0.5, 0.5) and the 20 unary functions on their
+    domains (A; D ~ U[-1, 1) for asin / acos / atanh; Q for
+    log / log2 / sqrt / invsqrt; E = 1 + |A| for acosh)        8 N each
   * sum / maximum / minimum / mean over axis 0, 1 and None,
     prod of P over axis 0, 1 and None                         4 N + |output|
   * Xoshiro128++ with 2^20 states: 2^28 uint32, float, normal 4 * 2^28 + 32 * 2^20
+
+After the timed steps every op's result is compared once with FP64 NumPy on
+a sample (its first 64 rows; every output of axis-0 and full reductions; the
+first 2^22 outputs of fresh generators against the oracle):
+"per_op_max_ulp" is the largest error in units of the FP32 spacing at the
+FP64 value (0 for the generators' bit-exact uint32 and float streams).
 
 value = algorithmic bytes of all ops / wall time of the timed steps (GB/s,
 inputs resident in HBM).  The dominant kernel (broadcast A + col) is also
@@ -100,48 +112,215 @@ class Timer:
         return total / max(n, 1)
 
 
-def build_suite(vk, gpu, A, B, col, row, P, rng_gen, ubuf, fbuf, timer):
-    """List of (name, algorithmic bytes, callable) for one step."""
+S_ROWS = 64  # rows of the 8192 x 16384 operands in the per-op error sample
+
+# unary functions of the suite: (input name, FP64 reference)
+UNARY_SUITE = {
+    "abs": ("A", np.abs), "sign": ("A", np.sign), "sin": ("A", np.sin), "cos": ("A", np.cos),
+    "tan": ("A", np.tan), "asin": ("D", np.arcsin), "acos": ("D", np.arccos),
+    "atan": ("A", np.arctan), "sinh": ("A", np.sinh), "cosh": ("A", np.cosh),
+    "tanh": ("A", np.tanh), "asinh": ("A", np.arcsinh), "acosh": ("E", np.arccosh),
+    "atanh": ("D", np.arctanh), "exp": ("A", np.exp), "log": ("Q", np.log),
+    "exp2": ("A", np.exp2), "log2": ("Q", np.log2), "sqrt": ("Q", np.sqrt),
+    "invsqrt": ("Q", lambda x: 1.0 / np.sqrt(x)),
+}
+
+F64_BIN = {"add": np.add, "sub": np.subtract, "mul": np.multiply, "div": np.divide,
+           "max": np.maximum, "min": np.minimum, "pow": np.power}
+
+
+def build_suite(vk, gpu, ops_in, rng_gen, ubuf, fbuf, timer):
+    """
+    The operations of one step: a list of (name, algorithmic bytes, run,
+    check).  ``run()`` enqueues the op; ``check()`` (used once, after the
+    timed steps) runs it again without changing any suite operand and returns
+    (result sample, FP64 value of the same elements): the first S_ROWS rows
+    for elementwise ops and axis-1 reductions, every output of axis-0 and
+    full reductions.
+    """
+    from vulkpy_amd import _lib
+    A, B, col, row, P = (ops_in[k] for k in ("A", "B", "col", "row", "P"))
+    Q, D, E = (ops_in[k] for k in ("Q", "D", "E"))
     ops = []
     colb, rowb = 4 * M_ROWS, 4 * N_COLS
 
-    def add(name, nbytes, fn):
-        ops.append((name, nbytes, fn))
+    def head(arr, n=None):
+        """first n elements (default: S_ROWS rows) of a device array, as FP64"""
+        n = n if n is not None else min(arr.size, S_ROWS * N_COLS)
+        out = np.empty(n, dtype=np.float32)
+        _lib.call("vkh_download", gpu._handle, out.ctypes.data, arr._ptr, out.nbytes)
+        return out.astype(np.float64)
 
-    for o, f in (("add", lambda x, y: x + y), ("sub", lambda x, y: x - y),
-                 ("mul", lambda x, y: x * y), ("div", lambda x, y: x / y)):
-        add(f"{o}_same", 12 * N, (lambda f: lambda: f(A, B))(f))
-    for o, f in (("add", lambda x, y: x + y), ("sub", lambda x, y: x - y),
-                 ("mul", lambda x, y: x * y), ("div", lambda x, y: x / y),
-                 ("max", lambda x, y: x.max(y)), ("min", lambda x, y: x.min(y))):
+    def rows(arr):
+        return head(arr).reshape(S_ROWS, N_COLS)
+
+    def colv():
+        return head(col, S_ROWS).reshape(S_ROWS, 1)
+
+    def rowv():
+        return head(row, N_COLS).reshape(1, N_COLS)
+
+    def add(name, nbytes, fn, check=None):
+        ops.append((name, nbytes, fn, check))
+
+    binf = {"add": lambda x, y: x + y, "sub": lambda x, y: x - y, "mul": lambda x, y: x * y,
+            "div": lambda x, y: x / y, "max": lambda x, y: x.max(y), "min": lambda x, y: x.min(y),
+            "pow": lambda x, y: x ** y}
+    # same-shape binary ops (pow on the positive base Q = |A| + 0.1)
+    for o, f in binf.items():
+        X = Q if o == "pow" else A
+        add(f"{o}_same", 12 * N, (lambda f, X: lambda: f(X, B))(f, X),
+            (lambda f, X, o: lambda: (rows(f(X, B)), F64_BIN[o](rows(X), rows(B))))(f, X, o))
+    # broadcast binary ops against the (8192, 1) column and the (1, 16384) row
+    for o, f in binf.items():
+        X = Q if o == "pow" else A
         if o == "add":
             def timed(f=f):
                 timer.next_kernel()
                 return f(A, col)
-            add("add_col", 8 * N + colb, timed)
+            add("add_col", 8 * N + colb, timed, lambda: (rows(A + col), rows(A) + colv()))
         else:
-            add(f"{o}_col", 8 * N + colb, (lambda f: lambda: f(A, col))(f))
-        add(f"{o}_row", 8 * N + rowb, (lambda f: lambda: f(A, row))(f))
+            add(f"{o}_col", 8 * N + colb, (lambda f, X: lambda: f(X, col))(f, X),
+                (lambda f, X, o: lambda: (rows(f(X, col)), F64_BIN[o](rows(X), colv())))(f, X, o))
+        add(f"{o}_row", 8 * N + rowb, (lambda f, X: lambda: f(X, row))(f, X),
+            (lambda f, X, o: lambda: (rows(f(X, row)), F64_BIN[o](rows(X), rowv())))(f, X, o))
+    # scalar operands
+    add("mul_scalar", 8 * N, lambda: A * 2.0, lambda: (rows(A * 2.0), rows(A) * 2.0))
+    add("add_scalar", 8 * N, lambda: A + 1.5, lambda: (rows(A + 1.5), rows(A) + 1.5))
+    add("pow_scalar", 8 * N, lambda: Q ** 1.5, lambda: (rows(Q ** 1.5), rows(Q) ** 1.5))
+    add("rpow_scalar", 8 * N, lambda: 1.5 ** A, lambda: (rows(1.5 ** A), 1.5 ** rows(A)))
 
-    def inplace():
-        A.__iadd__(col)
-        A.__isub__(col)
-        A.__imul__(row)
-        A.__itruediv__(row)
-    add("inplace_bcast_x4", 4 * (8 * N) + 2 * colb + 2 * rowb, inplace)
-    add("mul_scalar", 8 * N, lambda: A * 2.0)
-    for f in ("sum", "maximum", "minimum", "mean"):
-        add(f"{f}_axis0", 4 * N + 4 * N_COLS, (lambda f: lambda: getattr(A, f)(axis=0))(f))
-        add(f"{f}_axis1", 4 * N + 4 * M_ROWS, (lambda f: lambda: getattr(A, f)(axis=1))(f))
-        add(f"{f}_all", 4 * N + 4, (lambda f: lambda: getattr(A, f)())(f))
-    add("prod_axis0", 4 * N + 4 * N_COLS, lambda: P.prod(axis=0))
-    add("prod_axis1", 4 * N + 4 * M_ROWS, lambda: P.prod(axis=1))
-    add("prod_all", 4 * N + 4, lambda: P.prod())
+    # in-place ops, in pairs that leave the operands (nearly) as they were;
+    # each check applies the pair's first op to a copy
+    def inplace_check(X, op, other, ref_other):
+        def check():
+            Z = X * 1.0
+            getattr(Z, op)(other)
+            return rows(Z), F64_BIN[{"__iadd__": "add", "__isub__": "sub", "__imul__": "mul",
+                                     "__itruediv__": "div", "__ipow__": "pow"}[op]](rows(X), ref_other())
+        return check
+
+    def pair(X, op1, op2, other):
+        def run():
+            getattr(X, op1)(other)
+            getattr(X, op2)(other)
+        return run
+
+    add("inplace_same_x2", 2 * 12 * N, pair(A, "__iadd__", "__isub__", B),
+        inplace_check(A, "__iadd__", B, lambda: rows(B)))
+    add("inplace_col_x2", 2 * (8 * N + colb), pair(A, "__iadd__", "__isub__", col),
+        inplace_check(A, "__iadd__", col, colv))
+    add("inplace_row_x2", 2 * (8 * N + rowb), pair(A, "__imul__", "__itruediv__", row),
+        inplace_check(A, "__imul__", row, rowv))
+    add("inplace_scalar_x2", 2 * 8 * N, pair(A, "__imul__", "__itruediv__", 2.0),
+        inplace_check(A, "__imul__", 2.0, lambda: 2.0))
+    add("inplace_pow_scalar", 8 * N, lambda: Q.__ipow__(1.0),
+        inplace_check(Q, "__ipow__", 1.0, lambda: 1.0))
+    # clamp with scalar bounds
+    add("clamp", 8 * N, lambda: A.clamp(-0.5, 0.5),
+        lambda: (rows(A.clamp(-0.5, 0.5)), np.clip(rows(A), -0.5, 0.5)))
+    # unary math
+    for fn, (src, ref) in UNARY_SUITE.items():
+        X = ops_in[src]
+        add(fn, 8 * N, (lambda fn, X: lambda: getattr(X, fn)())(fn, X),
+            (lambda fn, X, ref: lambda: (rows(getattr(X, fn)()), ref(rows(X))))(fn, X, ref))
+
+    # reductions (prod on P ~ U[0.999, 1.001])
+    full_cache = {}
+
+    def full_f64(X):
+        """FP64 column sums, log-products, maxima and minima of a whole operand
+        (read in row blocks; kept for the operand's other reductions)"""
+        if id(X) in full_cache:
+            return full_cache[id(X)]
+        cs, pr = np.zeros(N_COLS), np.zeros(N_COLS)
+        mx, mn = np.full(N_COLS, -np.inf), np.full(N_COLS, np.inf)
+        step_rows = 512
+        for r0 in range(0, M_ROWS, step_rows):
+            blk = head_rows(X, r0, step_rows)
+            cs += blk.sum(axis=0)
+            pr += np.log(np.abs(blk)).sum(axis=0)
+            mx = np.maximum(mx, blk.max(axis=0))
+            mn = np.minimum(mn, blk.min(axis=0))
+        full_cache[id(X)] = (cs, pr, mx, mn)
+        return full_cache[id(X)]
+
+    def head_rows(X, r0, nr):
+        out = np.empty(nr * N_COLS, dtype=np.float32)
+        _lib.call("vkh_download", gpu._handle, out.ctypes.data, X._ptr + 4 * r0 * N_COLS, out.nbytes)
+        return out.astype(np.float64).reshape(nr, N_COLS)
+
+    def red_check(X, f, axis):
+        def check():
+            got = np.asarray(getattr(X, f)(axis=axis), dtype=np.float64).reshape(-1)
+            if axis == 1:
+                r = rows(X)
+                exact = {"sum": r.sum(1), "maximum": r.max(1), "minimum": r.min(1), "mean": r.mean(1),
+                         "prod": np.exp(np.log(r).sum(1))}[f]
+                return got[:S_ROWS], exact
+            cs, lp, mx, mn = full_f64(X)
+            exact0 = {"sum": cs, "maximum": mx, "minimum": mn, "mean": cs / M_ROWS, "prod": np.exp(lp)}[f]
+            if axis == 0:
+                return got, exact0
+            exact = {"sum": cs.sum(), "maximum": mx.max(), "minimum": mn.min(), "mean": cs.sum() / N,
+                     "prod": np.exp(lp.sum())}[f]
+            return got, np.array([exact])
+        return check
+
+    for f in ("sum", "maximum", "minimum", "mean", "prod"):
+        X = P if f == "prod" else A
+        add(f"{f}_axis0", 4 * N + 4 * N_COLS, (lambda f, X: lambda: getattr(X, f)(axis=0))(f, X),
+            red_check(X, f, 0))
+        add(f"{f}_axis1", 4 * N + 4 * M_ROWS, (lambda f, X: lambda: getattr(X, f)(axis=1))(f, X),
+            red_check(X, f, 1))
+        add(f"{f}_all", 4 * N + 4, (lambda f, X: lambda: getattr(X, f)())(f, X), red_check(X, f, None))
+
+    # Xoshiro128++ with 2^20 states: checked on fresh generators of the same
+    # seed against the oracle's first 2^22 outputs (see prng_check)
     prng_bytes = 4 * PRNG_N + 32 * PRNG_STATES
-    add("xoshiro_uint32", prng_bytes, lambda: rng_gen.randint(buffer=ubuf))
-    add("xoshiro_float", prng_bytes, lambda: rng_gen.random(buffer=fbuf))
-    add("xoshiro_normal", prng_bytes, lambda: rng_gen.normal(buffer=fbuf))
+    add("xoshiro_uint32", prng_bytes, lambda: rng_gen.randint(buffer=ubuf), "prng")
+    add("xoshiro_float", prng_bytes, lambda: rng_gen.random(buffer=fbuf), "prng")
+    add("xoshiro_normal", prng_bytes, lambda: rng_gen.normal(buffer=fbuf), "prng")
     return ops
+
+
+def max_ulp(got, exact) -> float:
+    """largest |got - exact| in units of the FP32 spacing at |exact|"""
+    got = np.asarray(got, dtype=np.float64)
+    exact = np.asarray(exact, dtype=np.float64)
+    ok = np.isfinite(exact)
+    if not ok.all():  # inf / NaN must match exactly
+        bad = ~ok & ~((got == exact) | (np.isnan(got) & np.isnan(exact)))
+        if bad.any():
+            return float("inf")
+    sp = np.spacing(np.abs(exact[ok]).astype(np.float32)).astype(np.float64)
+    if not ok.any():
+        return 0.0
+    return float((np.abs(got[ok] - exact[ok]) / sp).max())
+
+
+def prng_check(vk, gpu, n_check: int = 1 << 22) -> dict:
+    """Max ulp (0 = bit for bit) of the three generator ops against the oracle:
+    each on a fresh Xoshiro128pp(size=2^20, seed=1) drawing the suite's 2^28
+    outputs, whose first n_check (four per state) the oracle restates."""
+    from oracle import oracle as O
+    out = {}
+    for name in ("xoshiro_uint32", "xoshiro_float", "xoshiro_normal"):
+        g = vk.random.Xoshiro128pp(gpu, size=PRNG_STATES, seed=1)
+        o = O.Xoshiro128pp(PRNG_STATES, 1)
+        if name == "xoshiro_uint32":
+            got = np.asarray(g.randint(shape=(PRNG_N,)))[:n_check]
+            out[name] = 0.0 if np.array_equal(got, o.randint(n_check)) else float("inf")
+        elif name == "xoshiro_float":
+            got = np.asarray(g.random(shape=(PRNG_N,)))[:n_check]
+            want = o.random(n_check)
+            out[name] = 0.0 if np.array_equal(got.view(np.uint32), want.view(np.uint32)) else float("inf")
+        else:
+            got = np.asarray(g.normal(shape=(PRNG_N,)))[:n_check]
+            out[name] = round(max_ulp(got, o.normal(n_check).astype(np.float64)), 3)
+        del g
+        gpu.wait()
+    return out
 
 
 PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0, "bf16x3": 2500.0 / 3}  # split: 3 BF16 products
@@ -616,6 +795,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "per_op_GBps": c2["per_op"] if c2 else None,
+        "per_op_max_ulp": c2["per_op_ulp"] if c2 else None,
         "reduce_tuning": c2["reduce_tuning"] if c2 else None,
         "mlp": mlp,
         "gemm": gemm,
@@ -649,12 +829,15 @@ def c2_suite(vk, gpu, args, timer):
     b_host = rng.standard_normal((M_ROWS, N_COLS), dtype=np.float32)
     p_host = rng.uniform(0.999, 1.001, (M_ROWS, N_COLS)).astype(np.float32)
 
+    d_host = rng.uniform(-1, 1, (M_ROWS, N_COLS)).astype(np.float32)
+
     A = vk.Array(gpu, data=a_host)
     B = vk.Array(gpu, data=b_host)
     col = vk.Array(gpu, data=col_host)
     row = vk.Array(gpu, data=row_host)
     P = vk.Array(gpu, data=p_host)
-    del b_host
+    D = vk.Array(gpu, data=d_host)
+    del b_host, d_host
     if args.pmc_kernels:
         # For rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs: a plain copy of
         # known traffic (512 MiB read + 512 MiB written per launch) as the
@@ -688,12 +871,17 @@ def c2_suite(vk, gpu, args, timer):
                                "ms_default": r["ms"][0], "ms_best": r["ms"][r["best"]]}
         log(f"bench: reduce {name} {r}")
 
-    suite = build_suite(vk, gpu, A, B, col, row, P, gen, ubuf, fbuf, timer)
-    step_bytes = sum(nb for _, nb, _ in suite)
+    # domain operands of the unary functions: Q = |A| + 0.1 > 0 (log, sqrt,
+    # pow's base), E = 1 + |A| >= 1 (acosh), D ~ U[-1, 1) (asin, acos, atanh)
+    Q = A.abs() + 0.1
+    E = A.abs() + 1.0
+    ops_in = {"A": A, "B": B, "col": col, "row": row, "P": P, "Q": Q, "D": D, "E": E}
+    suite = build_suite(vk, gpu, ops_in, gen, ubuf, fbuf, timer)
+    step_bytes = sum(op[1] for op in suite)
 
     def step():
-        for _, _, f in suite:
-            f()
+        for op in suite:
+            op[2]()
 
     log(f"bench: warmup {args.warmup} steps")
     for _ in range(args.warmup):
@@ -718,7 +906,7 @@ def c2_suite(vk, gpu, args, timer):
     # stream, mean of 3 launches, so host latency is not counted
     per_op = {}
     op_timer = Timer(gpu)
-    for name, nb, f in suite:
+    for name, nb, f, _ in suite:
         gpu.wait()
         for _ in range(3):
             t = op_timer.start()
@@ -728,12 +916,22 @@ def c2_suite(vk, gpu, args, timer):
     gpu.wait()
     timer.mean_ms()  # drop the dominant op's pairs from this pass
 
-    del suite, ubuf, fbuf, gen, A, B, P  # free the suite's device buffers
+    # per-op error against FP64 on the sample each check returns (reported)
+    log("bench: per-op error against FP64")
+    per_op_ulp = {}
+    for name, _, _, check in suite:
+        if callable(check):
+            got, exact = check()
+            per_op_ulp[name] = round(max_ulp(got, exact), 3)
     gpu.wait()
+    timer.mean_ms()
+    del suite, ubuf, fbuf, gen, A, B, P, Q, D, E, ops_in  # free the suite's device buffers
+    gpu.wait()
+    per_op_ulp.update(prng_check(vk, gpu))
     return {
         "value": value, "ms_per_step": dt / args.steps * 1e3, "n_ops": n_ops,
         "step_bytes": step_bytes, "achieved": achieved, "dom_ms": dom_ms, "per_op": per_op,
-        "reduce_tuning": reduce_tuning,
+        "per_op_ulp": per_op_ulp, "reduce_tuning": reduce_tuning,
         "host": {"a": a_host, "col": col_host, "row": row_host, "p": p_host},
     }
 

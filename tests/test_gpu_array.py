@@ -319,6 +319,59 @@ def test_c1_transcendentals(gpu, c1):
     assert np.allclose(np.asarray(got), p, rtol=1e-6)
 
 
+# every unary function: the domain each computes on, the input's map from the
+# C1 operand A ~ U[-1, 1), and the FP64 reference
+UNARY = {
+    "abs": (lambda a: a * 8, np.abs), "sign": (lambda a: a, np.sign),
+    "sin": (lambda a: a * 8, np.sin), "cos": (lambda a: a * 8, np.cos),
+    "tan": (lambda a: a * 8, np.tan), "asin": (lambda a: a, np.arcsin),
+    "acos": (lambda a: a, np.arccos), "atan": (lambda a: a * 8, np.arctan),
+    "sinh": (lambda a: a * 8, np.sinh), "cosh": (lambda a: a * 8, np.cosh),
+    "tanh": (lambda a: a * 8, np.tanh), "asinh": (lambda a: a * 8, np.arcsinh),
+    "acosh": (lambda a: 1 + np.abs(a) * 8, np.arccosh), "atanh": (lambda a: a, np.arctanh),
+    "exp": (lambda a: a * 8, np.exp), "log": (lambda a: np.abs(a) * 8, np.log),
+    "exp2": (lambda a: a * 8, np.exp2), "log2": (lambda a: np.abs(a) * 8, np.log2),
+    "sqrt": (lambda a: np.abs(a) * 8, np.sqrt), "invsqrt": (lambda a: np.abs(a) * 8, lambda x: 1 / np.sqrt(x)),
+}
+
+# the bit patterns (magnitudes) each function is also checked over: random
+# FP32 values spread over its whole computing range, both signs where defined
+SPAN = {"sin": 0x49800000, "cos": 0x49800000, "tan": 0x49800000, "asin": 0x3f800001,
+        "acos": 0x3f800001, "atanh": 0x3f800000, "sinh": 0x42b30000, "cosh": 0x42b30000,
+        "exp": 0x42d00000, "exp2": 0x43160000, "acosh": 0x7f800000, "log": 0x7f800000,
+        "log2": 0x7f800000, "sqrt": 0x7f800000, "invsqrt": 0x7f800000}
+POSITIVE = {"log", "log2", "sqrt", "invsqrt", "acosh"}
+
+
+@pytest.mark.parametrize("fn", list(UNARY))
+def test_unary_ulp_every_function(gpu, c1, fn):
+    """Each of the 20 unary functions within 0.51 ulp of its FP64 value on
+    the C1 operand mapped into the function's domain, and on 2^20 FP32 values
+    drawn from its whole computing range (abs, sign, sqrt exactly)."""
+    a, _ = c1
+    mapf, ref = UNARY[fn]
+    rng = np.random.default_rng(list(UNARY).index(fn))
+    hi = SPAN.get(fn, 0x7f800000)
+    bits = rng.integers(0x3f800000 if fn == "acosh" else 0, hi, 1 << 20, dtype=np.int64).astype(np.uint32)
+    if fn not in POSITIVE:
+        bits[::2] |= np.uint32(0x80000000)
+    wide = bits.view(np.float32)
+    for x in (mapf(a).astype(np.float32), wide):
+        got = np.asarray(getattr(A(gpu, x), fn)())
+        with np.errstate(all="ignore"):
+            exact = ref(x.astype(np.float64))
+        finite = np.isfinite(exact) & (np.abs(exact) <= np.finfo(np.float32).max)
+        if fn in ("abs", "sign", "sqrt"):
+            assert_bits(got, exact.astype(np.float32))
+            continue
+        err = ulp_err(got[finite], exact[finite])
+        assert err.max() <= 0.51, (fn, float(err.max()), x[finite][np.argmax(err)])
+        # overflow to inf and NaN where the FP64 value is out of range
+        over = ~finite
+        assert np.array_equal(np.isnan(got[over]), np.isnan(exact[over])), fn
+        assert np.all(got[over & ~np.isnan(exact)] == exact[over & ~np.isnan(exact)].astype(np.float32)), fn
+
+
 def test_c1_reductions(gpu, c1):
     a, _ = c1
     ga = A(gpu, a)

@@ -399,7 +399,8 @@ def test_fastdiv_header_on_host(tmp_path):
 
 
 def test_unary_header_on_host(tmp_path):
-    # tanh / log / log2 / pow / sinh / cosh / asinh / acosh / atanh of
+    # tanh / log / log2 / pow / sinh / cosh / asinh / acosh / atanh / sin /
+    # cos / tan / asin / acos / atan / exp / exp2 / invsqrt of
     # vulkpy_amd/csrc/bmath.h (the elementwise kernels' FP64 evaluations)
     # against the FP64 library functions: every 997th FP32 input of each
     # computing range (pow: every 63811th positive base against 31 exponents)
@@ -412,9 +413,9 @@ def test_unary_header_on_host(tmp_path):
                     os.path.join(root, "tests", "unary_check.cpp"), "-o", str(exe)],
                    check=True, capture_output=True)
     out = subprocess.run([str(exe), "997"], check=True, capture_output=True, text=True).stdout.split()
-    assert len(out) == 11, out
-    assert all(float(e) <= 0.5 + 2 ** -13 for e in out[:9]), out
-    assert int(out[9]) > 10 ** 7 and out[10] == "0", out
+    assert len(out) == 20, out
+    assert all(float(e) <= 0.5 + 2 ** -13 for e in out[:18]), out
+    assert int(out[18]) > 3 * 10 ** 7 and out[19] == "0", out
 
 
 def test_sincos_header_on_host(tmp_path):

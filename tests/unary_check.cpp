@@ -1,13 +1,13 @@
-// Host check of the FP32 transcendentals of vulkpy_amd/csrc/bmath.h (tanh,
-// log, log2, pow, sinh, cosh, asinh, acosh, atanh) against the FP64 library
-// functions: every FP32 input whose bit pattern is a multiple of the stride
-// (argv[1]; 1 = every input) in the range where each function computes (both
-// signs where the function is defined for both), and for pow every positive
-// finite base at 64 times the stride against 31 exponents.  Prints the
-// largest error of each function in units of the last place of the FP64
-// reference rounded to float, in the order above, then the number of inputs
-// checked and the number of special-value mismatches (signed zeros,
-// negatives, infinities, NaN, 1, powers of two, domain edges).
+// Host check of the FP32 transcendentals of vulkpy_amd/csrc/bmath.h against
+// the FP64 library functions: every FP32 input whose bit pattern is a
+// multiple of the stride (argv[1]; 1 = every input) in the range where each
+// function computes (both signs where the function is defined for both), and
+// for pow every positive finite base at 64 times the stride against 31
+// exponents.  Prints the largest error of each function in units of the last
+// place of the FP64 reference rounded to float, in the order of the table
+// below (pow fourth), then the number of inputs checked and the number of
+// special-value mismatches (signed zeros, negatives, infinities, NaN, 1,
+// powers of two, domain edges).
 #include "../vulkpy_amd/csrc/bmath.h"
 
 #include <cmath>
@@ -49,6 +49,15 @@ static double d_cosh(double x) { return std::cosh(x); }
 static double d_asinh(double x) { return std::asinh(x); }
 static double d_acosh(double x) { return std::acosh(x); }
 static double d_atanh(double x) { return std::atanh(x); }
+static double d_sin(double x) { return std::sin(x); }
+static double d_cos(double x) { return std::cos(x); }
+static double d_tan(double x) { return std::tan(x); }
+static double d_asin(double x) { return std::asin(x); }
+static double d_acos(double x) { return std::acos(x); }
+static double d_atan(double x) { return std::atan(x); }
+static double d_exp(double x) { return std::exp(x); }
+static double d_exp2(double x) { return std::exp2(x); }
+static double d_invsqrt(double x) { return 1.0 / std::sqrt(x); }
 
 int main(int argc, char** argv) {
   const uint64_t stride = argc > 1 ? std::strtoull(argv[1], nullptr, 10) : 1;
@@ -62,6 +71,15 @@ int main(int argc, char** argv) {
       {vkh::asinh_f32, d_asinh, 0u, 0x7f800000u, true},      // all finite
       {vkh::acosh_f32, d_acosh, 0x3f800000u, 0x7f800000u, false},  // [1, max]
       {vkh::atanh_f32, d_atanh, 0u, 0x3f800000u, true},      // |x| < 1
+      {vkh::sin_f32, d_sin, 0u, 0x49800000u, true},          // |x| < 2^20 (reduced here)
+      {vkh::cos_f32, d_cos, 0u, 0x49800000u, true},
+      {vkh::tan_f32, d_tan, 0u, 0x49800000u, true},
+      {vkh::asin_f32, d_asin, 0u, 0x3f800001u, true},        // |x| <= 1
+      {vkh::acos_f32, d_acos, 0u, 0x3f800001u, true},
+      {vkh::atan_f32, d_atan, 0u, 0x7f800001u, true},        // all, inf included
+      {vkh::exp_f32, d_exp, 0u, 0x42d00000u, true},          // |x| < 104
+      {vkh::exp2_f32, d_exp2, 0u, 0x43160000u, true},        // |x| < 150
+      {vkh::invsqrt_f32, d_invsqrt, 1u, 0x7f800000u, false}, // positive finite
   };
   constexpr int NF = sizeof(fns) / sizeof(fns[0]);
   double err[NF] = {};

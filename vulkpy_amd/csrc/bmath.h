@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
+#include <cstring>
 
 namespace vkh {
 
@@ -35,23 +37,75 @@ __host__ __device__ inline void sincos_small(float x, float& s, float& c) {
 
 // The FP32 functions below evaluate in FP64 with a relative error below 2^-40
 // and round once, so they are within 0.5 ulp + 2^-16 ulp of the exact value,
-// like a full FP64 libm call rounded to FP32, at a fraction of its cost (the
-// FP64 tanh and log of the device library take 4x and 2x the time of
-// streaming the data on MI355X).
+// like a full FP64 libm call rounded to FP32, at a fraction of its cost: none
+// calls an FP64 library transcendental on its common path (the device
+// library's FP64 tan, asin, tanh or log take 2x to 4x the time of streaming
+// the data on MI355X).  The cores are polynomials in FP64 after an exact or
+// near-exact argument reduction; every constant is either 1/n!, 1/(2n+1), a
+// value of pi / 2 / ln 2 / atan split into pieces, or the coefficients of a
+// Chebyshev fit made for this file (asin below).
 
-// tanh: |x| < 2^-12: x (1 - x^2/3), whose omitted terms are below 2^-49
-// relative; otherwise (e - 1) / (e + 1) with e = exp(2|x|), where e - 1
-// (>= 2^-11) keeps all but 11 bits of e's precision; 1 from |x| >= 9.5 on,
-// where 1 - tanh(x) < 2^-26 rounds to 1.  NaN stays NaN; the sign of zero
-// is kept.
-__host__ __device__ inline float tanh_f32(float x) {
-  const double d = (double)x, a = fabs(d);
-  if (!(a < 9.5)) return a == a ? copysignf(1.0f, x) : x;
-  if (a < 0x1p-12) return (float)(d * fma(-d * d, 1.0 / 3.0, 1.0));
-  const double e = exp(2.0 * a);
-  return copysignf((float)((e - 1.0) / (e + 1.0)), x);
+// ---- pieces of constants ----------------------------------------------------
+// pi/2 = PIO2_1 + PIO2_2 + PIO2_3 to 2^-120: the first two have 33
+// significant bits, so q * PIO2_1 and q * PIO2_2 are exact for |q| < 2^20.
+constexpr double PIO2_1 = 0x1.921fb54400000p+0;
+constexpr double PIO2_2 = 0x1.0b4611a600000p-34;
+constexpr double PIO2_3 = 0x1.3198a2e037073p-69;
+constexpr double PIO2 = 1.5707963267948966, PI = 3.141592653589793;
+constexpr double TWO_OVER_PI = 0.6366197723675814;
+// ln 2 = LN2_HI + LN2_LO, LN2_HI with 32 significant bits (k * LN2_HI exact
+// for |k| < 2^21)
+constexpr double LN2_HI = 0x1.62e42fee00000p-1;
+constexpr double LN2_LO = 0x1.a39ef35793c76p-33;
+constexpr double LN2 = 0.6931471805599453, LOG2E = 1.4426950408889634;
+
+// 2^k for integral k in [-1022, 1023], from its bit pattern
+__host__ __device__ inline double pow2i(int k) {
+  const uint64_t bits = (uint64_t)(k + 1023) << 52;
+  double d;
+  memcpy(&d, &bits, 8);
+  return d;
 }
 
+// ---- exp ----------------------------------------------------------------------
+// e^t for |t| <= 700: t = k ln 2 + r with k = rint(t / ln 2) and |r| <= ln 2 / 2
+// (two-piece ln 2: r is exact to 2^-85 relative), then the Taylor polynomial
+// of e^r to r^12 (truncation below 2^-52 relative) and the exact scaling by
+// 2^k.
+__host__ __device__ inline double exp_core(double t) {
+  const double k = rint(t * LOG2E);
+  double r = fma(-k, LN2_HI, t);
+  r = fma(-k, LN2_LO, r);
+  double p = 1.0 / 479001600.0;            // 1/12!
+  p = fma(p, r, 1.0 / 39916800.0);         // 1/11!
+  p = fma(p, r, 1.0 / 3628800.0);
+  p = fma(p, r, 1.0 / 362880.0);
+  p = fma(p, r, 1.0 / 40320.0);
+  p = fma(p, r, 1.0 / 5040.0);
+  p = fma(p, r, 1.0 / 720.0);
+  p = fma(p, r, 1.0 / 120.0);
+  p = fma(p, r, 1.0 / 24.0);
+  p = fma(p, r, 1.0 / 6.0);
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  return p * pow2i((int)k);
+}
+
+// e^t rounded to FP32: 0 below -104 (e^-104 < 2^-150, half the smallest
+// denormal), inf above 89 (e^89 > FP32 max); NaN stays NaN
+__host__ __device__ inline float exp_to_f32(double t) {
+  if (!(t <= 89.0)) return t != t ? (float)t : INFINITY;
+  if (t < -104.0) return 0.0f;
+  return (float)exp_core(t);
+}
+
+__host__ __device__ inline float exp_f32(float x) { return exp_to_f32((double)x); }
+
+// 2^x: x ln 2 in FP64 carries an absolute error below 2^-46 for |x| < 150
+__host__ __device__ inline float exp2_f32(float x) { return exp_to_f32((double)x * LN2); }
+
+// ---- log ----------------------------------------------------------------------
 // atanh(s) = s + s^3/3 + ... + s^15/15 for |s| <= 0.1716, whose omitted
 // terms are below 2^-45 relative
 __host__ __device__ inline double atanh_series(double s) {
@@ -65,78 +119,94 @@ __host__ __device__ inline double atanh_series(double s) {
   return fma(s * z, p, s);
 }
 
-// Natural log before rounding: with x = m 2^k, m in [sqrt(1/2), sqrt(2)),
-// log m = 2 atanh(s) for s = (m - 1) / (m + 1), |s| <= 0.1716.  k_out
-// receives k and the result is log m; x = 1 gives exactly 0.  x must be
-// positive and finite.
+// Natural log before rounding: x = m 2^k with m in [sqrt(1/2), sqrt(2)) read
+// from the bit pattern, log m = 2 atanh(s) for s = (m - 1) / (m + 1),
+// |s| <= 0.1716.  k_out receives k and the result is log m; x = 1 gives
+// exactly 0.  x must be a positive normal double (every positive finite FP32
+// value is).
 __host__ __device__ inline double log_mant(double x, int& k_out) {
-  int k;
-  double m = frexp(x, &k);  // m in [0.5, 1)
-  if (m < 0.70710678118654752) {
-    m *= 2.0;
-    --k;
+  uint64_t bits;
+  memcpy(&bits, &x, 8);
+  int k = (int)(bits >> 52) - 1023;
+  bits = (bits & 0x000fffffffffffffull) | 0x3ff0000000000000ull;  // m in [1, 2)
+  double m;
+  memcpy(&m, &bits, 8);
+  if (m > 1.4142135623730951) {
+    m *= 0.5;
+    ++k;
   }
   k_out = k;
   return 2.0 * atanh_series((m - 1.0) / (m + 1.0));
 }
 
-// log of a positive finite double, FP64 error below 2^-44 relative
+// log of a positive normal double, FP64 error below 2^-44 relative
 __host__ __device__ inline double log_pos(double x) {
   int k;
   const double lm = log_mant(x, k);
-  return fma((double)k, 0.69314718055994531, lm);
+  return fma((double)k, LN2_HI, fma((double)k, LN2_LO, lm));
 }
 
 // log and log2 of FP32 x through log_mant: k ln 2 + log m and k + log m /
-// ln 2, each with one rounding in FP64 before the FP32 rounding; exact at
-// x = 1 (and log2 at every power of two).  0 -> -inf, negative -> NaN, +inf
-// and NaN pass through.
+// ln 2, each rounded in FP64 before the FP32 rounding; exact at x = 1 (and
+// log2 at every power of two).  0 -> -inf, negative -> NaN, +inf and NaN pass
+// through.
 __host__ __device__ inline float log_f32(float x) {
   if (!(x > 0.0f) || x == INFINITY) return x == 0.0f ? -INFINITY : (x < 0.0f ? NAN : x);
-  int k;
-  const double lm = log_mant(x, k);
-  return (float)fma((double)k, 0.69314718055994531, lm);
+  return (float)log_pos((double)x);
 }
 
 __host__ __device__ inline float log2_f32(float x) {
   if (!(x > 0.0f) || x == INFINITY) return x == 0.0f ? -INFINITY : (x < 0.0f ? NAN : x);
   int k;
   const double lm = log_mant(x, k);
-  return (float)fma(lm, 1.4426950408889634, (double)k);
+  return (float)fma(lm, LOG2E, (double)k);
 }
 
 // a ** b (C pow semantics) for FP32 operands.  A finite positive base with a
-// finite exponent takes exp(b log a) in FP64: log a from log_mant (relative
-// error below 2^-45) gives b log a with an absolute error below 2^-37 for
-// every |b log a| < 150, beyond which the FP32 result is 0 or inf either way,
-// so the result is within 0.5 ulp + 2^-13 ulp.  Every other case (zero,
+// finite exponent takes exp(b log a) in FP64: log a with a relative error
+// below 2^-44 gives b log a with an absolute error below 2^-36 for every
+// |b log a| < 105, beyond which the FP32 result is 0 or inf either way, so
+// the result is within 0.5 ulp + 2^-12 ulp.  Every other case (zero,
 // negative or infinite base, non-finite exponent) is the FP64 pow rounded.
 __host__ __device__ inline float pow_f32(float a, float b) {
-  if (a > 0.0f && a < INFINITY && fabsf(b) < INFINITY) {
-    int k;
-    const double lm = log_mant(a, k);
-    return (float)exp((double)b * fma((double)k, 0.69314718055994531, lm));
-  }
+  if (a > 0.0f && a < INFINITY && fabsf(b) < INFINITY) return exp_to_f32((double)b * log_pos((double)a));
   return (float)pow((double)a, (double)b);
+}
+
+// ---- hyperbolic functions ---------------------------------------------------
+// tanh: |x| < 2^-12: x (1 - x^2/3), whose omitted terms are below 2^-49
+// relative; otherwise (e - 1) / (e + 1) with e = exp(2|x|), where e - 1
+// (>= 2^-11) keeps all but 11 bits of e's precision; 1 from |x| >= 9.5 on,
+// where 1 - tanh(x) < 2^-26 rounds to 1.  NaN stays NaN; the sign of zero
+// is kept.
+__host__ __device__ inline float tanh_f32(float x) {
+  const double d = (double)x, a = fabs(d);
+  if (!(a < 9.5)) return a == a ? copysignf(1.0f, x) : x;
+  if (a < 0x1p-12) return (float)(d * fma(-d * d, 1.0 / 3.0, 1.0));
+  const double e = exp_core(2.0 * a);
+  return copysignf((float)((e - 1.0) / (e + 1.0)), x);
 }
 
 // sinh: |x| < 2^-5: x (1 + x^2/6 + x^4/120 + x^6/5040), omitted terms below
 // 2^-45 relative; otherwise (e - 1/e) / 2 with e = exp(|x|), where the
-// difference (>= 2^-4) keeps all but 5 bits of e's precision.  Overflow to
-// inf, NaN and the sign of zero follow the FP64 function.
+// difference (>= 2^-4) keeps all but 5 bits of e's precision; +-inf beyond
+// |x| = 90 (sinh(90) > FP32 max).  NaN and the sign of zero are kept.
 __host__ __device__ inline float sinh_f32(float x) {
   const double d = (double)x, a = fabs(d);
   if (a < 0x1p-5) {
     const double z = d * d;
     return (float)(d * fma(z, fma(z, fma(z, 1.0 / 5040.0, 1.0 / 120.0), 1.0 / 6.0), 1.0));
   }
-  const double e = exp(a);
+  if (!(a < 90.0)) return a == a ? copysignf(INFINITY, x) : x;
+  const double e = exp_core(a);
   return copysignf((float)(0.5 * (e - 1.0 / e)), x);
 }
 
-// cosh: (e + 1/e) / 2 with e = exp(|x|) (no cancellation)
+// cosh: (e + 1/e) / 2 with e = exp(|x|) (no cancellation); inf beyond 90
 __host__ __device__ inline float cosh_f32(float x) {
-  const double e = exp(fabs((double)x));
+  const double a = fabs((double)x);
+  if (!(a < 90.0)) return a == a ? INFINITY : x;
+  const double e = exp_core(a);
   return (float)(0.5 * (e + 1.0 / e));
 }
 
@@ -173,5 +243,164 @@ __host__ __device__ inline float atanh_f32(float x) {
   if (a == 1.0) return copysignf(INFINITY, x);
   return copysignf((float)(0.5 * log_pos((1.0 + a) / (1.0 - a))), x);
 }
+
+// ---- circular functions -------------------------------------------------------
+// Below 2^-13 in magnitude sin, tan, asin and atan differ from x by less than
+// x^2/3 < 2^-26 relative, half an FP32 ulp at most, and cos from 1 by less
+// than 2^-27: those round to x and to 1 (the sign of zero is kept).
+constexpr double TINY = 0x1p-13;
+
+// sin and cos of r, |r| <= pi/4 (plus rounding): Taylor polynomials to r^15
+// and r^16, truncation errors below 2^-55 relative
+__host__ __device__ inline double sin_poly(double r) {
+  const double z = r * r;
+  double p = -1.0 / 1307674368000.0;       // -1/15!
+  p = fma(p, z, 1.0 / 6227020800.0);       // 1/13!
+  p = fma(p, z, -1.0 / 39916800.0);
+  p = fma(p, z, 1.0 / 362880.0);
+  p = fma(p, z, -1.0 / 5040.0);
+  p = fma(p, z, 1.0 / 120.0);
+  p = fma(p, z, -1.0 / 6.0);
+  return fma(r * z, p, r);
+}
+
+__host__ __device__ inline double cos_poly(double r) {
+  const double z = r * r;
+  double p = 1.0 / 20922789888000.0;       // 1/16!
+  p = fma(p, z, -1.0 / 87178291200.0);     // -1/14!
+  p = fma(p, z, 1.0 / 479001600.0);
+  p = fma(p, z, -1.0 / 3628800.0);
+  p = fma(p, z, 1.0 / 40320.0);
+  p = fma(p, z, -1.0 / 720.0);
+  p = fma(p, z, 1.0 / 24.0);
+  p = fma(p, z, -0.5);
+  return fma(z, p, 1.0);
+}
+
+// x = q pi/2 + r with |r| <= pi/4, for |x| < 2^20: q from FP64 rounding,
+// r from the three pieces of pi/2 (the first two products exact), accurate to
+// 2^-70 absolute, far below r itself next to any multiple of pi/2 an FP32
+// value can come near.  Returns q mod 4.
+__host__ __device__ inline int reduce_pio2(double x, double& r) {
+  const double q = rint(x * TWO_OVER_PI);
+  double t = fma(-q, PIO2_1, x);
+  t = fma(-q, PIO2_2, t);
+  r = fma(-q, PIO2_3, t);
+  return (int)q & 3;
+}
+
+constexpr double REDUCE_MAX = 0x1p20;  // beyond: the FP64 library functions
+
+__host__ __device__ inline float sin_f32(float x) {
+  const double d = (double)x, a = fabs(d);
+  if (a < TINY) return x;
+  if (!(a < REDUCE_MAX)) return (float)sin(d);  // huge, inf, NaN
+  double r;
+  const int q = reduce_pio2(d, r);
+  const double v = (q & 1) ? cos_poly(r) : sin_poly(r);
+  return (float)((q & 2) ? -v : v);
+}
+
+__host__ __device__ inline float cos_f32(float x) {
+  const double d = (double)x, a = fabs(d);
+  if (a < TINY) return 1.0f;
+  if (!(a < REDUCE_MAX)) return (float)cos(d);
+  double r;
+  const int q = reduce_pio2(d, r);
+  const double v = (q & 1) ? sin_poly(r) : cos_poly(r);
+  return (float)(((q + 1) & 2) ? -v : v);
+}
+
+// tan = sin / cos of the reduced argument, or -cos / sin in odd quadrants
+__host__ __device__ inline float tan_f32(float x) {
+  const double d = (double)x, a = fabs(d);
+  if (a < TINY) return x;
+  if (!(a < REDUCE_MAX)) return (float)tan(d);
+  double r;
+  const int q = reduce_pio2(d, r);
+  const double s = sin_poly(r), c = cos_poly(r);
+  return (float)((q & 1) ? -c / s : s / c);
+}
+
+// atan: for |x| > 1, pi/2 - atan(1/|x|); then with c = k/4 nearest to
+// t = min(|x|, 1/|x|), atan(t) = atan(c) + atan(u), u = (t - c) / (1 + t c),
+// |u| <= 0.125, and atan(u) by its Taylor series to u^17 (truncation below
+// 2^-52 relative).
+__host__ __device__ inline double atan_k4(int k) {
+  // atan(k / 4), k = 0..4
+  return k == 0 ? 0.0 : k == 1 ? 0.24497866312686414 : k == 2 ? 0.4636476090008061
+       : k == 3 ? 0.6435011087932844 : 0.7853981633974483;
+}
+
+__host__ __device__ inline double atan_series(double u) {
+  const double z = u * u;
+  double p = 1.0 / 17.0;
+  p = fma(p, -z, 1.0 / 15.0);
+  p = fma(p, -z, 1.0 / 13.0);
+  p = fma(p, -z, 1.0 / 11.0);
+  p = fma(p, -z, 1.0 / 9.0);
+  p = fma(p, -z, 1.0 / 7.0);
+  p = fma(p, -z, 1.0 / 5.0);
+  p = fma(p, -z, 1.0 / 3.0);
+  return fma(-u * z, p, u);
+}
+
+__host__ __device__ inline float atan_f32(float x) {
+  const double d = (double)x, a = fabs(d);
+  if (a < TINY || a != a) return x;
+  const bool inv = a > 1.0;
+  const double t = inv ? 1.0 / a : a;
+  const int k = (int)rint(t * 4.0);
+  const double c = 0.25 * k;
+  const double u = (t - c) / fma(t, c, 1.0);
+  double v = atan_k4(k) + atan_series(u);
+  if (inv) v = PIO2 - v;
+  return copysignf((float)v, x);
+}
+
+// asin(s) = s + s^3 P(s^2) for |s| <= 1/2, with P the degree-10 polynomial of
+// a Chebyshev fit (made for this file) to (asin(sqrt z) - sqrt z) / z^1.5 on
+// z in [0, 1/4]: asin's relative error below 2^-49.
+__host__ __device__ inline double asin_half(double s) {
+  const double z = s * s;
+  double p = 0x1.c8a4a8d5d7026p-6;
+  p = fma(p, z, -0x1.bf16e7c9f283cp-8);
+  p = fma(p, z, 0x1.fa1b2b4831188p-7);
+  p = fma(p, z, 0x1.512bc40e88a9ep-7);
+  p = fma(p, z, 0x1.cf5ed14c7cb7ep-7);
+  p = fma(p, z, 0x1.1c0d74beb3610p-6);
+  p = fma(p, z, 0x1.6e8f34a32a3ecp-6);
+  p = fma(p, z, 0x1.f1c6ff7f5507fp-6);
+  p = fma(p, z, 0x1.6db6dba99e56dp-5);
+  p = fma(p, z, 0x1.33333333030cfp-4);
+  p = fma(p, z, 0x1.55555555555bbp-3);
+  return fma(s * z, p, s);
+}
+
+// asin: |x| <= 1/2 directly; above, asin|x| = pi/2 - 2 asin(sqrt((1 - |x|) / 2)),
+// where 1 - |x| is exact in FP64 and the result is at least pi/6.  |x| > 1 and
+// NaN give NaN.
+__host__ __device__ inline float asin_f32(float x) {
+  const double d = (double)x, a = fabs(d);
+  if (a < TINY) return x;
+  if (a <= 0.5) return (float)asin_half(d);
+  if (!(a <= 1.0)) return NAN;
+  const double v = PIO2 - 2.0 * asin_half(sqrt(0.5 * (1.0 - a)));
+  return copysignf((float)v, x);
+}
+
+// acos: |x| <= 1/2: pi/2 - asin(x) (at least pi/3); x > 1/2:
+// 2 asin(sqrt((1 - x) / 2)); x < -1/2: pi - 2 asin(sqrt((1 + x) / 2)).
+__host__ __device__ inline float acos_f32(float x) {
+  const double d = (double)x, a = fabs(d);
+  if (a <= 0.5) return (float)(PIO2 - asin_half(d));
+  if (!(a <= 1.0)) return NAN;
+  const double h = 2.0 * asin_half(sqrt(0.5 * (1.0 - a)));
+  return (float)(d > 0.0 ? h : PI - h);
+}
+
+// 1 / sqrt(x) for positive finite x: FP64 sqrt and division, each correctly
+// rounded, then one rounding to FP32 (0 -> +inf, negative -> NaN, inf -> 0)
+__host__ __device__ inline float invsqrt_f32(float x) { return (float)(1.0 / sqrt((double)x)); }
 
 }  // namespace vkh

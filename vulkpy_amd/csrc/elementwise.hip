@@ -380,24 +380,24 @@ template <int FN>
 __device__ __forceinline__ float unop(float x) {
   if constexpr (FN == VKH_ABS) return fabsf(x);
   else if constexpr (FN == VKH_SIGN) return x > 0.f ? 1.f : (x < 0.f ? -1.f : (x == 0.f ? 0.f : x));
-  else if constexpr (FN == VKH_SIN) return (float)sin((double)x);
-  else if constexpr (FN == VKH_COS) return (float)cos((double)x);
-  else if constexpr (FN == VKH_TAN) return (float)tan((double)x);
-  else if constexpr (FN == VKH_ASIN) return (float)asin((double)x);
-  else if constexpr (FN == VKH_ACOS) return (float)acos((double)x);
-  else if constexpr (FN == VKH_ATAN) return (float)atan((double)x);
+  else if constexpr (FN == VKH_SIN) return sin_f32(x);
+  else if constexpr (FN == VKH_COS) return cos_f32(x);
+  else if constexpr (FN == VKH_TAN) return tan_f32(x);
+  else if constexpr (FN == VKH_ASIN) return asin_f32(x);
+  else if constexpr (FN == VKH_ACOS) return acos_f32(x);
+  else if constexpr (FN == VKH_ATAN) return atan_f32(x);
   else if constexpr (FN == VKH_SINH) return sinh_f32(x);
   else if constexpr (FN == VKH_COSH) return cosh_f32(x);
   else if constexpr (FN == VKH_TANH) return tanh_f32(x);
   else if constexpr (FN == VKH_ASINH) return asinh_f32(x);
   else if constexpr (FN == VKH_ACOSH) return acosh_f32(x);
   else if constexpr (FN == VKH_ATANH) return atanh_f32(x);
-  else if constexpr (FN == VKH_EXP) return (float)exp((double)x);
+  else if constexpr (FN == VKH_EXP) return exp_f32(x);
   else if constexpr (FN == VKH_LOG) return log_f32(x);
-  else if constexpr (FN == VKH_EXP2) return (float)exp2((double)x);
+  else if constexpr (FN == VKH_EXP2) return exp2_f32(x);
   else if constexpr (FN == VKH_LOG2) return log2_f32(x);
   else if constexpr (FN == VKH_SQRT) return sqrtf(x);
-  else return (float)(1.0 / sqrt((double)x));
+  else return invsqrt_f32(x);
 }
 
 template <int FN>

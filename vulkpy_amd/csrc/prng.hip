@@ -281,11 +281,11 @@ k_xoshiro_chunked(const uint4* init, uint4* state, const uint32_t* jumps, void* 
 // product recovered by an FMA.  On this range (no zero, denormal or infinite
 // input) these are the device logf's own steps, so the result is the same
 // without its range handling.
-constexpr float LN2_HI = 0x1.62e42ep-1f, LN2_LO = 0x1.efa39ep-25f;  // hi + lo = ln 2
+constexpr float LN2F_HI = 0x1.62e42ep-1f, LN2F_LO = 0x1.efa39ep-25f;  // hi + lo = ln 2
 __device__ __forceinline__ float log_unit(float t) {
   const float l2 = __builtin_amdgcn_logf(t);
-  const float p = l2 * LN2_HI;
-  const float e = fmaf(l2, LN2_LO, fmaf(l2, LN2_HI, -p));
+  const float p = l2 * LN2F_HI;
+  const float e = fmaf(l2, LN2F_LO, fmaf(l2, LN2F_HI, -p));
   return p + e;
 }
 
