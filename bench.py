@@ -24,7 +24,10 @@ After the timed steps every op's result is compared once with FP64 NumPy on
 a sample (its first 64 rows; every output of axis-0 and full reductions; the
 first 2^22 outputs of fresh generators against the oracle):
 "per_op_max_ulp" is the largest error in units of the FP32 spacing at the
-FP64 value (0 for the generators' bit-exact uint32 and float streams).
+FP64 value (0 for the generators' bit-exact uint32 and float streams); for
+sums and means the unit is 2^-24 times the sum of the magnitudes reduced
+(divided by their count for means), for products 2^-24 times the value, so
+a length-k FP32 recurrence is bounded by k.
 
 value = algorithmic bytes of all ops / wall time of the timed steps (GB/s,
 inputs resident in HBM).  The dominant kernel (broadcast A + col) is also
@@ -39,11 +42,14 @@ Two more sections of the same JSON line cover the rest of the metric:
     Adam), plain FP32 step against the BF16 step captured as one hipGraph:
     steps/s of each, the speed-up and the final losses after the same number
     of steps, and the matrix-core utilisation of each product of the step;
+    its "seed_study" repeats the FP32 and BF16 runs over R weight and data
+    seeds (plus BF16 arms with one product family in split-BF16) and reports
+    the losses' means and standard deviations;
   * "gemm": 8192^3 and 65536 x 1024 x 1024 products in FP32, BF16 and
     split-BF16, TFLOP/s and error against an FP64 product.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-                [--mlp-steps S] [--no-gemm] [--no-c2]
+                [--mlp-steps S] [--mlp-seeds R] [--no-gemm] [--no-c2]
                 [--pmc-kernels | --gemm-pmc [fp32|bf16|bf16x3]]   (counter runs only)
                 [--mlp-trace S]   (kernel traces of the MLP steps only)
 """
@@ -229,20 +235,22 @@ def build_suite(vk, gpu, ops_in, rng_gen, ubuf, fbuf, timer):
     full_cache = {}
 
     def full_f64(X):
-        """FP64 column sums, log-products, maxima and minima of a whole operand
-        (read in row blocks; kept for the operand's other reductions)"""
+        """FP64 column sums, sums of magnitudes, log-products, maxima and
+        minima of a whole operand (read in row blocks; kept for the operand's
+        other reductions)"""
         if id(X) in full_cache:
             return full_cache[id(X)]
-        cs, pr = np.zeros(N_COLS), np.zeros(N_COLS)
+        cs, ca, pr = np.zeros(N_COLS), np.zeros(N_COLS), np.zeros(N_COLS)
         mx, mn = np.full(N_COLS, -np.inf), np.full(N_COLS, np.inf)
         step_rows = 512
         for r0 in range(0, M_ROWS, step_rows):
             blk = head_rows(X, r0, step_rows)
             cs += blk.sum(axis=0)
+            ca += np.abs(blk).sum(axis=0)
             pr += np.log(np.abs(blk)).sum(axis=0)
             mx = np.maximum(mx, blk.max(axis=0))
             mn = np.minimum(mn, blk.min(axis=0))
-        full_cache[id(X)] = (cs, pr, mx, mn)
+        full_cache[id(X)] = (cs, ca, pr, mx, mn)
         return full_cache[id(X)]
 
     def head_rows(X, r0, nr):
@@ -251,20 +259,27 @@ def build_suite(vk, gpu, ops_in, rng_gen, ubuf, fbuf, timer):
         return out.astype(np.float64).reshape(nr, N_COLS)
 
     def red_check(X, f, axis):
+        """(result, FP64 value, scale): sums and means are measured against
+        2^-24 times the sum of magnitudes of the reduced elements (mean: over
+        their count), products against 2^-24 times the value, so a length-k
+        FP32 recurrence stays within k; maxima and minima in ulps"""
         def check():
             got = np.asarray(getattr(X, f)(axis=axis), dtype=np.float64).reshape(-1)
             if axis == 1:
                 r = rows(X)
                 exact = {"sum": r.sum(1), "maximum": r.max(1), "minimum": r.min(1), "mean": r.mean(1),
                          "prod": np.exp(np.log(r).sum(1))}[f]
-                return got[:S_ROWS], exact
-            cs, lp, mx, mn = full_f64(X)
-            exact0 = {"sum": cs, "maximum": mx, "minimum": mn, "mean": cs / M_ROWS, "prod": np.exp(lp)}[f]
+                mag = {"sum": np.abs(r).sum(1), "mean": np.abs(r).mean(1), "prod": np.abs(exact)}.get(f)
+                return got[:S_ROWS], exact, mag
+            cs, ca, lp, mx, mn = full_f64(X)
             if axis == 0:
-                return got, exact0
-            exact = {"sum": cs.sum(), "maximum": mx.max(), "minimum": mn.min(), "mean": cs.sum() / N,
-                     "prod": np.exp(lp.sum())}[f]
-            return got, np.array([exact])
+                exact = {"sum": cs, "maximum": mx, "minimum": mn, "mean": cs / M_ROWS, "prod": np.exp(lp)}[f]
+                mag = {"sum": ca, "mean": ca / M_ROWS, "prod": np.abs(exact)}.get(f)
+                return got, exact, mag
+            exact = np.array([{"sum": cs.sum(), "maximum": mx.max(), "minimum": mn.min(), "mean": cs.sum() / N,
+                               "prod": np.exp(lp.sum())}[f]])
+            mag = {"sum": np.array([ca.sum()]), "mean": np.array([ca.sum() / N]), "prod": np.abs(exact)}.get(f)
+            return got, exact, mag
         return check
 
     for f in ("sum", "maximum", "minimum", "mean", "prod"):
@@ -509,6 +524,92 @@ def mlp_compare(vk, gpu, steps: int, warmup: int = 2, uncaptured: bool = False, 
         out[name]["rel_diff_at_step"] = {
             k: round(abs(v - ref["loss_at_step"][k]) / abs(ref["loss_at_step"][k]), 5)
             for k, v in r["loss_at_step"].items()}
+        out[name]["meets_final_loss_tol"] = out[name]["final_loss_rel_diff"] <= 0.01
+    return out
+
+
+SEED_ARMS = (
+    ("fp32", None),
+    ("bf16", "bf16"),
+    ("bf16_wgrad_split", {"forward": "bf16", "input_grad": "bf16", "weight_grad": "bf16x3"}),
+    ("bf16_dgrad_split", {"forward": "bf16", "input_grad": "bf16x3", "weight_grad": "bf16"}),
+    ("bf16_fwd_split", {"forward": "bf16x3", "input_grad": "bf16", "weight_grad": "bf16"}),
+)
+
+
+def mlp_seed_study(vk, gpu, seeds, steps: int = 500) -> dict:
+    """
+    The regression MLP of mlp_compare over several weight and data seeds:
+    for each seed, fresh inputs, targets, held-out batch and HeNormal weights,
+    trained `steps` steps by the plain FP32 step (nn.Sequence) and by captured
+    steps (nn.GraphSequence) whose products are BF16, or BF16 with one product
+    family (weight gradients, input gradients or forward) in split-BF16.
+    Reports each arm's final training loss, held-out loss and loss at step
+    100 per seed, their mean and standard deviation over the seeds, and
+    whether each BF16 arm's mean lies within one FP32 standard deviation of
+    the FP32 mean and within 1% of it.
+    """
+    from vulkpy_amd import nn
+    B, D, H = 16384, 256, 4096
+    dims = [(D, H), (H, H), (H, 1)]
+    per = {name: {"final": [], "heldout": [], "step100": []} for name, _ in SEED_ARMS}
+    for seed in seeds:
+        rng = np.random.default_rng(1000 + seed)
+        x = rng.standard_normal((B, D), dtype=np.float32)
+        proj = (rng.standard_normal((D, 1)) / np.sqrt(D)).astype(np.float32)
+        y = (np.sin(x @ proj) + 0.1 * rng.standard_normal((B, 1))).astype(np.float32)
+        xt = rng.standard_normal((B, D), dtype=np.float32)
+        yt = (np.sin(xt @ proj) + 0.1 * rng.standard_normal((B, 1))).astype(np.float32)
+        X, Y, Xt, Yt = (vk.Array(gpu, data=v) for v in (x, y, xt, yt))
+
+        def layers(precision):
+            out = []
+            for k, (i, o) in enumerate(dims):
+                out.append(nn.Dense(gpu, i, o, w_init=nn.HeNormal(gpu, i, seed=10 * seed + k),
+                                    w_opt=nn.Adam(gpu, lr=1e-3), b_opt=nn.Adam(gpu, lr=1e-3),
+                                    w_reg=nn.Ridge(1e-4), precision=precision))
+                if k + 1 < len(dims):
+                    out.append(nn.ReLU())
+            return out
+
+        for name, prec in SEED_ARMS:
+            if prec is None:
+                model = nn.Sequence(layers("fp32"), nn.HuberLoss(), regularize=True)
+            else:
+                model = nn.GraphSequence(layers("bf16"), nn.HuberLoss(), precision=prec,
+                                         regularize=True, autotune=True)
+            l100 = None
+            for s in range(steps):
+                _, L = model.train(X, Y)
+                if s + 1 == 100:
+                    l100 = float(np.asarray(L))
+            final = float(np.asarray(L))
+            for Lr in model.L:
+                if isinstance(Lr, nn.Dense):
+                    Lr.precision = "fp32"
+            h = Xt
+            for Lr in model.L:
+                h = Lr(h)
+            held = float(np.asarray(nn.HuberLoss()(h, Yt)))
+            per[name]["final"].append(final)
+            per[name]["heldout"].append(held)
+            per[name]["step100"].append(l100)
+            log(f"bench: MLP seed {seed} {name}: final {final:.6g} heldout {held:.6g} step100 {l100:.6g}")
+            del model
+            gpu.wait()
+    out = {"seeds": list(seeds), "steps": steps}
+    ref = {k: (float(np.mean(v)), float(np.std(v, ddof=1))) for k, v in per["fp32"].items()}
+    for name, _ in SEED_ARMS:
+        arm = {}
+        for k, v in per[name].items():
+            mean, std = float(np.mean(v)), float(np.std(v, ddof=1))
+            arm[k] = {"per_seed": [round(t, 8) for t in v], "mean": round(mean, 8), "std": round(std, 8)}
+            if name != "fp32":
+                rm, rs = ref[k]
+                arm[k]["mean_rel_diff"] = round(abs(mean - rm) / rm, 5)
+                arm[k]["within_fp32_std"] = abs(mean - rm) <= rs
+                arm[k]["within_1pct"] = abs(mean - rm) <= 0.01 * rm
+        out[name] = arm
     return out
 
 
@@ -696,6 +797,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--mlp-steps", type=int, default=500,
                     help="training steps per arm of the MLP comparison (0 skips it)")
+    ap.add_argument("--mlp-seeds", type=int, default=5,
+                    help="weight and data seeds of the MLP seed study (0 skips it)")
     ap.add_argument("--no-gemm", action="store_true", help="skip the GEMM precision report")
     ap.add_argument("--no-c2", action="store_true",
                     help="skip the bandwidth suite (and the CPU baseline that samples it)")
@@ -743,6 +846,9 @@ def main() -> int:
     mlp = None
     if args.mlp_steps > 0:
         mlp = mlp_compare(vk, gpu, args.mlp_steps, timer=timer)
+        if args.mlp_seeds > 0:
+            log(f"bench: MLP seed study ({args.mlp_seeds} seeds)")
+            mlp["seed_study"] = mlp_seed_study(vk, gpu, range(1, args.mlp_seeds + 1), args.mlp_steps)
     gemm = None
     if not args.no_gemm:
         log("bench: GEMM report")
@@ -921,8 +1027,12 @@ def c2_suite(vk, gpu, args, timer):
     per_op_ulp = {}
     for name, _, _, check in suite:
         if callable(check):
-            got, exact = check()
-            per_op_ulp[name] = round(max_ulp(got, exact), 3)
+            res = check()
+            if len(res) == 3 and res[2] is not None:
+                got, exact, mag = res
+                per_op_ulp[name] = round(float((np.abs(got - exact) / (mag * 2.0 ** -24)).max()), 3)
+            else:
+                per_op_ulp[name] = round(max_ulp(res[0], res[1]), 3)
     gpu.wait()
     timer.mean_ms()
     del suite, ubuf, fbuf, gen, A, B, P, Q, D, E, ops_in  # free the suite's device buffers

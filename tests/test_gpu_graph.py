@@ -248,3 +248,30 @@ def test_graph_autotuned_matches_eager(gpu):
         # reduction choices stay on the context: restore the defaults
         for d in (1, 4, 256):
             _lib.call("vkh_reduce_set_config", gpu._handle, 1, 1, B, d, -1)
+
+
+def test_graph_precision_per_product_family(gpu):
+    """precision given per product family: all three "bf16" is the same step
+    as precision="bf16"; a mix (split-BF16 weight gradients) captured equals
+    the same kernels launched one at a time, and trains."""
+    dims, acts = [(32, 128), (128, 128), (128, 1)], ["relu", "relu", None]
+    rng = np.random.default_rng(17)
+    x = rng.standard_normal((256, 32)).astype(np.float32)
+    y = rng.standard_normal((256, 1)).astype(np.float32)
+    X, Y = vk.Array(gpu, data=x), vk.Array(gpu, data=y)
+    same = {"forward": "bf16", "input_grad": "bf16", "weight_grad": "bf16"}
+    mixed = {"forward": "bf16", "input_grad": "bf16", "weight_grad": "bf16x3"}
+    nets = [nn.GraphSequence(make_net(gpu, dims, acts, 40, lambda: nn.Ridge(1e-4)), nn.HuberLoss(),
+                             precision=p, capture=c)
+            for p, c in (("bf16", True), (same, True), (mixed, True), (mixed, False))]
+    losses = []
+    for _ in range(4):
+        ls = [np.asarray(n.train(X, Y)[1]).item() for n in nets]
+        assert ls[0] == ls[1] and ls[2] == ls[3], ls
+        losses.append(ls[2])
+    for a, b in zip(weights(nets[0].L), weights(nets[1].L)):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(weights(nets[2].L), weights(nets[3].L)):
+        np.testing.assert_array_equal(a, b)
+    with pytest.raises(ValueError):
+        nn.GraphSequence(make_net(gpu, dims, acts, 40), nn.HuberLoss(), precision={"backward": "bf16"})

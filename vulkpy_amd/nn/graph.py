@@ -93,7 +93,9 @@ class _Plan:
         self.gpu = gpu
         h = gpu._handle
         self.h = h
-        self.mode = GEMM_MODE[seq.precision]
+        # GEMM mode per product family: forward x W^T, input gradient dY W,
+        # weight gradient dY^T X
+        self.mode_fwd, self.mode_dx, self.mode_dw = (GEMM_MODE[p] for p in _family_precisions(seq.precision))
         units = _units(seq.L)
         self.units = units
         B = int(x_shape[0])
@@ -144,16 +146,19 @@ class _Plan:
         # the matrix cores, whose epilogue writes C^T coalesced
         self.hs_bft: List[Optional[_DeviceBuffer]] = [None] * len(units)
         self.ds_bft: List[Optional[_DeviceBuffer]] = [None] * len(units)
-        if self.mode == GEMM_MODE["bf16"]:
-            rows = -(-B // 256) * 256
+        bf16 = GEMM_MODE["bf16"]
+        rows = -(-B // 256) * 256
+        if self.mode_fwd == bf16:
             for i in range(len(units) - 1):   # consumer: forward product of unit i + 1
                 k = units[i].dense.output_dim
                 if _packs_a(B, units[i + 1].dense.output_dim, k):
                     self.hs_bf[i] = _DeviceBuffer(gpu, rows * k, np.uint16)
+        if self.mode_dx == bf16:
             for i in range(1, len(units) - 1):  # consumer: dX product of unit i
                 k = units[i].dense.output_dim
                 if _packs_a(B, units[i].dense.input_dim, k):
                     self.ds_bf[i] = _DeviceBuffer(gpu, rows * k, np.uint16)
+        if self.mode_dw == bf16:
             for i in range(len(units) - 1):
                 d, nxt = units[i].dense, units[i + 1].dense
                 trows = -(-d.output_dim // 256) * 256
@@ -203,14 +208,14 @@ class _Plan:
 
     # ---- the step -------------------------------------------------------------------
     def _enqueue(self):
-        h, mode, B = self.h, self.mode, self.B
+        h, B = self.h, self.B
         units = self.units
         # forward
         src, src_bf = self.x, None
         for i, (u, hout) in enumerate(zip(units, self.hs)):
             d = u.dense
             hb, hbt = self.hs_bf[i], self.hs_bft[i]
-            call("vkh_gemm_bf16io", h, mode, 0, 1, B, d.output_dim, d.input_dim, 1.0, src._ptr,
+            call("vkh_gemm_bf16io", h, self.mode_fwd, 0, 1, B, d.output_dim, d.input_dim, 1.0, src._ptr,
                  d.input_dim, d.w.value._ptr, d.input_dim, 0.0, hout._ptr, d.output_dim, d.b.value._ptr,
                  ACT[u.act], hb.ptr if hb else None, None, 0, 0, src_bf.ptr if src_bf else None,
                  None, hbt.ptr if hbt else None, B)
@@ -248,7 +253,7 @@ class _Plan:
                 beta = 0.0
                 if is_w:
                     dyt, xt = self.ds_bft[i], (self.hs_bft[i - 1] if i > 0 else None)
-                    call("vkh_gemm_bf16io", h, mode, 1, 0, d.output_dim, d.input_dim, B, 1.0, dy._ptr,
+                    call("vkh_gemm_bf16io", h, self.mode_dw, 1, 0, d.output_dim, d.input_dim, B, 1.0, dy._ptr,
                          d.output_dim, xin._ptr, d.input_dim, beta, p.grad._ptr, d.input_dim,
                          None, 0, None, None, 0, 0, dyt.ptr if dyt else None, xt.ptr if xt else None,
                          None, 0)
@@ -257,7 +262,7 @@ class _Plan:
             if i > 0:
                 prev = units[i - 1]
                 out_bf, dy_bf, out_bft = self.ds_bf[i - 1], self.ds_bf[i], self.ds_bft[i - 1]
-                call("vkh_gemm_bf16io", h, mode, 0, 0, B, d.input_dim, d.output_dim, 1.0, dy._ptr,
+                call("vkh_gemm_bf16io", h, self.mode_dx, 0, 0, B, d.input_dim, d.output_dim, 1.0, dy._ptr,
                      d.output_dim, d.w.value._ptr, d.input_dim, 0.0, self.ds[i - 1]._ptr,
                      d.input_dim, None, 0, out_bf.ptr if out_bf else None,
                      self.hs[i - 1]._ptr if prev.act else None, d.input_dim, ACT[prev.act],
@@ -282,14 +287,14 @@ class _Plan:
                      float(st.opt.lr), float(st.opt.eps), kind, l1, s2)
 
     def _gemm_shapes(self):
-        """(trans_a, trans_b, M, N, K) of every product _enqueue issues"""
+        """(mode, trans_a, trans_b, M, N, K) of every product _enqueue issues"""
         B = self.B
         for i, u in enumerate(self.units):
             d = u.dense
-            yield 0, 1, B, d.output_dim, d.input_dim      # forward x W^T
-            yield 1, 0, d.output_dim, d.input_dim, B      # dW = dY^T X
+            yield self.mode_fwd, 0, 1, B, d.output_dim, d.input_dim      # forward x W^T
+            yield self.mode_dw, 1, 0, d.output_dim, d.input_dim, B       # dW = dY^T X
             if i > 0:
-                yield 0, 0, B, d.input_dim, d.output_dim  # dX = dY W
+                yield self.mode_dx, 0, 0, B, d.input_dim, d.output_dim   # dX = dY W
 
     def _reduce_shapes(self):
         """(prev, axis, post) of every axis reduction _enqueue issues"""
@@ -301,10 +306,10 @@ class _Plan:
     def _capture(self):
         # kernel configuration (when tuning) and the graph's GEMM packing
         # region for every product of the step
-        for ta, tb, M, N, K in self._gemm_shapes():
+        for mode, ta, tb, M, N, K in self._gemm_shapes():
             if self.seq.autotune:
-                call("vkh_gemm_autotune", self.h, self.mode, ta, tb, M, N, K, 3, None, None)
-            call("vkh_gemm_reserve", self.h, self.mode, M, N, K)
+                call("vkh_gemm_autotune", self.h, mode, ta, tb, M, N, K, 3, None, None)
+            call("vkh_gemm_reserve", self.h, mode, M, N, K)
         if self.seq.autotune:
             for prev, K, post in self._reduce_shapes():
                 # only configurations with the default summation order, so
@@ -370,6 +375,25 @@ class _Plan:
             pass
 
 
+_FAMILIES = ("forward", "input_grad", "weight_grad")
+
+
+def _family_precisions(precision) -> Tuple[str, str, str]:
+    """GEMM precision of the forward, input-gradient and weight-gradient
+    products from ``precision`` (one name for all three, or a dict by family)"""
+    if isinstance(precision, dict):
+        unknown = set(precision) - set(_FAMILIES)
+        if unknown:
+            raise ValueError(f"unknown product families {sorted(unknown)}")
+        out = tuple(precision.get(f, "bf16") for f in _FAMILIES)
+    else:
+        out = (precision,) * 3
+    for p in out:
+        if p not in GEMM_MODE:
+            raise ValueError(f"unknown precision {p!r}")
+    return out
+
+
 class _DeviceDoubles:
     """Small FP64 device buffer (Adam beta powers)."""
 
@@ -395,9 +419,11 @@ class GraphSequence:
         Dense layers, each optionally followed by ReLU or Sigmoid.
     loss : vulkpy_amd.nn.Loss
         SoftmaxCrossEntropyLoss, CrossEntropyLoss, MSELoss or HuberLoss.
-    precision : {"bf16", "fp32", "bf16x3"}
+    precision : {"bf16", "fp32", "bf16x3"} or dict
         Product precision of the GEMMs (FP32 accumulation and FP32 master
-        weights in every mode).
+        weights in every mode); a dict sets it per product family, keys
+        ``"forward"``, ``"input_grad"`` and ``"weight_grad"`` (missing keys:
+        ``"bf16"``).
     regularize : bool
         Add each parameter's regularizer gradient before the update
         (``Parameter.regular_grad``).  The reference's ``Sequence.train``
@@ -425,8 +451,7 @@ class GraphSequence:
     def __init__(self, layers: Iterable[Module], loss: Loss, *,
                  precision: str = "bf16", regularize: bool = True, autotune: bool = False,
                  capture: bool = True):
-        if precision not in GEMM_MODE:
-            raise ValueError(f"unknown precision {precision!r}")
+        _family_precisions(precision)  # validates
         self.L: Tuple[Module, ...] = tuple(layers)
         self.loss = loss
         self.precision = precision
