@@ -52,6 +52,7 @@ python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
                 [--mlp-steps S] [--mlp-seeds R] [--no-gemm] [--no-c2]
                 [--pmc-kernels | --gemm-pmc [fp32|bf16|bf16x3]]   (counter runs only)
                 [--mlp-trace S]   (kernel traces of the MLP steps only)
+                [--mlp-trace-graphs S]   (the same as graph replays)
 """
 from __future__ import annotations
 
@@ -391,7 +392,8 @@ def mlp_gemm_utilisation(vk, gpu, timer, B: int, dims, reps: int = 3) -> dict:
     return out
 
 
-def mlp_compare(vk, gpu, steps: int, warmup: int = 2, uncaptured: bool = False, timer=None):
+def mlp_compare(vk, gpu, steps: int, warmup: int = 2, uncaptured: bool = False, timer=None,
+                graphs_only: bool = False):
     """
     Regression MLP Dense(256->4096)-ReLU-Dense(4096->4096)-ReLU-Dense(4096->1),
     HuberLoss(delta=1), Ridge(1e-4) on every weight (applied, see
@@ -407,7 +409,8 @@ def mlp_compare(vk, gpu, steps: int, warmup: int = 2, uncaptured: bool = False, 
     their first step).  Besides the final training loss, each arm's trained
     weights are evaluated in FP32 on a held-out batch of the same
     distribution.  With `uncaptured` only the BF16 and split-BF16 steps run,
-    their kernels launched one at a time instead of as a graph replay.
+    their kernels launched one at a time instead of as a graph replay; with
+    `graphs_only` only those two steps run, as graph replays.
     """
     from vulkpy_amd import nn
     B, D, H = 16384, 256, 4096
@@ -478,7 +481,9 @@ def mlp_compare(vk, gpu, steps: int, warmup: int = 2, uncaptured: bool = False, 
             ("bf16x3_graph", lambda: nn.GraphSequence(layers("bf16x3"), nn.HuberLoss(),
                                                       precision="bf16x3", regularize=True,
                                                       autotune=True), None))
-    if uncaptured:
+    if graphs_only:
+        arms = arms[2:]
+    elif uncaptured:
         # the graph arms' kernels launched one at a time (for kernel traces)
         arms = tuple((name.replace("graph", "uncaptured"),
                       lambda p=name.split("_")[0]: nn.GraphSequence(
@@ -507,7 +512,7 @@ def mlp_compare(vk, gpu, steps: int, warmup: int = 2, uncaptured: bool = False, 
         "timed_steps": steps - warmup,
         "gemm_flop_per_step": 2 * (B * H * D + B * H * H + B * H) * 2 + 2 * (B * H * H + B * H),
     }
-    if uncaptured:
+    if uncaptured or graphs_only:
         out.update(res)
         return out
     if timer is not None:
@@ -812,6 +817,9 @@ def main() -> int:
                     help="only run S steps of the BF16 and split-BF16 MLP steps with their "
                          "kernels launched one at a time, not as graph replays (for kernel "
                          "traces), and exit")
+    ap.add_argument("--mlp-trace-graphs", type=int, default=0, metavar="S",
+                    help="only run S steps of the BF16 and split-BF16 MLP steps as graph "
+                         "replays (for kernel traces), and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -836,6 +844,10 @@ def main() -> int:
         return 0
     if args.mlp_trace > 0:
         print(json.dumps({"mlp_trace": mlp_compare(vk, gpu, args.mlp_trace, uncaptured=True)}))
+        return 0
+    if args.mlp_trace_graphs > 0:
+        print(json.dumps({"mlp_trace_graphs": mlp_compare(vk, gpu, args.mlp_trace_graphs,
+                                                          graphs_only=True)}))
         return 0
 
     c2 = None

@@ -145,7 +145,9 @@ int vkh_fill_u32(vkh_gpu* gpu, uint32_t* dst, uint32_t value, size_t n);
 
 /* ---- jobs ---------------------------------------------------------------- */
 /* replaces Job (_vkarray.cc:392-457): an event recorded after the work
- * enqueued so far on the stream */
+ * enqueued so far on the stream.  VKH_INVALID during a graph capture (the
+ * record would become a graph node that outlives the event); likewise
+ * vkh_timer_record. */
 int vkh_event_record(vkh_gpu* gpu, vkh_event** out);
 /* replaces Job::wait(timeout_ns) (_vkarray.cc:446-456); UINT64_MAX waits
  * forever; returns VKH_TIMEOUT with "Timeout at Command Wait" */
@@ -279,6 +281,7 @@ int vkh_onehot(vkh_gpu* gpu, const uint32_t* idx, uint64_t n, uint32_t classes,
  * from seed with the splitmix64 chain and the jump exactly as written there */
 int vkh_xoshiro_create(vkh_gpu* gpu, uint32_t size, uint64_t seed,
                        vkh_xoshiro** out);
+/* waits for the stream: VKH_INVALID during a graph capture */
 int vkh_xoshiro_destroy(vkh_xoshiro* rng);
 int vkh_xoshiro_size(vkh_xoshiro* rng, uint32_t* out);
 /* device pointer of the [size][4] uint32 state (for tests) */
@@ -494,6 +497,26 @@ int vkh_regularizer_grad(vkh_gpu* gpu, int kind, const float* w, float* grad,
  * Dense bias gradient (nn/layers.py:126-127) */
 int vkh_bias_grad(vkh_gpu* gpu, const float* dy, float* db, uint64_t rows,
                   uint64_t cols, float beta);
+/* Backward of a last Dense layer with one output (w: [1, N]) that follows an
+ * activation h = act(z) [B, N], in one pass over h, with the results of the
+ * separate steps bit for bit (nn/layers.py:126-141 for the layer, 207-210 /
+ * 264-267 for the activation, and the previous layer's bias gradient):
+ *   dx = act'(h) * (dy w)        as vkh_gemm (K = 1) with aux = h
+ *   dx_bf / dx_bft (may be NULL) its BF16 copies, row-major [B][N] and
+ *                  transposed [N][ldt] (ldt % 8 == 0, 16-byte aligned)
+ *   dw = dy^T h                  as vkh_gemm (1 x N x B), written (beta 0)
+ *   db = column sums of dx       as vkh_bias_grad with beta 0
+ * dw or db NULL: not computed.  No counterpart in the reference.
+ * VKH_INVALID unless vkh_dense1_backward_supported(B, N) and the operands
+ * are 16-byte aligned (the caller then takes the separate steps). */
+int vkh_dense1_backward(vkh_gpu* gpu, const float* dy, const float* w,
+                        const float* h, int act, uint64_t B, uint64_t N,
+                        float* dx, uint16_t* dx_bf, uint16_t* dx_bft,
+                        uint64_t ldt, float* dw, float* db);
+/* *ok = 1 when vkh_dense1_backward takes a [B, N] layer on this device:
+ * B % 128 == 0, N % 32 == 0 and the column-sum chunk lengths of the
+ * separate steps divide 128 rows. */
+int vkh_dense1_backward_supported(vkh_gpu* gpu, uint64_t B, uint64_t N, int* ok);
 /* dst[i] = BF16(src[i]) (round to nearest even) */
 int vkh_to_bf16(vkh_gpu* gpu, const float* src, uint16_t* dst, uint64_t n);
 
@@ -503,6 +526,7 @@ int vkh_to_bf16(vkh_gpu* gpu, const float* src, uint16_t* dst, uint64_t n);
 int vkh_graph_begin(vkh_gpu* gpu);
 int vkh_graph_end(vkh_gpu* gpu, void** graph_exec);
 int vkh_graph_launch(vkh_gpu* gpu, void* graph_exec);
+/* waits for the stream: VKH_INVALID while the context captures a graph */
 int vkh_graph_destroy(void* graph_exec);
 
 #ifdef __cplusplus

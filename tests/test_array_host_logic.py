@@ -86,3 +86,36 @@ def test_result_of_an_op_has_no_pending_upload(fake):
     del y
     gc.collect()
     assert fake.log == []
+
+
+def test_delete_during_capture_waits_after_it(fake):
+    """A finalizer that runs while a graph is captured (garbage collection at
+    any allocation) must not wait on the capturing stream: the wait is queued
+    and runs, mirror still held, when the capture ends."""
+    x = make(fake)
+    host = x._host
+    fake.log.clear()
+    vkarray._capture_begin()
+    try:
+        del x
+        gc.collect()
+        assert fake.log == []                 # nothing waited for inside the capture
+        assert len(vkarray._deferred) == 1
+    finally:
+        vkarray._capture_end()
+    assert fake.log == [("wait", 1)]
+    assert vkarray._deferred == []
+    assert host is not None
+
+
+def test_capture_end_runs_queued_work_once_outermost():
+    ran = []
+    vkarray._capture_begin()
+    vkarray._capture_begin()
+    assert vkarray._after_capture(lambda: ran.append(1))
+    vkarray._capture_end()
+    assert ran == []                          # an outer capture still runs
+    vkarray._capture_end()
+    assert ran == [1]
+    assert not vkarray._after_capture(lambda: ran.append(2))
+    assert ran == [1]

@@ -3,6 +3,7 @@ Sequence.train path: identical weights in FP32 after every replay (with eager
 work and synchronisations between replays), and a BF16 run that tracks the
 FP32 loss."""
 import ctypes
+import gc
 
 import numpy as np
 import pytest
@@ -202,6 +203,110 @@ def test_graph_uncaptured_matches_captured(gpu, precision):
             assert p.hs_bft[0] is not None and p.ds_bft[0] is not None
         assert np.asarray(lc).item() == np.asarray(lu).item(), step
         for a, b in zip(weights(cap_layers), weights(unc_layers)):
+            np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision,hidden_act", [("fp32", "relu"), ("fp32", "sigmoid"), ("fp32", None),
+                                                  ("bf16", "relu"), ("bf16x3", "sigmoid")])
+def test_graph_fused_last_layer_matches_eager(gpu, precision, hidden_act):
+    """A one-output last layer at batch 1024: its input gradient (and BF16
+    copies), weight gradient and the hidden layer's bias gradient come from
+    one kernel (vkh_dense1_backward) with the eager step's bits."""
+    dims, acts = [(32, 256), (256, 192), (192, 1)], ["relu", hidden_act, None]
+    B = 1024
+    rng = np.random.default_rng(23)
+    x = rng.standard_normal((B, 32)).astype(np.float32)
+    y = rng.standard_normal((B, 1)).astype(np.float32)
+    X, Y = vk.Array(gpu, data=x), vk.Array(gpu, data=y)
+    eager_layers = make_net(gpu, dims, acts, 50, lambda: nn.Ridge(1e-4), precision=precision)
+    graph_layers = make_net(gpu, dims, acts, 50, lambda: nn.Ridge(1e-4), precision=precision)
+    eager = nn.Sequence(eager_layers, nn.HuberLoss(), regularize=True)
+    graph = nn.GraphSequence(graph_layers, nn.HuberLoss(), precision=precision, regularize=True)
+    for step in range(3):
+        _, le = eager.train(X, Y)
+        _, lg = graph.train(X, Y)
+        gpu.wait()
+        p = graph._plan
+        assert p.fuse_last
+        if precision == "bf16":
+            assert p.ds_bf[1] is not None and p.ds_bft[1] is not None
+        assert np.asarray(le).item() == np.asarray(lg).item(), step
+        for a, b in zip(weights(eager_layers), weights(graph_layers)):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_dense1_backward_matches_separate_steps(gpu):
+    """vkh_dense1_backward against vkh_gemm (input and weight gradients) and
+    vkh_bias_grad, bit for bit, BF16 copies included, on a 512 x 96 layer."""
+    B, N = 512, 96
+    h = gpu._handle
+    rng = np.random.default_rng(29)
+    dy = vk.Array(gpu, data=rng.standard_normal((B, 1)).astype(np.float32))
+    w = vk.Array(gpu, data=rng.standard_normal((1, N)).astype(np.float32))
+    hid = vk.Array(gpu, data=np.maximum(rng.standard_normal((B, N)), 0).astype(np.float32))
+    ok = ctypes.c_int(0)
+    _lib.call("vkh_dense1_backward_supported", h, B, N, ctypes.byref(ok))
+    assert ok.value == 1
+    dx, dw, db = (vk.Array(gpu, shape=s) for s in ((B, N), (1, N), (N,)))
+    dxbf = vk.U32Array(gpu, shape=(B * N // 2,))
+    dxbft = vk.U32Array(gpu, shape=(B * N // 2,))
+    _lib.call("vkh_dense1_backward", h, dy._ptr, w._ptr, hid._ptr, 1, B, N, dx._ptr, dxbf._ptr,
+              dxbft._ptr, B, dw._ptr, db._ptr)
+    rx, rw, rb = (vk.Array(gpu, shape=s) for s in ((B, N), (1, N), (N,)))
+    _lib.call("vkh_gemm_ex", h, 0, 0, 0, B, N, 1, 1.0, dy._ptr, 1, w._ptr, N, 0.0, rx._ptr, N,
+              None, 0, None, hid._ptr, N, 1)
+    _lib.call("vkh_gemm", h, 0, 1, 0, 1, N, B, 1.0, dy._ptr, 1, hid._ptr, N, 0.0, rw._ptr, N,
+              None, 0, None)
+    _lib.call("vkh_bias_grad", h, rx._ptr, rb._ptr, B, N, 0.0)
+    gpu.wait()
+    got_x = np.asarray(dx)
+    np.testing.assert_array_equal(got_x.view(np.uint32), np.asarray(rx).view(np.uint32))
+    np.testing.assert_array_equal(np.asarray(dw).view(np.uint32), np.asarray(rw).view(np.uint32))
+    np.testing.assert_array_equal(np.asarray(db).view(np.uint32), np.asarray(rb).view(np.uint32))
+    # BF16 copies: round to nearest even of the FP32 values
+    u = got_x.view(np.uint32).astype(np.uint64)
+    bf = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    np.testing.assert_array_equal(np.asarray(dxbf).view(np.uint16).reshape(B, N), bf)
+    np.testing.assert_array_equal(np.asarray(dxbft).view(np.uint16).reshape(N, B), bf.T)
+
+
+def test_graph_capture_survives_finalizers(gpu, monkeypatch):
+    """Objects collected while the step is captured (an array whose upload
+    may still be queued, with a pinned mirror, and a generator) release
+    their resources after the capture; the graph stays valid and matches
+    the eager step."""
+    from vulkpy_amd.nn import graph as graph_mod
+    big = vk.Array(gpu, data=np.ones(1 << 22, dtype=np.float32))  # pinned mirror, chunked upload
+    assert big._upload_pending
+    doomed = [big, vk.random.Xoshiro128pp(gpu, seed=3)]
+    del big
+    orig = graph_mod._Plan._enqueue
+
+    def enqueue(self):
+        if doomed and vk.vkarray._captures:
+            doomed.clear()
+            gc.collect()                      # their finalizers run inside the capture
+            assert vk.vkarray._deferred
+        orig(self)
+
+    monkeypatch.setattr(graph_mod._Plan, "_enqueue", enqueue)
+    dims, acts = [(12, 40), (40, 4)], ["relu", None]
+    B = 96
+    rng = np.random.default_rng(31)
+    X = vk.Array(gpu, data=rng.standard_normal((B, 12)).astype(np.float32))
+    Y = vk.Array(gpu, data=np.identity(4, dtype=np.float32)[rng.integers(0, 4, B)])
+    eager_layers = make_net(gpu, dims, acts, 60, opt="adagrad")
+    graph_layers = make_net(gpu, dims, acts, 60, opt="adagrad")
+    eager = nn.Sequence(eager_layers, nn.SoftmaxCrossEntropyLoss())
+    graph = nn.GraphSequence(graph_layers, nn.SoftmaxCrossEntropyLoss(), precision="fp32")
+    for step in range(3):
+        _, le = eager.train(X, Y)
+        _, lg = graph.train(X, Y)
+        gpu.wait()
+        assert graph._plan.graph is not None
+        assert not doomed and not vk.vkarray._deferred
+        assert np.asarray(le).item() == np.asarray(lg).item(), step
+        for a, b in zip(weights(eager_layers), weights(graph_layers)):
             np.testing.assert_array_equal(a, b)
 
 

@@ -124,6 +124,11 @@ def test_upload_download_refused_during_capture(gpu):
         assert "graph capture" in _lib.last_error()
         assert _lib.lib.vkh_download(h, host.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(dev._ptr),
                                      ctypes.c_size_t(64)) == _lib.VKH_INVALID
+        # nor are event records (a job wait): the node would outlive its event
+        ev = ctypes.c_void_p()
+        assert _lib.lib.vkh_event_record(h, ctypes.byref(ev)) == _lib.VKH_INVALID
+        assert _lib.lib.vkh_timer_record(h, ctypes.byref(ev)) == _lib.VKH_INVALID
+        assert ev.value is None
         _lib.call("vkh_fill_f32", h, dev._ptr, 1.0, 16)    # the capture itself goes on
     finally:
         ex = ctypes.c_void_p()

@@ -1325,6 +1325,19 @@ k_skinny_combine(SkinnyArgs a, const float* part, uint64_t nchunk) {
 
 constexpr uint64_t SKINNY_MAX = 16;  // largest small dimension of the skinny path
 
+// k per chunk (and the chunk count) of the column walk over P outputs and K
+// terms: about 1024 threads per compute unit, at least 64 k per chunk; v4:
+// four outputs per thread.  The chunks fix the summation order.
+static uint64_t colwalk_chunk(vkh_gpu* g, uint64_t P, uint64_t K, bool v4, uint64_t& nchunk_out) {
+  const uint64_t pthreads = v4 ? P / 4 : P;
+  const uint64_t target = (uint64_t)(g->num_cus > 0 ? g->num_cus : 256) * 1024;
+  uint64_t nchunk = std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(target, pthreads), ceil_div(K, 64)));
+  nchunk = std::min<uint64_t>(nchunk, 65535);
+  const uint64_t kchunk = ceil_div(std::max<uint64_t>(K, 1), nchunk);
+  nchunk_out = ceil_div(std::max<uint64_t>(K, 1), kchunk);
+  return kchunk;
+}
+
 static void gemm_skinny(vkh_gpu* g, bool ta, bool tb, const GemmArgs& e) {
   SkinnyArgs a{};
   a.e = e;
@@ -1367,15 +1380,10 @@ static void gemm_skinny(vkh_gpu* g, bool ta, bool tb, const GemmArgs& e) {
     check_launch("gemm (skinny row-dot)");
     return;
   }
-  // column walk with k chunks: about 1024 threads per compute unit, at
-  // least 64 k per chunk
   const bool v4 = a.P % 4 == 0 && ((uintptr_t)a.X & 15u) == 0 && a.sxk % 4 == 0;
   const uint64_t pthreads = v4 ? a.P / 4 : a.P;
-  const uint64_t target = (uint64_t)(g->num_cus > 0 ? g->num_cus : 256) * 1024;
-  uint64_t nchunk = std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(target, pthreads), ceil_div(a.K, 64)));
-  nchunk = std::min<uint64_t>(nchunk, 65535);
-  const uint64_t kchunk = ceil_div(std::max<uint64_t>(a.K, 1), nchunk);
-  nchunk = ceil_div(std::max<uint64_t>(a.K, 1), kchunk);
+  uint64_t nchunk = 0;
+  const uint64_t kchunk = colwalk_chunk(g, a.P, a.K, v4, nchunk);
   Workspace ws(g, sizeof(float) * nchunk * (uint64_t)a.Q * a.P);
   float* part = (float*)ws.get();
   const uint64_t bx = ceil_div(pthreads, 256);
@@ -1454,6 +1462,108 @@ static void gemm_smallk(vkh_gpu* g, bool ta, bool tb, const GemmArgs& a) {
   dim3 grid((unsigned)ceil_div(groups, 256), (unsigned)std::min<uint64_t>(a.M, 65535));
   hipLaunchKernelGGL(k_gemm_smallk, grid, dim3(256), 0, g->stream, a, sam, sak, sbk, sbn, vb);
   check_launch("gemm (small k)");
+}
+
+// ---- backward of a one-output Dense layer after an activation ---------------------
+// For a last layer y = h w^T + b with w [1, N] and h = act(z) [B, N], one pass
+// over h produces everything its backward and the previous layer's bias
+// gradient need, with the bits of the separate steps:
+//   dx[b, n] = act'(h[b, n]) * (dy[b] * w[n])    (vkh_gemm, K = 1, aux = h)
+//   dx_bf, dx_bft: dx as BF16, row-major and transposed (the next products'
+//                  packed operands)
+//   dw[n]    = sum_b h[b, n] * dy[b]             (the column walk of vkh_gemm
+//                  for dY^T h: 0 + products in k order per chunk of kc_dw rows,
+//                  chunks summed in order from 0)
+//   db[n]    = sum_b dx[b, n]                    (vkh_bias_grad: per chunk of
+//                  kc_db rows in row order from its first element, chunks
+//                  summed in order)
+// A block stages a 128-row x 32-column tile of dx and of h in LDS (33 KB, four
+// blocks a compute unit); the chunk sums walk its columns and the transposed
+// copy leaves in 256-byte runs (16 bytes a lane).  kc_dw and kc_db divide 128.
+constexpr int D1_TR = 128, D1_TC = 32, D1_PITCH = D1_TC + 1;  // odd pitch: column walks conflict-free
+constexpr int D1_LOADS = D1_TR * D1_TC / 4 / 256;                 // float4 loads of h a thread
+
+__global__ void __launch_bounds__(256)
+k_dense1_backward(const float* __restrict__ dy, const float* __restrict__ w, const float* __restrict__ h,
+                  int act, uint64_t N, float* __restrict__ dx, uint16_t* __restrict__ dx_bf,
+                  uint16_t* __restrict__ dx_bft, uint64_t ldt, float* __restrict__ part_dw, int kc_dw,
+                  float* __restrict__ part_db, int kc_db) {
+  __shared__ float D[D1_TR * D1_PITCH], H[D1_TR * D1_PITCH], Y[D1_TR];
+  const uint64_t n0 = (uint64_t)blockIdx.x * D1_TC, b0 = (uint64_t)blockIdx.y * D1_TR;
+  const int t = threadIdx.x, c4 = t % (D1_TC / 4), r0 = t / (D1_TC / 4);
+  const uint64_t n = n0 + c4 * 4;
+  const float4 wv = *reinterpret_cast<const float4*>(w + n);
+  const float wq[4] = {wv.x, wv.y, wv.z, wv.w};
+  // every load of the tile first, then the arithmetic
+  float4 hv[D1_LOADS];
+  float d[D1_LOADS];
+#pragma unroll
+  for (int it = 0; it < D1_LOADS; ++it) {
+    const uint64_t b = b0 + r0 + it * (256 / (D1_TC / 4));
+    hv[it] = ld_stream(reinterpret_cast<const float4*>(h + b * N + n));
+    d[it] = dy[b];
+  }
+  if (t < D1_TR) Y[t] = dy[b0 + t];
+#pragma unroll
+  for (int it = 0; it < D1_LOADS; ++it) {
+    const int r = r0 + it * (256 / (D1_TC / 4));
+    const uint64_t b = b0 + r;
+    const float hq[4] = {hv[it].x, hv[it].y, hv[it].z, hv[it].w};
+    float o[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      o[q] = act_deriv_mul(act, hq[q], d[it] * wq[q]);
+      D[r * D1_PITCH + c4 * 4 + q] = o[q];
+      H[r * D1_PITCH + c4 * 4 + q] = hq[q];
+    }
+    st_stream(reinterpret_cast<float4*>(dx + b * N + n), make_float4(o[0], o[1], o[2], o[3]));
+    if (dx_bf) {
+      uint2 hb;
+      hb.x = f2bf(o[0]) | ((uint32_t)f2bf(o[1]) << 16);
+      hb.y = f2bf(o[2]) | ((uint32_t)f2bf(o[3]) << 16);
+      *reinterpret_cast<uint2*>(dx_bf + b * N + n) = hb;
+    }
+  }
+  __syncthreads();
+  // chunk sums: one thread per (chunk, column), rows in order
+  const int ndb = part_db ? D1_TR / kc_db : 0, ndw = part_dw ? D1_TR / kc_dw : 0;
+  for (int item = t; item < (ndb + ndw) * D1_TC; item += 256) {
+    const int col = item % D1_TC, j = item / D1_TC;
+    if (j < ndb) {
+      const int rs = j * kc_db;
+      float acc = D[rs * D1_PITCH + col];
+      for (int r = 1; r < kc_db; ++r) acc = acc + D[(rs + r) * D1_PITCH + col];
+      part_db[(b0 / kc_db + j) * N + n0 + col] = acc;
+    } else {
+      const int rs = (j - ndb) * kc_dw;
+      float acc = 0.f;
+      for (int r = 0; r < kc_dw; ++r) acc += H[(rs + r) * D1_PITCH + col] * Y[rs + r];
+      part_dw[(b0 / kc_dw + j - ndb) * N + n0 + col] = acc;
+    }
+  }
+  if (dx_bft) {
+    // D1_TC columns x 16 runs of 8 rows; 16 consecutive lanes continue one column
+    for (int item = t; item < D1_TC * (D1_TR / 8); item += 256) {
+      const int col = item / (D1_TR / 8), seg = item % (D1_TR / 8);
+      uint32_t wd[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        wd[q] = f2bf(D[(seg * 8 + 2 * q) * D1_PITCH + col]) |
+                ((uint32_t)f2bf(D[(seg * 8 + 2 * q + 1) * D1_PITCH + col]) << 16);
+      *reinterpret_cast<uint4*>(dx_bft + (n0 + col) * ldt + b0 + seg * 8) = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+    }
+  }
+}
+
+// chunk lengths of the separate steps (dw: the column walk over N outputs and
+// B terms; db: vkh_bias_grad's column sums); false where they do not tile the
+// kernel's 128 rows
+static bool dense1_chunks(vkh_gpu* g, uint64_t B, uint64_t N, uint64_t& kc_dw, uint64_t& ndw, uint64_t& kc_db) {
+  if (B == 0 || B % D1_TR != 0 || N % D1_TC != 0 || B / D1_TR >= 65536 || N / D1_TC >= (uint64_t(1) << 31))
+    return false;
+  kc_dw = colwalk_chunk(g, N, B, true, ndw);
+  kc_db = bias_grad_chunk(g, B, N, true);
+  return kc_dw <= (uint64_t)D1_TR && D1_TR % kc_dw == 0 && kc_db <= (uint64_t)D1_TR && D1_TR % kc_db == 0;
 }
 
 static void gemm_impl(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M, uint64_t N,
@@ -1683,6 +1793,56 @@ int vkh_gemm_bf16io(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M, 
   return guarded([&] {
     gemm_impl(g, mode, trans_a, trans_b, M, N, K, alpha, A, lda, B, ldb, beta, Cp, ldc, bias, act,
               c_bf16, aux, ldaux, aux_act, a_bf16, b_bf16, c_bf16t, ldt);
+  });
+}
+
+int vkh_dense1_backward_supported(vkh_gpu* g, uint64_t B, uint64_t N, int* ok) {
+  return guarded([&] {
+    VKH_REQUIRE(g && ok, "dense1_backward_supported: null argument");
+    uint64_t kc_dw = 0, ndw = 0, kc_db = 0;
+    *ok = dense1_chunks(g, B, N, kc_dw, ndw, kc_db) ? 1 : 0;
+  });
+}
+
+int vkh_dense1_backward(vkh_gpu* g, const float* dy, const float* w, const float* h, int act, uint64_t B,
+                        uint64_t N, float* dx, uint16_t* dx_bf, uint16_t* dx_bft, uint64_t ldt, float* dw,
+                        float* db) {
+  return guarded([&] {
+    use(g);
+    VKH_REQUIRE(dy && w && h && dx, "dense1_backward: null pointer");
+    VKH_REQUIRE(act >= VKH_ACT_NONE && act <= VKH_ACT_SIGMOID, "dense1_backward: unknown activation");
+    auto a16 = [](const void* p) { return ((uintptr_t)p & 15u) == 0; };
+    VKH_REQUIRE(a16(w) && a16(h) && a16(dx), "dense1_backward: operands not 16-byte aligned");
+    VKH_REQUIRE(!dx_bf || ((uintptr_t)dx_bf & 7u) == 0, "dense1_backward: dx_bf not 8-byte aligned");
+    VKH_REQUIRE(!dx_bft || (a16(dx_bft) && ldt % 8 == 0 && ldt >= B), "dense1_backward: dx_bft layout");
+    uint64_t kc_dw = 0, ndw = 0, kc_db = 0;
+    VKH_REQUIRE(dense1_chunks(g, B, N, kc_dw, ndw, kc_db), "dense1_backward: shape not supported");
+    const uint64_t ndb = B / kc_db;
+    Workspace wdw(g, dw ? sizeof(float) * ndw * N : 1), wdb(g, db ? sizeof(float) * ndb * N : 1);
+    float* pdw = dw ? (float*)wdw.get() : nullptr;
+    float* pdb = db ? (float*)wdb.get() : nullptr;
+    dim3 grid((unsigned)(N / D1_TC), (unsigned)(B / D1_TR));
+    hipLaunchKernelGGL(k_dense1_backward, grid, dim3(256), 0, g->stream, dy, w, h, act, N, dx, dx_bf, dx_bft,
+                       ldt, pdw, (int)kc_dw, pdb, (int)kc_db);
+    check_launch("dense1_backward");
+    if (dw) {  // the column walk's combine: 0 + chunks in order, then 1 * sum
+      SkinnyArgs a{};
+      a.P = N;
+      a.Q = 1;
+      a.swap = 1;
+      a.e.C = dw;
+      a.e.ldc = N;
+      a.e.M = 1;
+      a.e.N = N;
+      a.e.alpha = 1.0f;
+      a.e.beta = 0.0f;
+      a.e.act = VKH_ACT_NONE;
+      a.e.aux_act = VKH_ACT_NONE;
+      hipLaunchKernelGGL(k_skinny_combine, dim3(full_grid(N, 256)), dim3(256), 0, g->stream, a,
+                         (const float*)pdw, ndw);
+      check_launch("dense1_backward dw");
+    }
+    if (db) sum_chunks(g, (const float*)pdb, db, N, ndb);
   });
 }
 

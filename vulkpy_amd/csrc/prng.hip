@@ -576,6 +576,7 @@ int vkh_xoshiro_destroy(vkh_xoshiro* r) {
   return guarded([&] {
     if (!r) return;
     use(r->gpu);
+    VKH_REQUIRE(!r->gpu->capturing, "xoshiro_destroy: not allowed during graph capture");
     if (r->from_alloc) r->gpu->alloc->release(r->state);
     else { VKH_HIP(hipStreamSynchronize(r->gpu->stream)); VKH_HIP(hipFree(r->state)); }
     if (r->jumps) {

@@ -393,6 +393,29 @@ static void reduce_all_noalloc(vkh_gpu* g, const float* a, uint64_t n, float* ou
   launch_all<OP>(g, a, n, out, mean, c, nb, scratch_address(g));
 }
 
+// Rows per chunk (and the chunk count) of the strided-axis reduction of
+// [prev, K, post] in configuration cfg.  Split k when there are fewer than
+// cfg.par threads of columns per CU and the axis is long enough to give every
+// chunk a reasonable amount of work (default 512: on 8192 x 16384, 32 chunks
+// of 256 rows reach 6.4 TB/s, 64 chunks 6.1).  The chunk count fixes the
+// summation order of sums and products, so it is part of the shape's
+// configuration.
+static uint64_t cols_chunk(vkh_gpu* g, const RedCfg& cfg, uint64_t prev, uint64_t K, uint64_t post, bool vec,
+                           uint64_t& nchunk_out) {
+  const unsigned cus = g->num_cus > 0 ? g->num_cus : 256;
+  const uint64_t threads = prev * (post / (vec ? 4 : 1));
+  const uint64_t want = (uint64_t)cus * (uint64_t)cfg.par;
+  uint64_t nchunk = 1;
+  if (threads < want && K >= 64) {
+    nchunk = std::min<uint64_t>(ceil_div(want, threads), K / 32);
+    if (nchunk > 1024) nchunk = 1024;
+    if (nchunk < 1) nchunk = 1;
+  }
+  const uint64_t kchunk = ceil_div(K, nchunk);
+  nchunk_out = ceil_div(K, kchunk);
+  return kchunk;
+}
+
 template <int OP>
 static void reduce_axis(vkh_gpu* g, const float* a, float* out, uint64_t prev, uint64_t K,
                         uint64_t post, int rebroadcast, float mul, float dv, int index = -1) {
@@ -422,20 +445,8 @@ static void reduce_axis(vkh_gpu* g, const float* a, float* out, uint64_t prev, u
   const bool vec = (post % 4 == 0) && (((uintptr_t)a & 15u) == 0) && (((uintptr_t)out & 15u) == 0);
   const int VW = vec ? 4 : 1;
   const uint64_t threads = prev * (post / VW);
-  // Split k when there are fewer than cfg.par threads of columns per CU and
-  // the axis is long enough to give every chunk a reasonable amount of work
-  // (default 512: on 8192 x 16384, 32 chunks of 256 rows reach 6.4 TB/s, 64
-  // chunks 6.1).  The chunk count fixes the summation order of sums and
-  // products, so it is part of the shape's configuration.
-  const uint64_t want = (uint64_t)cus * (uint64_t)cfg.par;
-  uint64_t nchunk = 1;
-  if (threads < want && K >= 64) {
-    nchunk = std::min<uint64_t>(ceil_div(want, threads), K / 32);
-    if (nchunk > 1024) nchunk = 1024;
-    if (nchunk < 1) nchunk = 1;
-  }
-  const uint64_t kchunk = ceil_div(K, nchunk);
-  nchunk = ceil_div(K, kchunk);
+  uint64_t nchunk = 0;
+  const uint64_t kchunk = cols_chunk(g, cfg, prev, K, post, vec, nchunk);
   const uint64_t m = prev * post;
   Workspace ws(g, (nchunk > 1 || rebroadcast) ? sizeof(float) * nchunk * m : 1);
   float* dst = (nchunk > 1 || rebroadcast) ? (float*)ws.get() : out;
@@ -503,6 +514,29 @@ int vkh_reduce_all(vkh_gpu* g, int op, const float* a, uint64_t n, float* out, i
     }
   });
 }
+
+}  // extern "C"
+
+namespace vkh {
+// chunk length of the column sums of a [rows, cols] operand (vkh_bias_grad's
+// reduction), for kernels that must sum in the same order; vec: the operand
+// and output allow 16-byte accesses
+uint64_t bias_grad_chunk(vkh_gpu* g, uint64_t rows, uint64_t cols, bool vec) {
+  const RedCfg cfg = red_cfg(g, red_family(1, rows, cols), 1, rows, cols, -1);
+  uint64_t nchunk = 0;
+  return cols_chunk(g, cfg, 1, rows, cols, vec, nchunk);
+}
+
+// combine of chunk partials [nchunk, m] in chunk order (k_combine_chunks)
+void sum_chunks(vkh_gpu* g, const float* part, float* out, uint64_t m, uint64_t nchunk) {
+  const unsigned cus = g->num_cus > 0 ? g->num_cus : 256;
+  unsigned gb = (unsigned)std::min<uint64_t>(ceil_div(m, 256), (uint64_t)cus * 16);
+  hipLaunchKernelGGL(k_combine_chunks<VKH_RSUM>, dim3(gb), dim3(256), 0, g->stream, part, out, m, nchunk, 1.0f, 1.0f);
+  check_launch("sum_chunks");
+}
+}  // namespace vkh
+
+extern "C" {
 
 int vkh_reduce_all_noalloc(vkh_gpu* g, int op, const float* a, uint64_t n, float* out, int mean) {
   return guarded([&] {

@@ -417,6 +417,9 @@ int vkh_event_record(vkh_gpu* g, vkh_event** out) {
   return guarded([&] {
     use(g);
     VKH_REQUIRE(out, "null out");
+    // a record on a capturing stream would become a node of the graph that
+    // refers to this event after its owner has destroyed it
+    VKH_REQUIRE(!g->capturing, "event_record: not allowed during graph capture");
     auto* e = new vkh_event();
     e->device = g->device;
     hipError_t r = hipEventCreateWithFlags(&e->ev, hipEventDisableTiming);
@@ -431,6 +434,7 @@ int vkh_timer_record(vkh_gpu* g, vkh_event** out) {
   return guarded([&] {
     use(g);
     VKH_REQUIRE(out, "null out");
+    VKH_REQUIRE(!g->capturing, "timer_record: not allowed during graph capture");
     auto* e = new vkh_event();
     e->device = g->device;
     hipError_t r = hipEventCreate(&e->ev);
@@ -587,6 +591,8 @@ int vkh_graph_destroy(void* exec) {
     if (!exec) return;
     auto* gr = (vkh::Graph*)exec;
     use(gr->gpu);
+    // the release waits for the stream, which a capture cannot do
+    VKH_REQUIRE(!gr->gpu->capturing, "graph_destroy: not allowed during graph capture");
     VKH_HIP(hipStreamSynchronize(gr->gpu->stream));
     if (gr->exec) VKH_HIP(hipGraphExecDestroy(gr->exec));
     if (gr->arena) gr->gpu->alloc->release(gr->arena);

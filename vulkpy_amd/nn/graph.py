@@ -32,11 +32,13 @@ overwrites).
 from __future__ import annotations
 
 import ctypes
+import functools
 from typing import Iterable, List, Optional, Tuple
 
 import numpy as np
 
 from vulkpy_amd._lib import ACT, GEMM_MODE, call
+from vulkpy_amd import vkarray
 from vulkpy_amd.vkarray import Array, _DeviceBuffer
 from .core import Loss, Module
 from .layers import Dense, ReLU, Sigmoid
@@ -146,6 +148,14 @@ class _Plan:
         # the matrix cores, whose epilogue writes C^T coalesced
         self.hs_bft: List[Optional[_DeviceBuffer]] = [None] * len(units)
         self.ds_bft: List[Optional[_DeviceBuffer]] = [None] * len(units)
+        # a last layer with one output after a hidden layer: its backward and
+        # the hidden layer's bias gradient in one pass (vkh_dense1_backward)
+        last = units[-1].dense
+        self.fuse_last = False
+        if len(units) > 1 and last.output_dim == 1:
+            ok = ctypes.c_int(0)
+            call("vkh_dense1_backward_supported", gpu._handle, B, last.input_dim, ctypes.byref(ok))
+            self.fuse_last = bool(ok.value)
         bf16 = GEMM_MODE["bf16"]
         rows = -(-B // 256) * 256
         if self.mode_fwd == bf16:
@@ -166,9 +176,11 @@ class _Plan:
                 if (nxt.w.grad is not None and _mfma(B, d.output_dim, d.input_dim)
                         and _packs_a(nxt.output_dim, d.output_dim, B)):
                     self.hs_bft[i] = _DeviceBuffer(gpu, trows * B, np.uint16)
-                # ds[i]^T: dX product of unit i + 1 -> op(A) of dW of unit i
-                if (d.w.grad is not None and _mfma(B, d.output_dim, nxt.output_dim)
-                        and _packs_a(d.output_dim, d.input_dim, B)):
+                # ds[i]^T: dX product of unit i + 1 (or the fused last-layer
+                # backward) -> op(A) of dW of unit i
+                producer = _mfma(B, d.output_dim, nxt.output_dim) or (
+                    self.fuse_last and i == len(units) - 2)
+                if d.w.grad is not None and producer and _packs_a(d.output_dim, d.input_dim, B):
                     self.ds_bft[i] = _DeviceBuffer(gpu, trows * B, np.uint16)
         # parameters, their optimizer states, regularizer terms and device
         # beta powers
@@ -242,13 +254,17 @@ class _Plan:
             call("vkh_activation_backward", h, ACT[units[-1].act], out._ptr, self.g_out._ptr,
                  self.ds[-1]._ptr, B * O)
         # backward
-        for i in range(len(units) - 1, -1, -1):
+        last = len(units) - 1
+        for i in range(last, -1, -1):
             d = units[i].dense
             dy = self.ds[i]
             xin = self.x if i == 0 else self.hs[i - 1]
+            fused = self.fuse_last and i == last
             for p, is_w in ((d.w, True), (d.b, False)):
                 if p.grad is None:
                     continue
+                if (is_w and fused) or (not is_w and self.fuse_last and i == last - 1):
+                    continue  # taken by vkh_dense1_backward
                 # the regularizer term is added by the optimizer kernel
                 beta = 0.0
                 if is_w:
@@ -259,7 +275,16 @@ class _Plan:
                          None, 0)
                 else:
                     call("vkh_bias_grad", h, dy._ptr, p.grad._ptr, B, d.output_dim, beta)
-            if i > 0:
+            if fused:
+                prev = units[i - 1]
+                out_bf, out_bft = self.ds_bf[i - 1], self.ds_bft[i - 1]
+                pb = prev.dense.b.grad
+                call("vkh_dense1_backward", h, dy._ptr, d.w.value._ptr, self.hs[i - 1]._ptr,
+                     ACT[prev.act], B, d.input_dim, self.ds[i - 1]._ptr,
+                     out_bf.ptr if out_bf else None, out_bft.ptr if out_bft else None, B,
+                     d.w.grad._ptr if d.w.grad is not None else None,
+                     pb._ptr if pb is not None else None)
+            elif i > 0:
                 prev = units[i - 1]
                 out_bf, dy_bf, out_bft = self.ds_bf[i - 1], self.ds_bf[i], self.ds_bft[i - 1]
                 call("vkh_gemm_bf16io", h, self.mode_dx, 0, 0, B, d.input_dim, d.output_dim, 1.0, dy._ptr,
@@ -319,18 +344,22 @@ class _Plan:
         if not self.seq.capture:
             return
         call("vkh_graph_begin", self.h)
+        vkarray._capture_begin()
         try:
-            self._enqueue()
-        except BaseException:
+            try:
+                self._enqueue()
+            except BaseException:
+                ex = ctypes.c_void_p()
+                from vulkpy_amd import _lib
+                _lib.lib.vkh_graph_end(self.h, ctypes.byref(ex))
+                if ex.value:
+                    _lib.lib.vkh_graph_destroy(ex.value)
+                raise
             ex = ctypes.c_void_p()
-            from vulkpy_amd import _lib
-            _lib.lib.vkh_graph_end(self.h, ctypes.byref(ex))
-            if ex.value:
-                _lib.lib.vkh_graph_destroy(ex.value)
-            raise
-        ex = ctypes.c_void_p()
-        call("vkh_graph_end", self.h, ctypes.byref(ex))
-        self.graph = ex.value
+            call("vkh_graph_end", self.h, ctypes.byref(ex))
+            self.graph = ex.value
+        finally:
+            vkarray._capture_end()
 
     def run(self, x: Array, y: Array) -> Tuple[Array, Array]:
         h = self.h
@@ -369,8 +398,10 @@ class _Plan:
         try:
             if self.graph:
                 from vulkpy_amd import _lib
-                _lib.lib.vkh_graph_destroy(self.graph)
-                self.graph = None
+                graph, self.graph = self.graph, None
+                # not while another step is being captured
+                if not vkarray._after_capture(functools.partial(_lib.lib.vkh_graph_destroy, graph)):
+                    _lib.lib.vkh_graph_destroy(graph)
         except Exception:
             pass
 

@@ -27,6 +27,7 @@ Standard normal floats, continuing the same streams:
 from __future__ import annotations
 
 import ctypes
+import functools
 import os
 from typing import cast, Iterable, Optional
 
@@ -182,8 +183,10 @@ class Xoshiro128pp(PRNG):
     def __del__(self):
         try:
             if self._rng:
-                _lib.lib.vkh_xoshiro_destroy(self._rng)
-                self._rng = None
+                rng, self._rng = self._rng, None
+                # the release waits for the stream: not during a graph capture
+                if not vk._after_capture(functools.partial(_lib.lib.vkh_xoshiro_destroy, rng)):
+                    _lib.lib.vkh_xoshiro_destroy(rng)
         except Exception:
             pass
 

@@ -79,6 +79,46 @@ class _Context:
             self.done_ticket = ticket
 
 
+# ---- finalizers during a graph capture -------------------------------------------
+# A capturing stream cannot be waited on, and a host wait, an event's
+# destruction or a pinned block's release in the capturing thread would end the
+# capture with an error.  Finalizers run by garbage collection meanwhile (any
+# allocation can trigger one) queue that work here instead; it runs when the
+# capture ends.
+_captures = 0
+_deferred: List = []
+
+
+def _capture_begin() -> None:
+    global _captures
+    _captures += 1
+
+
+def _capture_end() -> None:
+    global _captures
+    _captures -= 1
+    if _captures == 0:
+        work = list(_deferred)
+        del _deferred[:]
+        for fn in work:
+            try:
+                fn()
+            except Exception:
+                pass
+
+
+def _after_capture(fn) -> bool:
+    """Queue ``fn`` for the end of the running capture; False when none runs."""
+    if _captures == 0:
+        return False
+    _deferred.append(fn)
+    return True
+
+
+def _destroy_event(ev: int) -> None:
+    _lib.lib.vkh_event_destroy(ev)
+
+
 class _Event:
     """One recorded HIP event and the tickets it covers."""
 
@@ -93,8 +133,9 @@ class _Event:
     def __del__(self):
         try:
             if self.ev:
-                _lib.lib.vkh_event_destroy(self.ev)
-                self.ev = None
+                ev, self.ev = self.ev, None
+                if not _after_capture(functools.partial(_destroy_event, ev)):
+                    _destroy_event(ev)
         except Exception:
             pass
 
@@ -142,6 +183,8 @@ class Job:
         ctx = self._ctx
         if self._ticket <= ctx.done_ticket:
             return
+        if _captures:
+            raise RuntimeError("Job.wait during a graph capture")
         e = ctx.covering(self._ticket)  # held here while waiting
         t = _UINT64_MAX if timeout_ns is None else int(timeout_ns)
         _lib.check(_lib.lib.vkh_event_wait(e.ev, t), "Error at Command Wait")
@@ -299,9 +342,18 @@ _PIN_MIN_BYTES = 1 << 20
 _UPLOAD_CHUNK_BYTES = 16 << 20
 
 
+def _wait_then_drop(job, host) -> None:
+    job.wait()
+
+
+def _host_free(ptr: int) -> None:
+    _lib.load().vkh_host_free(ptr)
+
+
 def _host_release(ptr: int) -> None:
     try:
-        _lib.load().vkh_host_free(ptr)
+        if not _after_capture(functools.partial(_host_free, ptr)):
+            _host_free(ptr)
     except Exception:
         pass
 
@@ -393,7 +445,9 @@ class _GPUArray(Resource):
         # host wait; only an upload still reading the host mirror is waited for.
         try:
             if self._upload_pending and self.job is not None:
-                self.job.wait()
+                # during a capture: wait afterwards, the mirror kept till then
+                if not _after_capture(functools.partial(_wait_then_drop, self.job, self._host)):
+                    self.job.wait()
         except Exception:
             pass
 
