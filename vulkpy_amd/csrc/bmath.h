@@ -89,6 +89,40 @@ __host__ __device__ inline double rsqrt_pos(double x) {
   return fma(0.5 * y, e, y);
 }
 
+// ---- FP64 fused multiply-adds with a constant operand ---------------------------
+// A 64-bit constant is not an inline operand of a vector instruction, so the
+// compiler copies it into vector registers, two moves per use, which cost as
+// much issue time as the FMA itself.  These name it as an operand held in
+// scalar registers instead (filled by scalar moves, which issue beside the
+// vector instructions): one v_fma_f64 (v_mul_f64) in all, with the IEEE result
+// of fma(a, b, c) (a * b).  The operand marked k must be the same in every
+// lane (a literal constant).  Host: the standard functions.
+__device__ inline double fma_kc(double a, double b, double c) {  // c constant
+  double r;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+  return r;
+}
+__host__ inline double fma_kc(double a, double b, double c) { return fma(a, b, c); }
+__device__ inline double fma_kb(double a, double b, double c) {  // b constant
+  double r;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+  return r;
+}
+__host__ inline double fma_kb(double a, double b, double c) { return fma(a, b, c); }
+// -a * b + c, b constant
+__device__ inline double fnma_kb(double a, double b, double c) {
+  double r;
+  asm("v_fma_f64 %0, -%1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+  return r;
+}
+__host__ inline double fnma_kb(double a, double b, double c) { return fma(-a, b, c); }
+__device__ inline double mul_kb(double a, double b) {  // b constant
+  double r;
+  asm("v_mul_f64 %0, %1, %2" : "=v"(r) : "v"(a), "s"(b));
+  return r;
+}
+__host__ inline double mul_kb(double a, double b) { return a * b; }
+
 // ---- exp ----------------------------------------------------------------------
 // e^t for |t| <= 700: t = k ln 2 + r with k = rint(t / ln 2) and |r| <= ln 2 / 2
 // (two-piece ln 2: r is exact to 2^-85 relative), then the Taylor polynomial
@@ -97,18 +131,17 @@ __host__ __device__ inline double rsqrt_pos(double x) {
 // as the FP64 library exp, and faster than a 2^(j/64) table with a degree-5
 // polynomial (whose per-lane table reads cost more than the terms saved).
 __host__ __device__ inline double exp_core(double t) {
-  const double k = rint(t * LOG2E);
-  double r = fma(-k, LN2_HI, t);
-  r = fma(-k, LN2_LO, r);
-  double p = 1.0 / 39916800.0;             // 1/11!
-  p = fma(p, r, 1.0 / 3628800.0);
-  p = fma(p, r, 1.0 / 362880.0);
-  p = fma(p, r, 1.0 / 40320.0);
-  p = fma(p, r, 1.0 / 5040.0);
-  p = fma(p, r, 1.0 / 720.0);
-  p = fma(p, r, 1.0 / 120.0);
-  p = fma(p, r, 1.0 / 24.0);
-  p = fma(p, r, 1.0 / 6.0);
+  const double k = rint(mul_kb(t, LOG2E));
+  double r = fnma_kb(k, LN2_HI, t);
+  r = fnma_kb(k, LN2_LO, r);
+  double p = fma_kb(r, 1.0 / 39916800.0, 1.0 / 3628800.0);  // 1/11!, 1/10!
+  p = fma_kc(p, r, 1.0 / 362880.0);
+  p = fma_kc(p, r, 1.0 / 40320.0);
+  p = fma_kc(p, r, 1.0 / 5040.0);
+  p = fma_kc(p, r, 1.0 / 720.0);
+  p = fma_kc(p, r, 1.0 / 120.0);
+  p = fma_kc(p, r, 1.0 / 24.0);
+  p = fma_kc(p, r, 1.0 / 6.0);
   p = fma(p, r, 0.5);
   p = fma(p, r, 1.0);
   p = fma(p, r, 1.0);
@@ -133,12 +166,12 @@ __host__ __device__ inline float exp2_f32(float x) { return exp_to_f32((double)x
 // terms are below 2^-45 relative
 __host__ __device__ inline double atanh_series(double s) {
   const double z = s * s;
-  double p = fma(z, 1.0 / 15.0, 1.0 / 13.0);
-  p = fma(z, p, 1.0 / 11.0);
-  p = fma(z, p, 1.0 / 9.0);
-  p = fma(z, p, 1.0 / 7.0);
-  p = fma(z, p, 1.0 / 5.0);
-  p = fma(z, p, 1.0 / 3.0);
+  double p = fma_kb(z, 1.0 / 15.0, 1.0 / 13.0);
+  p = fma_kc(z, p, 1.0 / 11.0);
+  p = fma_kc(z, p, 1.0 / 9.0);
+  p = fma_kc(z, p, 1.0 / 7.0);
+  p = fma_kc(z, p, 1.0 / 5.0);
+  p = fma_kc(z, p, 1.0 / 3.0);
   return fma(s * z, p, s);
 }
 
@@ -166,7 +199,7 @@ __host__ __device__ inline double log_mant(double x, int& k_out) {
 __host__ __device__ inline double log_pos(double x) {
   int k;
   const double lm = log_mant(x, k);
-  return fma((double)k, LN2_HI, fma((double)k, LN2_LO, lm));
+  return fma_kb((double)k, LN2_HI, fma_kb((double)k, LN2_LO, lm));
 }
 
 // log and log2 of FP32 x through log_mant: k ln 2 + log m and k + log m /
@@ -182,7 +215,7 @@ __host__ __device__ inline float log2_f32(float x) {
   if (!(x > 0.0f) || x == INFINITY) return x == 0.0f ? -INFINITY : (x < 0.0f ? NAN : x);
   int k;
   const double lm = log_mant(x, k);
-  return (float)fma(lm, LOG2E, (double)k);
+  return (float)fma_kb(lm, LOG2E, (double)k);
 }
 
 // a ** b (C pow semantics) for FP32 operands.  A finite positive base with a
@@ -278,13 +311,12 @@ constexpr double TINY = 0x1p-13;
 // z in [0, (pi/2)^2]: relative error below 2^-50
 __host__ __device__ inline double sin_pio2(double r) {
   const double z = r * r;
-  double p = -0x1.a1095286ba17ap-41;
-  p = fma(p, z, 0x1.60f6f0d66f8acp-33);
-  p = fma(p, z, -0x1.ae63bdfa8502ap-26);
-  p = fma(p, z, 0x1.71de394e5888cp-19);
-  p = fma(p, z, -0x1.a01a019e69965p-13);
-  p = fma(p, z, 0x1.1111111110067p-7);
-  p = fma(p, z, -0x1.555555555554fp-3);
+  double p = fma_kb(z, -0x1.a1095286ba17ap-41, 0x1.60f6f0d66f8acp-33);
+  p = fma_kc(p, z, -0x1.ae63bdfa8502ap-26);
+  p = fma_kc(p, z, 0x1.71de394e5888cp-19);
+  p = fma_kc(p, z, -0x1.a01a019e69965p-13);
+  p = fma_kc(p, z, 0x1.1111111110067p-7);
+  p = fma_kc(p, z, -0x1.555555555554fp-3);
   return fma(r * z, p, r);
 }
 
@@ -292,21 +324,19 @@ __host__ __device__ inline double sin_pio2(double r) {
 // z in [0, (pi/4)^2], relative errors below 2^-45 and 2^-49
 __host__ __device__ inline double sin_pio4(double r) {
   const double z = r * r;
-  double p = -0x1.aa285788aaa42p-26;
-  p = fma(p, z, 0x1.71d9a9f41c5a9p-19);
-  p = fma(p, z, -0x1.a019fd9b35ee5p-13);
-  p = fma(p, z, 0x1.1111110fd3d43p-7);
-  p = fma(p, z, -0x1.555555555516bp-3);
+  double p = fma_kb(z, -0x1.aa285788aaa42p-26, 0x1.71d9a9f41c5a9p-19);
+  p = fma_kc(p, z, -0x1.a019fd9b35ee5p-13);
+  p = fma_kc(p, z, 0x1.1111110fd3d43p-7);
+  p = fma_kc(p, z, -0x1.555555555516bp-3);
   return fma(r * z, p, r);
 }
 
 __host__ __device__ inline double cos_pio4(double r) {
   const double z = r * r;
-  double p = 0x1.1c81c3531fff2p-29;
-  p = fma(p, z, -0x1.27e25f4bb4e6fp-22);
-  p = fma(p, z, 0x1.a019ff53a6a1cp-16);
-  p = fma(p, z, -0x1.6c16c16b614fcp-10);
-  p = fma(p, z, 0x1.5555555555437p-5);
+  double p = fma_kb(z, 0x1.1c81c3531fff2p-29, -0x1.27e25f4bb4e6fp-22);
+  p = fma_kc(p, z, 0x1.a019ff53a6a1cp-16);
+  p = fma_kc(p, z, -0x1.6c16c16b614fcp-10);
+  p = fma_kc(p, z, 0x1.5555555555437p-5);
   return fma(z * z, p, fma(z, -0.5, 1.0));
 }
 
@@ -316,9 +346,9 @@ __host__ __device__ inline double cos_pio4(double r) {
 // for |m| < 2^20), accurate to 2^-70 absolute, far below r itself next to any
 // multiple of pi/2 an FP32 value can come near.
 __host__ __device__ inline double reduce_pio2(double x, double m) {
-  double t = fma(-m, PIO2_1, x);
-  t = fma(-m, PIO2_2, t);
-  return fma(-m, PIO2_3, t);
+  double t = fnma_kb(m, PIO2_1, x);
+  t = fnma_kb(m, PIO2_2, t);
+  return fnma_kb(m, PIO2_3, t);
 }
 
 constexpr double REDUCE_MAX = 0x1p20;  // beyond: the FP64 library functions
@@ -329,7 +359,7 @@ __host__ __device__ inline float sin_f32(float x) {
   const double d = (double)x, a = fabs(d);
   if (a < TINY) return x;
   if (!(a < REDUCE_MAX)) return (float)sin(d);  // huge, inf, NaN
-  const double q = rint(d * (1.0 / PI));
+  const double q = rint(mul_kb(d, 1.0 / PI));
   const double v = sin_pio2(reduce_pio2(d, 2.0 * q));
   return (float)(((int)q & 1) ? -v : v);
 }
@@ -338,7 +368,7 @@ __host__ __device__ inline float cos_f32(float x) {
   const double d = (double)x, a = fabs(d);
   if (a < TINY) return 1.0f;
   if (!(a < REDUCE_MAX)) return (float)cos(d);
-  const double q = rint(fma(d, 1.0 / PI, 0.5));
+  const double q = rint(fma_kb(d, 1.0 / PI, 0.5));
   const double v = sin_pio2(reduce_pio2(d, 2.0 * q - 1.0));
   return (float)(((int)q & 1) ? -v : v);
 }
@@ -348,7 +378,7 @@ __host__ __device__ inline float tan_f32(float x) {
   const double d = (double)x, a = fabs(d);
   if (a < TINY) return x;
   if (!(a < REDUCE_MAX)) return (float)tan(d);
-  const double m = rint(d * TWO_OVER_PI);
+  const double m = rint(mul_kb(d, TWO_OVER_PI));
   const double r = reduce_pio2(d, m);
   const double s = sin_pio4(r), c = cos_pio4(r);
   return (float)(((int)m & 1) ? -c * recip(s) : s * recip(c));
@@ -364,16 +394,15 @@ __host__ __device__ inline float atan_f32(float x) { return (float)atan((double)
 // z in [0, 1/4]: asin's relative error below 2^-46.
 __host__ __device__ inline double asin_half(double s) {
   const double z = s * s;
-  double p = 0x1.c93a92d53b4f1p-6;
-  p = fma(p, z, -0x1.815314c864b09p-9);
-  p = fma(p, z, 0x1.00d47e7966d94p-6);
-  p = fma(p, z, 0x1.b02442413f6bap-7);
-  p = fma(p, z, 0x1.1dc2ef640046fp-6);
-  p = fma(p, z, 0x1.6e72146fda29ep-6);
-  p = fma(p, z, 0x1.f1c81c59ea536p-6);
-  p = fma(p, z, 0x1.6db6d8e71341bp-5);
-  p = fma(p, z, 0x1.33333335a9cd6p-4);
-  p = fma(p, z, 0x1.5555555554f05p-3);
+  double p = fma_kb(z, 0x1.c93a92d53b4f1p-6, -0x1.815314c864b09p-9);
+  p = fma_kc(p, z, 0x1.00d47e7966d94p-6);
+  p = fma_kc(p, z, 0x1.b02442413f6bap-7);
+  p = fma_kc(p, z, 0x1.1dc2ef640046fp-6);
+  p = fma_kc(p, z, 0x1.6e72146fda29ep-6);
+  p = fma_kc(p, z, 0x1.f1c81c59ea536p-6);
+  p = fma_kc(p, z, 0x1.6db6d8e71341bp-5);
+  p = fma_kc(p, z, 0x1.33333335a9cd6p-4);
+  p = fma_kc(p, z, 0x1.5555555554f05p-3);
   return fma(s * z, p, s);
 }
 
