@@ -319,6 +319,47 @@ def test_c1_transcendentals(gpu, c1):
     assert np.allclose(np.asarray(got), p, rtol=1e-6)
 
 
+def _pow_inputs():
+    rng = np.random.default_rng(41)
+    bits = rng.integers(0, 2 ** 32, 1 << 18, dtype=np.uint64).astype(np.uint32)
+    special = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1.0, -1.0, 2.0, 0.5, 1e-45, -1e-45,
+                        1.17549435e-38, 3.4028235e38, -3.4028235e38, 1.5, -2.5], dtype=np.float32)
+    return np.concatenate([special, bits.view(np.float32)])
+
+
+@pytest.mark.parametrize("s", [1.0, 2.0, -1.0, 0.0, -0.0, 0.5])
+def test_scalar_pow_fast_paths(gpu, s):
+    """x ** s with a scalar s in {1, 2, -1, 0, 1/2} is one correctly rounded
+    operation: bit for bit the FP64 pow rounded once (double rounding is
+    innocuous for these), over every kind of FP32 input (zeros, infinities,
+    NaN, denormals, negatives); in place too.  The general kernel is within
+    its 0.51-ulp bar of the same values."""
+    x = _pow_inputs()
+    with np.errstate(all="ignore"):
+        exact = x.astype(np.float64) ** np.float64(s)
+    if s == 0.5:
+        # NumPy takes sqrt for ** 0.5; C pow(-0, 0.5) = +0 and pow(-inf, 0.5) = +inf
+        exact[x == 0] = 0.0
+        exact[x == -np.inf] = np.inf
+    want = exact.astype(np.float32)
+    gx = A(gpu, x)
+    assert_bits(gx ** s, want)
+    y = gx * 1.0
+    y **= s
+    assert_bits(y, want)
+    general = np.asarray(gx ** A(gpu, np.full(x.shape, s, dtype=np.float32)))
+    fin = np.isfinite(want) & (want != 0)
+    assert ulp_err(general[fin], exact[fin]).max() <= 0.51
+    assert_bits(general[~fin], want[~fin])
+
+
+@pytest.mark.parametrize("s", [1.5, -2.0])
+def test_scalar_pow_general_equals_array_exponent(gpu, s):
+    x = _pow_inputs()
+    gx = A(gpu, x)
+    assert_bits(gx ** s, gx ** A(gpu, np.full(x.shape, s, dtype=np.float32)))
+
+
 # every unary function: the domain each computes on, the input's map from the
 # C1 operand A ~ U[-1, 1), and the FP64 reference
 UNARY = {

@@ -89,6 +89,15 @@ __host__ __device__ inline double rsqrt_pos(double x) {
   return fma(0.5 * y, e, y);
 }
 
+// sqrt x for finite x that is 0 or not below the smallest normal double:
+// x * rsqrt_pos(x) (relative error below 2^-49), x for +-0, NaN for negative
+// x and NaN.  The FP64 square root the compiler emits for sqrt() costs twice
+// the instructions (scaling for denormals and its own refinement); the
+// callers below never pass a positive denormal or an infinity.
+__host__ __device__ inline double sqrt_nonneg(double x) {
+  return x > 0.0 ? x * rsqrt_pos(x) : (x == 0.0 ? x : NAN);
+}
+
 // ---- FP64 fused multiply-adds with a constant operand ---------------------------
 // A 64-bit constant is not an inline operand of a vector instruction, so the
 // compiler copies it into vector registers, two moves per use, which cost as
@@ -276,7 +285,7 @@ __host__ __device__ inline float asinh_f32(float x) {
     return (float)(d * fma(z, fma(z, fma(z, -5.0 / 112.0, 3.0 / 40.0), -1.0 / 6.0), 1.0));
   }
   if (!(a < INFINITY)) return x;  // inf, NaN
-  return copysignf((float)log_pos(a + sqrt(fma(a, a, 1.0))), x);
+  return copysignf((float)log_pos(a + sqrt_nonneg(fma(a, a, 1.0))), x);
 }
 
 // acosh (x >= 1): log(x + sqrt(x^2 - 1)); x^2 - 1 is exact in FP64 for FP32
@@ -286,7 +295,7 @@ __host__ __device__ inline float acosh_f32(float x) {
   const double d = (double)x;
   if (!(d >= 1.0)) return NAN;
   if (d == INFINITY) return x;
-  return (float)log_pos(d + sqrt(fma(d, d, -1.0)));
+  return (float)log_pos(d + sqrt_nonneg(fma(d, d, -1.0)));
 }
 
 // atanh (|x| <= 1): |x| <= 0.1716: the series; otherwise
@@ -414,7 +423,7 @@ __host__ __device__ inline float asin_f32(float x) {
   const double d = (double)x, a = fabs(d);
   if (a < TINY) return x;
   const bool inner = a <= 0.5;
-  const double p = asin_half(inner ? a : sqrt(0.5 * (1.0 - a)));
+  const double p = asin_half(inner ? a : sqrt_nonneg(0.5 * (1.0 - a)));
   const double v = inner ? p : PIO2 - 2.0 * p;
   return copysignf((float)v, x);
 }
@@ -424,7 +433,7 @@ __host__ __device__ inline float asin_f32(float x) {
 __host__ __device__ inline float acos_f32(float x) {
   const double d = (double)x, a = fabs(d);
   const bool inner = a <= 0.5;
-  const double p = asin_half(inner ? d : sqrt(0.5 * (1.0 - a)));
+  const double p = asin_half(inner ? d : sqrt_nonneg(0.5 * (1.0 - a)));
   return (float)(inner ? PIO2 - p : (d > 0.0 ? 2.0 * p : PI - 2.0 * p));
 }
 

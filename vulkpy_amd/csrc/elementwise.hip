@@ -38,6 +38,28 @@ struct BinF {
   __device__ __forceinline__ float operator()(float a, float b, float) const { return binop<OP>(a, b); }
 };
 
+// a ** s for the scalar exponents whose power is a single correctly rounded
+// FP32 operation, as NumPy's scalar power does: the FP64 pow rounded once
+// (pow's special cases included; exact ties round to even, which the general
+// exp(b log a) evaluation can only approach) at the cost of that operation.
+// E: 1 -> a, 2 -> a * a, -1 -> 1 / a, 0 -> 1, 5 -> sqrt(a) (pow(-0, 0.5) = +0
+// and pow(-inf, 0.5) = +inf, where sqrt gives -0 and NaN)
+template <int E>
+struct PowScalarF {
+  static constexpr int nin = 2;
+  __device__ __forceinline__ float operator()(float a, float, float) const {
+    if constexpr (E == 1) return a;
+    else if constexpr (E == 2) return a * a;
+    else if constexpr (E == -1) return 1.0f / a;
+    else if constexpr (E == 0) return 1.0f;
+    else {
+      if (a == 0.0f) return 0.0f;
+      if (a == -INFINITY) return INFINITY;
+      return sqrtf(a);
+    }
+  }
+};
+
 struct ClampF {
   static constexpr int nin = 3;
   // GLSL clamp(x, lo, hi) == min(max(x, lo), hi) == numpy.clip
@@ -472,6 +494,14 @@ int vkh_binary_scalar(vkh_gpu* g, int op, const float* a, float sv, float* c, ui
     VKH_REQUIRE(a && c, "binary_scalar: null pointer");
     std::vector<uint64_t> s{n};
     std::vector<InSpec> ins{{a, 0.f, s}, {nullptr, sv, {}}};
+    if (op == VKH_POW && (sv == 1.0f || sv == 2.0f || sv == -1.0f || sv == 0.0f || sv == 0.5f)) {
+      if (sv == 1.0f) run_ew(g, PowScalarF<1>{}, ins, c, s);
+      else if (sv == 2.0f) run_ew(g, PowScalarF<2>{}, ins, c, s);
+      else if (sv == -1.0f) run_ew(g, PowScalarF<-1>{}, ins, c, s);
+      else if (sv == 0.0f) run_ew(g, PowScalarF<0>{}, ins, c, s);
+      else run_ew(g, PowScalarF<5>{}, ins, c, s);
+      return;
+    }
     dispatch_binop<BinRun>(op, g, ins, c, s);
   });
 }
