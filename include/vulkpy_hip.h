@@ -497,6 +497,28 @@ int vkh_regularizer_grad(vkh_gpu* gpu, int kind, const float* w, float* grad,
  * Dense bias gradient (nn/layers.py:126-127) */
 int vkh_bias_grad(vkh_gpu* gpu, const float* dy, float* db, uint64_t rows,
                   uint64_t cols, float beta);
+/* vkh_gemm_bf16io that also writes C's column sums to db[N] with the bits of
+ * vkh_bias_grad(C, db, M, N, beta = 0), from its epilogue (the sums of the
+ * final values over 128-row chunks in row order, then the chunks in order):
+ * the input gradient of a Dense layer and the previous layer's bias gradient
+ * (nn/layers.py:126-141) in one product.  store_c = 0 leaves the FP32 C
+ * unwritten (its BF16 copies and the sums are still produced; beta must be
+ * 0).  No counterpart in the reference.  VKH_INVALID unless
+ * vkh_gemm_colsum_supported says the shape is taken: the matrix-core path, a
+ * configuration (tuned or default) whose waves hold 128-row slabs, no split
+ * k, and vkh_bias_grad chunking [M, N] by 128 rows. */
+int vkh_gemm_colsum_supported(vkh_gpu* gpu, int mode, int trans_a,
+                              int trans_b, uint64_t M, uint64_t N, uint64_t K,
+                              int* ok);
+int vkh_gemm_bf16io_colsum(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
+                           uint64_t M, uint64_t N, uint64_t K, float alpha,
+                           const float* A, uint64_t lda, const float* B,
+                           uint64_t ldb, float beta, float* C, uint64_t ldc,
+                           const float* bias, int act, uint16_t* c_bf16,
+                           const float* aux, uint64_t ldaux, int aux_act,
+                           const uint16_t* a_bf16, const uint16_t* b_bf16,
+                           uint16_t* c_bf16t, uint64_t ldt, float* db,
+                           int store_c);
 /* Backward of a last Dense layer with one output (w: [1, N]) that follows an
  * activation h = act(z) [B, N], in one pass over h, with the results of the
  * separate steps bit for bit (nn/layers.py:126-141 for the layer, 207-210 /

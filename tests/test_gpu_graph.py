@@ -270,6 +270,48 @@ def test_dense1_backward_matches_separate_steps(gpu):
     np.testing.assert_array_equal(np.asarray(dxbft).view(np.uint16).reshape(N, B), bf.T)
 
 
+@pytest.mark.parametrize("mode", ["fp32", "bf16"])
+def test_gemm_colsum_matches_separate_steps(gpu, mode):
+    """vkh_gemm_bf16io_colsum: C as vkh_gemm_ex gives it and C's column sums
+    as vkh_bias_grad gives them, bit for bit; with store_c = 0 the FP32 C is
+    left as it was.  2048 x 32768 x 64 with a ReLU' factor: the shape's bias
+    gradient sums 128-row chunks, as the epilogue does."""
+    M, N, K = 2048, 32768, 64
+    h = gpu._handle
+    md = _lib.GEMM_MODE[mode]
+    ok = ctypes.c_int(0)
+    _lib.call("vkh_gemm_colsum_supported", h, md, 0, 0, M, N, K, ctypes.byref(ok))
+    assert ok.value == 1
+    rng = np.random.default_rng(37)
+    A = vk.Array(gpu, data=rng.standard_normal((M, K)).astype(np.float32))
+    B = vk.Array(gpu, data=rng.standard_normal((K, N)).astype(np.float32))
+    aux = vk.Array(gpu, data=np.maximum(rng.standard_normal((M, N)), 0).astype(np.float32))
+    ref, db_ref = vk.Array(gpu, shape=(M, N)), vk.Array(gpu, shape=(N,))
+    _lib.call("vkh_gemm_ex", h, md, 0, 0, M, N, K, 1.0, A._ptr, K, B._ptr, N, 0.0, ref._ptr, N,
+              None, 0, None, aux._ptr, N, 1)
+    _lib.call("vkh_bias_grad", h, ref._ptr, db_ref._ptr, M, N, 0.0)
+    for store in (1, 0):
+        C = vk.Array(gpu, data=np.full((M, N), 7.0, dtype=np.float32))
+        db = vk.Array(gpu, shape=(N,))
+        cbft = vk.U32Array(gpu, shape=(M * N // 2,))
+        _lib.call("vkh_gemm_bf16io_colsum", h, md, 0, 0, M, N, K, 1.0, A._ptr, K, B._ptr, N, 0.0,
+                  C._ptr, N, None, 0, None, aux._ptr, N, 1, None, None, cbft._ptr, M, db._ptr, store)
+        for out in (C, db, cbft):
+            out._set_job([A, B, aux])   # written on the device: the host mirrors are stale
+        gpu.wait()
+        np.testing.assert_array_equal(np.asarray(db).view(np.uint32), np.asarray(db_ref).view(np.uint32))
+        got = np.asarray(C)
+        if store:
+            np.testing.assert_array_equal(got.view(np.uint32), np.asarray(ref).view(np.uint32))
+        else:
+            assert np.all(got == 7.0)
+        # the transposed BF16 copy: round to nearest even of the FP32 values
+        u = np.asarray(ref).view(np.uint32).astype(np.uint64)
+        bf = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+        np.testing.assert_array_equal(np.asarray(cbft).view(np.uint16).reshape(N, M), bf.T)
+        del C, db, cbft, got
+
+
 def test_graph_capture_survives_finalizers(gpu, monkeypatch):
     """Objects collected while the step is captured (an array whose upload
     may still be queued, with a pinned mirror, and a generator) release

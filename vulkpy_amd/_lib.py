@@ -143,6 +143,10 @@ _SIGNATURES = {
     "vkh_sgd": (c_int, [P, P, P, U64, c_float, c_int, c_float, c_float]),
     "vkh_regularizer_grad": (c_int, [P, c_int, P, P, U64, c_float, c_float, c_int]),
     "vkh_dense1_backward": (c_int, [P, P, P, P, c_int, U64, U64, P, P, P, U64, P, P]),
+    "vkh_gemm_colsum_supported": (c_int, [P, c_int, c_int, c_int, U64, U64, U64, POINTER(c_int)]),
+    "vkh_gemm_bf16io_colsum": (c_int, [P, c_int, c_int, c_int, U64, U64, U64, c_float, P, U64, P, U64,
+                                       c_float, P, U64, P, c_int, P, P, U64, c_int, P, P, P, U64, P,
+                                       c_int]),
     "vkh_dense1_backward_supported": (c_int, [P, U64, U64, POINTER(c_int)]),
     "vkh_adagrad": (c_int, [P, P, P, P, U64, c_float, c_float, c_int, c_float, c_float]),
     "vkh_bias_grad": (c_int, [P, P, P, U64, U64, c_float]),

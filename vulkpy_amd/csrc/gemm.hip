@@ -52,7 +52,13 @@ struct GemmArgs {
   uint32_t tiles_m, tiles_n;
   uint16_t* Cbft;  // C^T as BF16: Cbft[n * ldt + m] (optional)
   uint64_t ldt;
+  // column sums of the final C over each chunk of CS_ROWS rows, in row order
+  // from the chunk's first row: cs[(m / CS_ROWS) * N + n] (optional; staged
+  // epilogues with 128-row wave slabs only)
+  float* cs;
+  int no_c;  // 1: the FP32 C is not stored (only Cbf / Cbft / cs)
 };
+constexpr int CS_ROWS = 128;
 
 __device__ __forceinline__ float act_apply(int act, float v) {
   if (act == VKH_ACT_RELU) return (v >= 0.0f || v != v) ? v : 0.0f;  // np.maximum(v, 0)
@@ -299,7 +305,7 @@ __device__ __forceinline__ float4 epilogue4(const GemmArgs& e, uint64_t m, uint6
     v[q] = t;
   }
   if (full) {
-    *reinterpret_cast<float4*>(cp) = make_float4(v[0], v[1], v[2], v[3]);
+    if (!e.no_c) *reinterpret_cast<float4*>(cp) = make_float4(v[0], v[1], v[2], v[3]);
     if (e.Cbf) {
       uint2 h;
       h.x = f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
@@ -310,7 +316,7 @@ __device__ __forceinline__ float4 epilogue4(const GemmArgs& e, uint64_t m, uint6
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       if (q < cnt) {
-        cp[q] = v[q];
+        if (!e.no_c) cp[q] = v[q];
         if (e.Cbf) e.Cbf[m * e.ldc + n + q] = f2bf(v[q]);
       }
     }
@@ -360,6 +366,10 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
   const uint64_t nb = n0 + wn * WN;
   const bool tr = e.Cbft && !part;
   const bool tr_vec = e.ldt % 8 == 0 && ((uintptr_t)e.Cbft & 15u) == 0;
+  // column sums: the wave's WM = 128 rows are one chunk; lane c < WN walks
+  // column c down the staged slices in row order, 32 rows per pass
+  const bool csum = WM == CS_ROWS && e.cs && !part;
+  float csacc = 0.f;
 #pragma unroll
   for (int i = 0; i < TM; ++i) {
 #pragma unroll
@@ -378,11 +388,19 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
           store_part4(part, e.N, m, n, v);
         } else {
           const float4 f = epilogue4(e, m, n, v, false);
-          if (tr) *reinterpret_cast<float4*>(st + rr * SROW + cc) = f;
+          if (tr || csum) *reinterpret_cast<float4*>(st + rr * SROW + cc) = f;
         }
       }
     }
     __syncthreads();
+    if (csum && lane < WN) {
+      for (int rr = 0; rr < 32; ++rr) {
+        if (mb + rr >= e.M) break;
+        const float v = st[rr * SROW + lane];
+        csacc = (i == 0 && rr == 0) ? v : csacc + v;
+      }
+    }
+    if (csum && !tr) __syncthreads();  // the slice is read before the next pass writes it
     if (tr) {
       // four lanes per column, 8 rows (16 bytes) each: one store covers 16
       // columns in whole 64-byte runs
@@ -405,6 +423,10 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
       }
       __syncthreads();
     }
+  }
+  if (csum && lane < WN) {
+    const uint64_t m = m0 + wm * WM, n = nb + lane;
+    if (m < e.M && n < e.N) e.cs[(m / CS_ROWS) * e.N + n] = csacc;
   }
 }
 
@@ -633,14 +655,14 @@ static void launch_f32(dim3 grid, hipStream_t s, const GemmArgs& a, bool ta, boo
 #undef VKH_F32_LAUNCH
 }
 struct F32Cfg {
-  int bm, bn, waves;
+  int bm, bn, waves, wm;  // wm: rows of a wave's slab (BM / WGM)
   F32Launch launch;
 };
 static const F32Cfg F32_CFGS[] = {
-    {128, 128, 4, launch_f32<128, 128, 2, 2>},
-    {256, 128, 8, launch_f32<256, 128, 4, 2>},
-    {128, 256, 8, launch_f32<128, 256, 2, 4>},
-    {256, 256, 8, launch_f32<256, 256, 2, 4>},
+    {128, 128, 4, 64, launch_f32<128, 128, 2, 2>},
+    {256, 128, 8, 64, launch_f32<256, 128, 4, 2>},
+    {128, 256, 8, 64, launch_f32<128, 256, 2, 4>},
+    {256, 256, 8, 128, launch_f32<256, 256, 2, 4>},
 };
 constexpr int F32_NCFG = sizeof(F32_CFGS) / sizeof(F32_CFGS[0]);
 
@@ -946,25 +968,25 @@ static void launch_direct(dim3 grid, hipStream_t s, const PackedArgs& pa) {
   hipLaunchKernelGGL((k_gemm_direct<BM, BN, WGM, WGN>), grid, dim3(64 * WGM * WGN), 0, s, pa);
 }
 struct PackedCfg {
-  int bm, bn, bk, waves;
+  int bm, bn, bk, waves, wm;  // wm: rows of a wave's slab (BM / WGM)
   PackedLaunch launch;
   int direct = 0;  // operand slices loaded from global memory straight into LDS
 };
 static const PackedCfg BF16_CFGS[] = {
-    {256, 256, 64, 4, launch_packed<256, 256, 64, false, 2, 2>},
-    {256, 256, 64, 8, launch_packed<256, 256, 64, false, 2, 4>},
-    {128, 128, 64, 4, launch_packed<128, 128, 64, false, 2, 2>},
-    {256, 128, 64, 4, launch_packed<256, 128, 64, false, 2, 2>},
-    {128, 256, 64, 4, launch_packed<128, 256, 64, false, 2, 2>},
-    {256, 256, 64, 8, launch_direct<256, 256, 2, 4>, 1},
+    {256, 256, 64, 4, 128, launch_packed<256, 256, 64, false, 2, 2>},
+    {256, 256, 64, 8, 128, launch_packed<256, 256, 64, false, 2, 4>},
+    {128, 128, 64, 4, 64, launch_packed<128, 128, 64, false, 2, 2>},
+    {256, 128, 64, 4, 128, launch_packed<256, 128, 64, false, 2, 2>},
+    {128, 256, 64, 4, 64, launch_packed<128, 256, 64, false, 2, 2>},
+    {256, 256, 64, 8, 128, launch_direct<256, 256, 2, 4>, 1},
     // (direct loads with 4 waves of 128 x 128, a third less LDS fragment
     // traffic, one wave per SIMD: 894 against 977 TFLOP/s at 8192^3 and
     // slower on every MLP shape, so not kept)
 };
 static const PackedCfg SPLIT_CFGS[] = {
-    {128, 128, 32, 4, launch_packed<128, 128, 32, true, 2, 2>},
-    {256, 128, 32, 8, launch_packed<256, 128, 32, true, 4, 2>},
-    {128, 128, 32, 8, launch_packed<128, 128, 32, true, 2, 4>},
+    {128, 128, 32, 4, 64, launch_packed<128, 128, 32, true, 2, 2>},
+    {256, 128, 32, 8, 64, launch_packed<256, 128, 32, true, 4, 2>},
+    {128, 128, 32, 8, 64, launch_packed<128, 128, 32, true, 2, 4>},
 };
 constexpr int PK_MAXTILE = 256;  // largest bm / bn of any configuration
 
@@ -1571,7 +1593,8 @@ static void gemm_impl(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M
                       uint64_t ldb, float beta, float* Cp, uint64_t ldc, const float* bias, int act,
                       uint16_t* c_bf16, const float* aux, uint64_t ldaux, int aux_act,
                       const uint16_t* a_bf16 = nullptr, const uint16_t* b_bf16 = nullptr,
-                      uint16_t* c_bf16t = nullptr, uint64_t ldt = 0) {
+                      uint16_t* c_bf16t = nullptr, uint64_t ldt = 0, float* cs = nullptr,
+                      int no_c = 0) {
   use(g);
   if (M == 0 || N == 0) return;
   VKH_REQUIRE(Cp, "gemm: null C");
@@ -1592,6 +1615,8 @@ static void gemm_impl(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M
   a.alpha = alpha; a.beta = beta; a.act = act;
   a.aux = aux; a.ldaux = ldaux; a.aux_act = aux_act;
   a.Cbft = c_bf16t; a.ldt = ldt;
+  a.cs = cs; a.no_c = no_c;
+  VKH_REQUIRE(!no_c || beta == 0.0f, "gemm: beta * C needs C");
   a.vec_c = ((uintptr_t)Cp & 15u) == 0 && ldc % 4 == 0 && (!bias || ((uintptr_t)bias & 15u) == 0) &&
             (!aux || (((uintptr_t)aux & 15u) == 0 && ldaux % 4 == 0)) &&
             (!c_bf16 || ((uintptr_t)c_bf16 & 7u) == 0);
@@ -1793,6 +1818,49 @@ int vkh_gemm_bf16io(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M, 
   return guarded([&] {
     gemm_impl(g, mode, trans_a, trans_b, M, N, K, alpha, A, lda, B, ldb, beta, Cp, ldc, bias, act,
               c_bf16, aux, ldaux, aux_act, a_bf16, b_bf16, c_bf16t, ldt);
+  });
+}
+
+// Whether an M x N x K product takes C's column sums in its epilogue with
+// vkh_bias_grad's bits: the matrix-core path, a configuration (tuned or
+// default) whose waves hold 128-row slabs, no split k, and bias_grad's column
+// sums chunked by 128 rows for [M, N].
+static bool colsum_ok(vkh_gpu* g, int mode, int ta, int tb, uint64_t M, uint64_t N, uint64_t K) {
+  if (std::min(M, N) <= SKINNY_MAX || K <= SMALLK_MAX) return false;
+  if (mode < VKH_GEMM_FP32 || mode > VKH_GEMM_BF16X3) return false;
+  if (bias_grad_chunk(g, M, N, N % 4 == 0) != (uint64_t)CS_ROWS) return false;
+  auto it = g->gemm_cfg.find(cfg_key(mode, ta, tb, M, N, K));
+  const int cfg = it != g->gemm_cfg.end() ? it->second : default_cfg(mode, M, N, K);
+  if (mode == VKH_GEMM_FP32)
+    return F32_CFGS[cfg].wm == CS_ROWS && ceil_div(K, f32_splitk_chunk(M, N, K)) == 1;
+  int n = 0;
+  const PackedCfg* c = packed_cfgs(mode, n);
+  const uint64_t Kp = ceil_div(K, (uint64_t)PK_KALIGN) * PK_KALIGN;
+  return c[cfg].wm == CS_ROWS && ceil_div(Kp, splitk_chunk(M, N, Kp)) == 1;
+}
+
+int vkh_gemm_colsum_supported(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M, uint64_t N,
+                              uint64_t K, int* ok) {
+  return guarded([&] {
+    VKH_REQUIRE(g && ok, "gemm_colsum_supported: null argument");
+    *ok = colsum_ok(g, mode, trans_a, trans_b, M, N, K) ? 1 : 0;
+  });
+}
+
+int vkh_gemm_bf16io_colsum(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M, uint64_t N,
+                           uint64_t K, float alpha, const float* A, uint64_t lda, const float* B, uint64_t ldb,
+                           float beta, float* Cp, uint64_t ldc, const float* bias, int act, uint16_t* c_bf16,
+                           const float* aux, uint64_t ldaux, int aux_act, const uint16_t* a_bf16,
+                           const uint16_t* b_bf16, uint16_t* c_bf16t, uint64_t ldt, float* db, int store_c) {
+  return guarded([&] {
+    use(g);
+    VKH_REQUIRE(db, "gemm_colsum: null db");
+    VKH_REQUIRE(colsum_ok(g, mode, trans_a, trans_b, M, N, K), "gemm_colsum: shape or configuration not supported");
+    const uint64_t nchunk = ceil_div(M, (uint64_t)CS_ROWS);
+    Workspace ws(g, sizeof(float) * nchunk * N);
+    gemm_impl(g, mode, trans_a, trans_b, M, N, K, alpha, A, lda, B, ldb, beta, Cp, ldc, bias, act, c_bf16, aux,
+              ldaux, aux_act, a_bf16, b_bf16, c_bf16t, ldt, (float*)ws.get(), store_c ? 0 : 1);
+    sum_chunks(g, (const float*)ws.get(), db, N, nchunk);
   });
 }
 

@@ -265,6 +265,8 @@ class _Plan:
                     continue
                 if (is_w and fused) or (not is_w and self.fuse_last and i == last - 1):
                     continue  # taken by vkh_dense1_backward
+                if not is_w and (i + 1) in self.colsum:
+                    continue  # taken by the next unit's input-gradient product
                 # the regularizer term is added by the optimizer kernel
                 beta = 0.0
                 if is_w:
@@ -287,11 +289,16 @@ class _Plan:
             elif i > 0:
                 prev = units[i - 1]
                 out_bf, dy_bf, out_bft = self.ds_bf[i - 1], self.ds_bf[i], self.ds_bft[i - 1]
-                call("vkh_gemm_bf16io", h, self.mode_dx, 0, 0, B, d.input_dim, d.output_dim, 1.0, dy._ptr,
-                     d.output_dim, d.w.value._ptr, d.input_dim, 0.0, self.ds[i - 1]._ptr,
-                     d.input_dim, None, 0, out_bf.ptr if out_bf else None,
-                     self.hs[i - 1]._ptr if prev.act else None, d.input_dim, ACT[prev.act],
-                     dy_bf.ptr if dy_bf else None, None, out_bft.ptr if out_bft else None, B)
+                args = (h, self.mode_dx, 0, 0, B, d.input_dim, d.output_dim, 1.0, dy._ptr,
+                        d.output_dim, d.w.value._ptr, d.input_dim, 0.0, self.ds[i - 1]._ptr,
+                        d.input_dim, None, 0, out_bf.ptr if out_bf else None,
+                        self.hs[i - 1]._ptr if prev.act else None, d.input_dim, ACT[prev.act],
+                        dy_bf.ptr if dy_bf else None, None, out_bft.ptr if out_bft else None, B)
+                if i in self.colsum:
+                    # with the bias gradient of unit i - 1 from its epilogue
+                    call("vkh_gemm_bf16io_colsum", *args, prev.dense.b.grad._ptr, int(self.colsum[i]))
+                else:
+                    call("vkh_gemm_bf16io", *args)
         # optimizer: beta powers once per group, then one kernel per
         # parameter (regularizer term, gradient write-back and update)
         for buf, beta1, beta2 in self.bpow_groups:
@@ -328,6 +335,34 @@ class _Plan:
         for d in sorted({u.dense.output_dim for u in self.units if u.dense.b.grad is not None}):
             yield 1, B, d                                 # bias gradient (vkh_bias_grad)
 
+    def _plan_colsums(self):
+        """Input-gradient products that also give the previous layer's bias
+        gradient (vkh_gemm_bf16io_colsum), decided once the GEMM
+        configurations are chosen; and whether their FP32 output is read at
+        all (not when every consumer takes its BF16 copies)."""
+        self.colsum = {}
+        B, units = self.B, self.units
+        bf16 = GEMM_MODE["bf16"]
+        for i in range(1, len(units)):
+            if self.fuse_last and i == len(units) - 1:
+                continue
+            d, prev = units[i].dense, units[i - 1].dense
+            if prev.b.grad is None:
+                continue
+            ok = ctypes.c_int(0)
+            call("vkh_gemm_colsum_supported", self.h, self.mode_dx, 0, 0, B, d.input_dim, d.output_dim,
+                 ctypes.byref(ok))
+            if not ok.value:
+                continue
+            # FP32 dY of unit i - 1 is read by its weight gradient unless that
+            # takes the transposed BF16 copy, and by its own input gradient
+            # (unit i - 1 > 0) unless that takes the BF16 row copy
+            dw_bf = (prev.w.grad is None or (self.mode_dw == bf16 and self.ds_bft[i - 1] is not None
+                                             and B % 64 == 0))
+            dx_bf = i - 1 == 0 or (self.mode_dx == bf16 and self.ds_bf[i - 1] is not None
+                                   and prev.output_dim % 64 == 0)
+            self.colsum[i] = not (dw_bf and dx_bf)
+
     def _capture(self):
         # kernel configuration (when tuning) and the graph's GEMM packing
         # region for every product of the step
@@ -340,6 +375,7 @@ class _Plan:
                 # only configurations with the default summation order, so
                 # the step's results never depend on the timings
                 call("vkh_reduce_autotune", self.h, 1, prev, K, post, 3, 1, None, None)
+        self._plan_colsums()
         self.gpu.wait()
         if not self.seq.capture:
             return
