@@ -34,6 +34,10 @@ def make_net(gpu, dims, acts, seed, reg=None, opt="adam", precision=None):
     return layers
 
 
+def dense(layers):
+    return [L for L in layers if isinstance(L, nn.Dense)]
+
+
 def weights(layers):
     return [np.asarray(p).copy() for L in layers if isinstance(L, nn.Dense) for p in (L.w.value, L.b.value)]
 
@@ -310,6 +314,37 @@ def test_gemm_colsum_matches_separate_steps(gpu, mode):
         bf = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
         np.testing.assert_array_equal(np.asarray(cbft).view(np.uint16).reshape(N, M), bf.T)
         del C, db, cbft, got
+
+
+def test_graph_sees_inputs_and_weights_changed_between_steps(gpu):
+    """The step skips copying an input that is the array it copied last time
+    and unchanged: changing the input in place, or weights outside the step,
+    between replays must reach the next step (eager and captured stay equal)."""
+    dims, acts = [(64, 256), (256, 128), (128, 1)], ["relu", "relu", None]
+    B = 512
+    rng = np.random.default_rng(43)
+    X = vk.Array(gpu, data=rng.standard_normal((B, 64)).astype(np.float32))
+    Y = vk.Array(gpu, data=rng.standard_normal((B, 1)).astype(np.float32))
+    eager_layers = make_net(gpu, dims, acts, 80, lambda: nn.Ridge(1e-4), precision="bf16")
+    graph_layers = make_net(gpu, dims, acts, 80, lambda: nn.Ridge(1e-4), precision="bf16")
+    eager = nn.Sequence(eager_layers, nn.HuberLoss(), regularize=True)
+    graph = nn.GraphSequence(graph_layers, nn.HuberLoss(), precision="bf16", regularize=True)
+    for step in range(5):
+        _, le = eager.train(X, Y)
+        _, lg = graph.train(X, Y)
+        gpu.wait()
+        assert np.asarray(le).item() == np.asarray(lg).item(), step
+        for a, b in zip(weights(eager_layers), weights(graph_layers)):
+            np.testing.assert_array_equal(a, b)
+        if step == 1:
+            X *= 1.5                                  # same array, changed in place
+        if step == 2:
+            for layers in (eager_layers, graph_layers):
+                dense(layers)[0].w.value *= 0.75      # weights changed outside the step
+        if step == 3:
+            for layers in (eager_layers, graph_layers):
+                dense(layers)[1].w.value[3, 7] = -0.25  # through the host mirror
+                dense(layers)[2].w.value[0, 0] = 0.5
 
 
 def test_graph_capture_survives_finalizers(gpu, monkeypatch):

@@ -138,6 +138,7 @@ namespace vkh {
 // a [rows, cols] operand, and the in-order combine of chunk partials.
 uint64_t bias_grad_chunk(vkh_gpu* g, uint64_t rows, uint64_t cols, bool vec);
 void sum_chunks(vkh_gpu* g, const float* part, float* out, uint64_t m, uint64_t nchunk);
+void sum_chunks_from_zero(vkh_gpu* g, const float* part, float* out, uint64_t m, uint64_t nchunk);
 
 inline void use(vkh_gpu* g) {
   VKH_REQUIRE(g != nullptr, "null GPU handle");

@@ -1893,23 +1893,8 @@ int vkh_dense1_backward(vkh_gpu* g, const float* dy, const float* w, const float
     hipLaunchKernelGGL(k_dense1_backward, grid, dim3(256), 0, g->stream, dy, w, h, act, N, dx, dx_bf, dx_bft,
                        ldt, pdw, (int)kc_dw, pdb, (int)kc_db);
     check_launch("dense1_backward");
-    if (dw) {  // the column walk's combine: 0 + chunks in order, then 1 * sum
-      SkinnyArgs a{};
-      a.P = N;
-      a.Q = 1;
-      a.swap = 1;
-      a.e.C = dw;
-      a.e.ldc = N;
-      a.e.M = 1;
-      a.e.N = N;
-      a.e.alpha = 1.0f;
-      a.e.beta = 0.0f;
-      a.e.act = VKH_ACT_NONE;
-      a.e.aux_act = VKH_ACT_NONE;
-      hipLaunchKernelGGL(k_skinny_combine, dim3(full_grid(N, 256)), dim3(256), 0, g->stream, a,
-                         (const float*)pdw, ndw);
-      check_launch("dense1_backward dw");
-    }
+    // dw: the column walk's combine (0 + chunks in order; its 1 * sum is the sum)
+    if (dw) sum_chunks_from_zero(g, (const float*)pdw, dw, N, ndw);
     if (db) sum_chunks(g, (const float*)pdb, db, N, ndb);
   });
 }

@@ -527,12 +527,59 @@ uint64_t bias_grad_chunk(vkh_gpu* g, uint64_t rows, uint64_t cols, bool vec) {
   return cols_chunk(g, cfg, 1, rows, cols, vec, nchunk);
 }
 
-// combine of chunk partials [nchunk, m] in chunk order (k_combine_chunks)
+// Chunk partials part[c][m] summed per output in chunk order: FROM_ZERO 0
+// starts from part[0] (k_combine_chunks' order for a sum, no scaling), 1 from
+// +0 (the column walk's combine).  For few outputs and many chunks: a block
+// takes 64 outputs, its four waves load batches of up to 128 chunks (256-byte
+// rows, every load of a thread issued before the first wait) into LDS, and
+// lane j of the first wave adds output j's batch in order.  The same sums as
+// the one-thread-per-output combines, with all their loads in flight at once.
+constexpr int CC_OUT = 64, CC_BATCH = 128, CC_PER = CC_BATCH / 4;
+template <int FROM_ZERO>
+__global__ void __launch_bounds__(256)
+k_combine_cols(const float* __restrict__ part, float* __restrict__ out, uint64_t m, uint64_t nchunk) {
+  __shared__ float tile[CC_BATCH][CC_OUT + 1];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t j = (uint64_t)blockIdx.x * CC_OUT + lane;
+  float acc = 0.f;
+  for (uint64_t c0 = 0; c0 < nchunk; c0 += CC_BATCH) {
+    float r[CC_PER];
+#pragma unroll
+    for (int q = 0; q < CC_PER; ++q) {
+      const uint64_t c = c0 + w + 4 * q;
+      r[q] = (c < nchunk && j < m) ? part[c * m + j] : 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < CC_PER; ++q) tile[w + 4 * q][lane] = r[q];
+    __syncthreads();
+    if (w == 0) {
+      const int nb = (int)min<uint64_t>(CC_BATCH, nchunk - c0);
+      for (int c = 0; c < nb; ++c) {
+        const float v = tile[c][lane];
+        acc = (!FROM_ZERO && c0 == 0 && c == 0) ? v : acc + v;
+      }
+    }
+    __syncthreads();
+  }
+  if (w == 0 && j < m) out[j] = acc;
+}
+
+static void combine_cols(vkh_gpu* g, const float* part, float* out, uint64_t m, uint64_t nchunk, bool from_zero) {
+  const unsigned grid = (unsigned)ceil_div(m, (uint64_t)CC_OUT);
+  if (from_zero) hipLaunchKernelGGL(k_combine_cols<1>, dim3(grid), dim3(256), 0, g->stream, part, out, m, nchunk);
+  else hipLaunchKernelGGL(k_combine_cols<0>, dim3(grid), dim3(256), 0, g->stream, part, out, m, nchunk);
+  check_launch("combine chunks");
+}
+
+// combine of chunk partials [nchunk, m] in chunk order (k_combine_chunks' sums)
 void sum_chunks(vkh_gpu* g, const float* part, float* out, uint64_t m, uint64_t nchunk) {
-  const unsigned cus = g->num_cus > 0 ? g->num_cus : 256;
-  unsigned gb = (unsigned)std::min<uint64_t>(ceil_div(m, 256), (uint64_t)cus * 16);
-  hipLaunchKernelGGL(k_combine_chunks<VKH_RSUM>, dim3(gb), dim3(256), 0, g->stream, part, out, m, nchunk, 1.0f, 1.0f);
-  check_launch("sum_chunks");
+  combine_cols(g, part, out, m, nchunk, false);
+}
+
+// the same from +0 (the skinny column walk's combine, k_skinny_combine with
+// alpha 1, beta 0 and no epilogue)
+void sum_chunks_from_zero(vkh_gpu* g, const float* part, float* out, uint64_t m, uint64_t nchunk) {
+  combine_cols(g, part, out, m, nchunk, true);
 }
 }  // namespace vkh
 
