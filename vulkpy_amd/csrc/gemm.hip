@@ -16,6 +16,7 @@
 // Blocks are remapped so that consecutive tiles land on the same XCD (each of
 // the 8 XCDs has its own L2) and grouped along M for L2 reuse of B.
 #include "common.h"
+#include "fastdiv.h"
 
 namespace vkh {
 
@@ -138,24 +139,35 @@ __device__ __forceinline__ void pack8(const float v[8], uint16_t* hi, uint16_t* 
   }
 }
 
+// Row-major source: 4 consecutive elements a thread, one thread per 4 of
+// the padded [Rp][Kp] (Rp * Kp / 4 < 2^31: the row from one 32-bit
+// multiply-high division), so that each load instruction reads one contiguous
+// 1 KB run and each store writes 512 B.
+__device__ __forceinline__ uint32_t bf2x(float a, float b) { return f2bf(a) | ((uint32_t)f2bf(b) << 16); }
+
 template <bool SPLIT>
 __global__ void __launch_bounds__(256)
 k_pack_rows(const float* src, uint64_t ld, uint64_t R, uint64_t K, uint16_t* hi, uint16_t* lo,
-            uint64_t Rp, uint64_t Kp, int vec) {
-  const uint64_t kc = Kp / 8, total = Rp * kc;
-  const uint64_t stride = (uint64_t)gridDim.x * 256;
-  for (uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += stride) {
-    const uint64_t r = t / kc, k = (t - r * kc) * 8;
-    float v[8];
-    if (r < R && vec && k + 7 < K) {
-      const float4 a = *reinterpret_cast<const float4*>(src + r * ld + k);
-      const float4 b = *reinterpret_cast<const float4*>(src + r * ld + k + 4);
-      v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
-    } else {
+            uint32_t total4, FastDiv kq, uint64_t Kp, int vec) {
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= total4) return;
+  const uint32_t r = kq.div(t);
+  const uint64_t k = (uint64_t)(t - r * kq.d) * 4;
+  float v[4];
+  if (r < R && vec && k + 3 < K) {
+    const float4 a = ld_stream(reinterpret_cast<const float4*>(src + r * ld + k));
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  } else {
 #pragma unroll
-      for (int q = 0; q < 8; ++q) v[q] = (r < R && k + q < K) ? src[r * ld + k + q] : 0.f;
-    }
-    pack8<SPLIT>(v, hi + r * Kp + k, SPLIT ? lo + r * Kp + k : nullptr);
+    for (int q = 0; q < 4; ++q) v[q] = (r < R && k + q < K) ? src[r * ld + k + q] : 0.f;
+  }
+  const uint64_t off = (uint64_t)r * Kp + k;
+  *reinterpret_cast<uint2*>(hi + off) = make_uint2(bf2x(v[0], v[1]), bf2x(v[2], v[3]));
+  if constexpr (SPLIT) {
+    float l[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) l[q] = v[q] - bf2f(f2bf(v[q]));
+    *reinterpret_cast<uint2*>(lo + off) = make_uint2(bf2x(l[0], l[1]), bf2x(l[2], l[3]));
   }
 }
 
@@ -1098,9 +1110,10 @@ static void pack_operand(vkh_gpu* g, const float* src, uint64_t ld, bool trans, 
                          uint64_t K, uint16_t* hi, uint16_t* lo, uint64_t Rp, uint64_t Kp) {
   if (!trans) {
     const int vec = ((uintptr_t)src & 15u) == 0 && ld % 4 == 0;
-    unsigned grid = full_grid(Rp * (Kp / 8), 256);
-    hipLaunchKernelGGL((k_pack_rows<SPLIT>), dim3(grid), dim3(256), 0, g->stream, src, ld, R, K, hi,
-                       lo, Rp, Kp, vec);
+    const uint64_t total4 = Rp * (Kp / 4);
+    VKH_REQUIRE(total4 < (uint64_t(1) << 31), "gemm pack: operand too large");
+    hipLaunchKernelGGL((k_pack_rows<SPLIT>), dim3((unsigned)ceil_div(total4, 256)), dim3(256), 0, g->stream,
+                       src, ld, R, K, hi, lo, (uint32_t)total4, FastDiv((uint32_t)(Kp / 4)), Kp, vec);
   } else {
     const int vec = ((uintptr_t)src & 15u) == 0 && ld % 4 == 0;
     VKH_REQUIRE(Rp % PK_TR == 0, "gemm pack: padded rows not a multiple of the tile");
