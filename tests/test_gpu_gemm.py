@@ -107,6 +107,37 @@ def test_skinny_paths(gpu, ta, tb, M, N, K):
         assert err <= tol, (mode, err)
 
 
+def rowdot_order(X, w):
+    """The row-dot's summation order in FP32: lane l adds x[k] * w[k] for
+    k = 4l + 256i + j (i, then j = 0..3) from 0, then the wave's butterfly
+    (v_l += v_(l xor o) for o = 32, 16, ..., 1); lane 0's value."""
+    P, K = X.shape
+    lanes = np.zeros((P, 64), dtype=np.float32)
+    for i in range(K // 256):
+        for j in range(4):
+            k = np.arange(64) * 4 + 256 * i + j
+            lanes = lanes + (X[:, k] * w[k]).astype(np.float32)
+    o = 32
+    while o >= 1:
+        lanes = lanes + lanes[:, np.arange(64) ^ o]
+        o //= 2
+    return lanes[:, 0]
+
+
+@pytest.mark.parametrize("K", [4096, 1280, 1024, 512])
+def test_rowdot_summation_order(gpu, K):
+    """The one-output row-dot (the MLP's last forward layer) sums in its
+    documented order, whichever kernel takes the shape (weights held in
+    registers for K = 1024..4096, streamed otherwise): bit for bit against a
+    NumPy restatement of that order."""
+    P = 67
+    rng = np.random.default_rng(K)
+    X = rng.standard_normal((P, K)).astype(np.float32)
+    w = rng.standard_normal((1, K)).astype(np.float32)
+    got = run_gemm(gpu, "fp32", 0, 1, X, w)[:, 0]
+    np.testing.assert_array_equal(got.view(np.uint32), rowdot_order(X, w[0]).view(np.uint32))
+
+
 @pytest.mark.parametrize("M,N,K,ta,tb", [(300, 10, 64, 0, 1), (10, 300, 64, 1, 0), (257, 1, 33, 1, 1)])
 def test_skinny_epilogue(gpu, M, N, K, ta, tb):
     rng = np.random.default_rng(4)

@@ -1296,6 +1296,40 @@ __global__ void __launch_bounds__(256) k_skinny_rowdot(SkinnyArgs a, int vec) {
   if (lane < a.Q) skinny_store(a, p, lane, mine);
 }
 
+// k_skinny_rowdot for Q = 1 with 16-byte operands and K = 256 * NK: the
+// column of op(B) (K floats) is loaded into registers once per wave and a
+// wave takes RW rows, so that only X is streamed; every row sums its k in the
+// order of k_skinny_rowdot (lane-strided float4 steps, then the wave's
+// butterfly)
+constexpr int ROWDOT_RW = 4;
+template <int NK>
+__global__ void __launch_bounds__(256) k_skinny_rowdot1(SkinnyArgs a) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t p0 = ((uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * ROWDOT_RW;
+  if (p0 >= a.P) return;  // uniform per wave
+  float4 y[NK];
+#pragma unroll
+  for (int i = 0; i < NK; ++i) y[i] = *reinterpret_cast<const float4*>(a.Y + lane * 4 + i * 256);
+  for (int rw = 0; rw < ROWDOT_RW; ++rw) {
+    const uint64_t p = p0 + rw;
+    if (p >= a.P) break;
+    const float* xr = a.X + p * a.sxp;
+    float4 x[NK];
+#pragma unroll
+    for (int i = 0; i < NK; ++i) x[i] = ld_stream(reinterpret_cast<const float4*>(xr + lane * 4 + i * 256));
+    float acc = 0.f;
+#pragma unroll
+    for (int i = 0; i < NK; ++i) {
+      acc += x[i].x * y[i].x;
+      acc += x[i].y * y[i].y;
+      acc += x[i].z * y[i].z;
+      acc += x[i].w * y[i].w;
+    }
+    const float s = wave_sum64(acc);
+    if (lane == 0) skinny_store(a, p, 0, s);
+  }
+}
+
 // partial sums part[chunk][q][p] over k in [chunk*kchunk, (chunk+1)*kchunk);
 // VW consecutive p per thread (VW = 4: 16-byte loads, P % 4 == 0 and X rows
 // 16-byte aligned)
@@ -1402,6 +1436,17 @@ static void gemm_skinny(vkh_gpu* g, bool ta, bool tb, const GemmArgs& e) {
   if (a.sxk == 1) {
     const int vec = ((uintptr_t)a.X & 15u) == 0 && a.sxp % 4 == 0 && a.K % 4 == 0 && a.syk == 1 &&
                     ((uintptr_t)a.Y & 15u) == 0 && a.syq % 4 == 0;
+    if (qb == 1 && vec && a.K % 256 == 0 && a.K >= 1024 && a.K <= 4096) {
+      const unsigned blocks1 = (unsigned)ceil_div(a.P, (uint64_t)4 * ROWDOT_RW);
+      switch (a.K / 256) {
+#define VKH_ROWDOT1(NKV) case NKV: hipLaunchKernelGGL(k_skinny_rowdot1<NKV>, dim3(blocks1), dim3(256), 0, g->stream, a); break;
+        VKH_ROWDOT1(4) VKH_ROWDOT1(5) VKH_ROWDOT1(6) VKH_ROWDOT1(7) VKH_ROWDOT1(8) VKH_ROWDOT1(9) VKH_ROWDOT1(10)
+        VKH_ROWDOT1(11) VKH_ROWDOT1(12) VKH_ROWDOT1(13) VKH_ROWDOT1(14) VKH_ROWDOT1(15) VKH_ROWDOT1(16)
+#undef VKH_ROWDOT1
+      }
+      check_launch("gemm (skinny row-dot)");
+      return;
+    }
     const uint64_t blocks = ceil_div(a.P, 4);
     VKH_REQUIRE(blocks < (uint64_t(1) << 31), "GEMM too large");
     dim3 grid((unsigned)blocks), block(256);
