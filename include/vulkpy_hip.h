@@ -528,7 +528,8 @@ int vkh_gemm_bf16io_colsum(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
  *                  transposed [N][ldt] (ldt % 8 == 0, 16-byte aligned)
  *   dw = dy^T h                  as vkh_gemm (1 x N x B), written (beta 0)
  *   db = column sums of dx       as vkh_bias_grad with beta 0
- * dw or db NULL: not computed.  No counterpart in the reference.
+ * dx, dw or db NULL: not stored / not computed (db is still the column sums
+ * of the FP32 dx).  No counterpart in the reference.
  * VKH_INVALID unless vkh_dense1_backward_supported(B, N) and the operands
  * are 16-byte aligned (the caller then takes the separate steps). */
 int vkh_dense1_backward(vkh_gpu* gpu, const float* dy, const float* w,

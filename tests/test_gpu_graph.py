@@ -272,6 +272,14 @@ def test_dense1_backward_matches_separate_steps(gpu):
     bf = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
     np.testing.assert_array_equal(np.asarray(dxbf).view(np.uint16).reshape(B, N), bf)
     np.testing.assert_array_equal(np.asarray(dxbft).view(np.uint16).reshape(N, B), bf.T)
+    # without the FP32 dx: the same copies, weight and bias gradients
+    dw2, db2 = vk.Array(gpu, shape=(1, N)), vk.Array(gpu, shape=(N,))
+    dxbf2, dxbft2 = vk.U32Array(gpu, shape=(B * N // 2,)), vk.U32Array(gpu, shape=(B * N // 2,))
+    _lib.call("vkh_dense1_backward", h, dy._ptr, w._ptr, hid._ptr, 1, B, N, None, dxbf2._ptr,
+              dxbft2._ptr, B, dw2._ptr, db2._ptr)
+    gpu.wait()
+    for a, b in ((dw2, dw), (db2, db), (dxbf2, dxbf), (dxbft2, dxbft)):
+        np.testing.assert_array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32))
 
 
 @pytest.mark.parametrize("mode", ["fp32", "bf16"])

@@ -15,6 +15,9 @@
 // bias[n] and an activation, and can also emit a BF16 copy of the result.
 // Blocks are remapped so that consecutive tiles land on the same XCD (each of
 // the 8 XCDs has its own L2) and grouped along M for L2 reuse of B.
+#include <type_traits>
+#include <utility>
+
 #include "common.h"
 #include "fastdiv.h"
 
@@ -25,6 +28,17 @@ typedef __attribute__((ext_vector_type(16))) float f32x16_t;
 // register staging of 8 BF16 values; a native vector (unlike HIP's uint4
 // struct) stays in VGPRs when held in arrays across the K loop
 typedef __attribute__((ext_vector_type(4))) uint32_t u32x4_t;
+
+// f(integral_constant<int, 0>), ..., f(integral_constant<int, N - 1>): a
+// loop unrolled by construction
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_seq(F& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_seq(f, std::make_integer_sequence<int, N>{});
+}
 
 __device__ __forceinline__ uint16_t f2bf(float x) {
   // round to nearest even (NaN kept quiet)
@@ -61,9 +75,12 @@ struct GemmArgs {
 };
 constexpr int CS_ROWS = 128;
 
+// SIG = false: the caller has ruled out the sigmoid (whose FP64 exp is most
+// of the code of an unrolled epilogue)
+template <bool SIG = true>
 __device__ __forceinline__ float act_apply(int act, float v) {
   if (act == VKH_ACT_RELU) return (v >= 0.0f || v != v) ? v : 0.0f;  // np.maximum(v, 0)
-  if (act == VKH_ACT_SIGMOID) return 1.0f / (1.0f + (float)exp((double)(-v)));
+  if (SIG && act == VKH_ACT_SIGMOID) return 1.0f / (1.0f + (float)exp((double)(-v)));
   return v;
 }
 
@@ -266,53 +283,35 @@ struct PackedStager {
   }
 };
 
-// Epilogue of 4 consecutive outputs C[m, n..n+3] from their accumulators, in
-// the order alpha*acc (+ beta*C) (+ bias[n]), activation, times the aux
-// activation's derivative; 16-byte accesses when the whole group is in range.
-// Returns the final values.  The transposed BF16 copy is written here, one
-// element at a time, unless write_t is false (the staged epilogue writes it
-// in whole runs).
-__device__ __forceinline__ float4 epilogue4(const GemmArgs& e, uint64_t m, uint64_t n, float4 acc,
-                                            bool write_t = true) {
-  const bool full = e.vec_c && n + 3 < e.N;
-  const int cnt = full ? 4 : (int)min<uint64_t>(4, e.N - n);
+// 4 consecutive floats p[0 .. cnt) (one 16-byte load when `full`), zeros beyond.
+__device__ __forceinline__ f32x4_t load4(const float* p, bool full, int cnt) {
+  if (full) return *reinterpret_cast<const f32x4_t*>(p);
+  f32x4_t v;
+  v[0] = cnt > 0 ? p[0] : 0.f;
+  v[1] = cnt > 1 ? p[1] : 0.f;
+  v[2] = cnt > 2 ? p[2] : 0.f;
+  v[3] = cnt > 3 ? p[3] : 0.f;
+  return v;
+}
+
+// The final values of 4 consecutive outputs C[m, n..n+3] from their
+// accumulators, given C's old values (beta != 0), bias[n..n+3] and the aux
+// activation's outputs, in the order alpha*acc (+ beta*C) (+ bias[n]),
+// activation, times the aux activation's derivative; stored with 16-byte
+// accesses when `full` (cnt outputs of the group in range otherwise).  The
+// transposed BF16 copy is written here, one element at a time, unless write_t
+// is false (the staged epilogue writes it in whole runs).
+template <bool SIG = true>
+__device__ __forceinline__ float4 finish4(const GemmArgs& e, uint64_t m, uint64_t n, float4 acc, bool full,
+                                          int cnt, f32x4_t c, f32x4_t b, f32x4_t x, bool write_t) {
   float* cp = e.C + m * e.ldc + n;
   float v[4] = {acc.x, acc.y, acc.z, acc.w};
-  float c[4] = {0.f, 0.f, 0.f, 0.f}, b[4] = {0.f, 0.f, 0.f, 0.f}, x[4] = {0.f, 0.f, 0.f, 0.f};
-  if (e.beta != 0.f) {
-    if (full) {
-      const float4 t = *reinterpret_cast<const float4*>(cp);
-      c[0] = t.x; c[1] = t.y; c[2] = t.z; c[3] = t.w;
-    } else {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) c[q] = q < cnt ? cp[q] : 0.f;
-    }
-  }
-  if (e.bias) {
-    if (full) {
-      const float4 t = *reinterpret_cast<const float4*>(e.bias + n);
-      b[0] = t.x; b[1] = t.y; b[2] = t.z; b[3] = t.w;
-    } else {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) b[q] = q < cnt ? e.bias[n + q] : 0.f;
-    }
-  }
-  if (e.aux) {
-    const float* ap = e.aux + m * e.ldaux + n;
-    if (full) {
-      const float4 t = *reinterpret_cast<const float4*>(ap);
-      x[0] = t.x; x[1] = t.y; x[2] = t.z; x[3] = t.w;
-    } else {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) x[q] = q < cnt ? ap[q] : 0.f;
-    }
-  }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     float t = e.alpha * v[q];
     if (e.beta != 0.f) t += e.beta * c[q];
     if (e.bias) t += b[q];
-    t = act_apply(e.act, t);
+    t = act_apply<SIG>(e.act, t);
     if (e.aux) t = act_deriv_mul(e.aux_act, x[q], t);
     v[q] = t;
   }
@@ -339,6 +338,18 @@ __device__ __forceinline__ float4 epilogue4(const GemmArgs& e, uint64_t m, uint6
       if (q < cnt) e.Cbft[(n + q) * e.ldt + m] = f2bf(v[q]);
   }
   return make_float4(v[0], v[1], v[2], v[3]);
+}
+
+// finish4 with its inputs read here (C's old values, bias, aux).
+__device__ __forceinline__ float4 epilogue4(const GemmArgs& e, uint64_t m, uint64_t n, float4 acc,
+                                            bool write_t = true) {
+  const bool full = e.vec_c && n + 3 < e.N;
+  const int cnt = full ? 4 : (int)min<uint64_t>(4, e.N - n);
+  const f32x4_t z = {0.f, 0.f, 0.f, 0.f};
+  const f32x4_t c = e.beta != 0.f ? load4(e.C + m * e.ldc + n, full, cnt) : z;
+  const f32x4_t b = e.bias ? load4(e.bias + n, full, cnt) : z;
+  const f32x4_t x = e.aux ? load4(e.aux + m * e.ldaux + n, full, cnt) : z;
+  return finish4(e, m, n, acc, full, cnt, c, b, x, write_t);
 }
 
 // Raw accumulators C[m, n..n+3] of one k chunk into part [M][N] (16-byte
@@ -382,8 +393,38 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
   // column c down the staged slices in row order, 32 rows per pass
   const bool csum = WM == CS_ROWS && e.cs && !part;
   float csacc = 0.f;
+  // A lane's row segments: columns n .. n+3 of rows r0 + k*RPI of each pass.
+  // In the common case (a whole 4-column group, every row of the pass in
+  // range, no sigmoid) there are no global load waits between the pass's
+  // stores (a wait on a load also waits for every earlier store): bias is
+  // read once, and the aux values of the pass are in registers before its
+  // row pass, read during the previous pass where they fit (ITER <= 8), else
+  // in groups of PF at its start.  Other segments (and the split-k partial
+  // sums) take one loop iteration each, with their own loads.
+  constexpr int ITER = 32 / RPI, PF = ITER < 8 ? ITER : 8;
+  constexpr bool AHEAD = ITER == PF;
+  const int cc = (lane % F4R) * 4, r0 = lane / F4R;
+  const uint64_t n = nb + cc;
+  const bool ncol = n < e.N;
+  const bool lane_fast = !part && e.act != VKH_ACT_SIGMOID && e.vec_c && n + 3 < e.N;
+  const f32x4_t z = {0.f, 0.f, 0.f, 0.f};
+  const f32x4_t bq = lane_fast && e.bias ? *reinterpret_cast<const f32x4_t*>(e.bias + n) : z;
+  const bool aux = lane_fast && e.aux;
+  f32x4_t xq[PF];
 #pragma unroll
-  for (int i = 0; i < TM; ++i) {
+  for (int k = 0; k < PF; ++k) xq[k] = z;
+  auto load_aux = [&](int i, int k0) {
+#pragma unroll
+    for (int k = 0; k < PF; ++k) {
+      const uint64_t m = m0 + wm * WM + i * 32 + r0 + (k0 + k) * RPI;
+      xq[k] = m < e.M ? *reinterpret_cast<const f32x4_t*>(e.aux + m * e.ldaux + n) : z;
+    }
+  };
+  if (aux && AHEAD) load_aux(0, 0);
+  // pass i: rows i*32 .. i*32+31 of the sub-tile (a compile-time i, so the
+  // accumulators stay in registers however large the pass's code)
+  auto pass = [&](auto ic) __attribute__((always_inline)) {
+    constexpr int i = decltype(ic)::value;
 #pragma unroll
     for (int j = 0; j < TN; ++j)
 #pragma unroll
@@ -391,19 +432,34 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
         st[((r / 4) * 8 + lhi * 4 + (r % 4)) * SROW + j * 32 + l32] = acc[i][j][r];
     __syncthreads();
     const uint64_t mb = m0 + wm * WM + i * 32;
-    const int cc = (lane % F4R) * 4;
-    for (int rr = lane / F4R; rr < 32; rr += RPI) {
-      const uint64_t m = mb + rr, n = nb + cc;
-      if (m < e.M && n < e.N) {
-        const float4 v = *reinterpret_cast<const float4*>(st + rr * SROW + cc);
-        if (part) {
-          store_part4(part, e.N, m, n, v);
-        } else {
-          const float4 f = epilogue4(e, m, n, v, false);
+    if (lane_fast && mb + 31 < e.M) {
+#pragma unroll
+      for (int k0 = 0; k0 < ITER; k0 += PF) {
+        if (aux && !AHEAD) load_aux(i, k0);
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+          const int rr = r0 + (k0 + k) * RPI;
+          const float4 v = *reinterpret_cast<const float4*>(st + rr * SROW + cc);
+          const f32x4_t c = e.beta != 0.f ? *reinterpret_cast<const f32x4_t*>(e.C + (mb + rr) * e.ldc + n) : z;
+          const float4 f = finish4<false>(e, mb + rr, n, v, true, 4, c, bq, xq[k], false);
           if (tr || csum) *reinterpret_cast<float4*>(st + rr * SROW + cc) = f;
         }
       }
+    } else {
+      for (int rr = r0; rr < 32; rr += RPI) {
+        const uint64_t m = mb + rr;
+        if (m < e.M && ncol) {
+          const float4 v = *reinterpret_cast<const float4*>(st + rr * SROW + cc);
+          if (part) {
+            store_part4(part, e.N, m, n, v);
+          } else {
+            const float4 f = epilogue4(e, m, n, v, false);
+            if (tr || csum) *reinterpret_cast<float4*>(st + rr * SROW + cc) = f;
+          }
+        }
+      }
     }
+    if (aux && AHEAD && i + 1 < TM) load_aux(i + 1, 0);
     __syncthreads();
     if (csum && lane < WN) {
       for (int rr = 0; rr < 32; ++rr) {
@@ -418,9 +474,9 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
       // columns in whole 64-byte runs
       const int q = lane & 3;
       for (int c = lane >> 2; c < WN; c += 16) {
-        const uint64_t n = nb + c;
-        if (n >= e.N) continue;
-        uint16_t* dst = e.Cbft + n * e.ldt + mb + q * 8;
+        const uint64_t nt = nb + c;
+        if (nt >= e.N) continue;
+        uint16_t* dst = e.Cbft + nt * e.ldt + mb + q * 8;
         const float* col = st + q * 8 * SROW + c;
         if (tr_vec && mb + 31 < e.M) {
           uint32_t w[4];
@@ -435,10 +491,11 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
       }
       __syncthreads();
     }
-  }
+  };
+  static_for<TM>(pass);
   if (csum && lane < WN) {
-    const uint64_t m = m0 + wm * WM, n = nb + lane;
-    if (m < e.M && n < e.N) e.cs[(m / CS_ROWS) * e.N + n] = csacc;
+    const uint64_t m = m0 + wm * WM, nc = nb + lane;
+    if (m < e.M && nc < e.N) e.cs[(m / CS_ROWS) * e.N + nc] = csacc;
   }
 }
 
@@ -1550,7 +1607,7 @@ static void gemm_smallk(vkh_gpu* g, bool ta, bool tb, const GemmArgs& a) {
 // gradient need, with the bits of the separate steps:
 //   dx[b, n] = act'(h[b, n]) * (dy[b] * w[n])    (vkh_gemm, K = 1, aux = h)
 //   dx_bf, dx_bft: dx as BF16, row-major and transposed (the next products'
-//                  packed operands)
+//                  packed operands); the FP32 dx is not stored when dx is null
 //   dw[n]    = sum_b h[b, n] * dy[b]             (the column walk of vkh_gemm
 //                  for dY^T h: 0 + products in k order per chunk of kc_dw rows,
 //                  chunks summed in order from 0)
@@ -1596,7 +1653,7 @@ k_dense1_backward(const float* __restrict__ dy, const float* __restrict__ w, con
       D[r * D1_PITCH + c4 * 4 + q] = o[q];
       H[r * D1_PITCH + c4 * 4 + q] = hq[q];
     }
-    st_stream(reinterpret_cast<float4*>(dx + b * N + n), make_float4(o[0], o[1], o[2], o[3]));
+    if (dx) st_stream(reinterpret_cast<float4*>(dx + b * N + n), make_float4(o[0], o[1], o[2], o[3]));
     if (dx_bf) {
       uint2 hb;
       hb.x = f2bf(o[0]) | ((uint32_t)f2bf(o[1]) << 16);
@@ -1935,10 +1992,10 @@ int vkh_dense1_backward(vkh_gpu* g, const float* dy, const float* w, const float
                         float* db) {
   return guarded([&] {
     use(g);
-    VKH_REQUIRE(dy && w && h && dx, "dense1_backward: null pointer");
+    VKH_REQUIRE(dy && w && h, "dense1_backward: null pointer");
     VKH_REQUIRE(act >= VKH_ACT_NONE && act <= VKH_ACT_SIGMOID, "dense1_backward: unknown activation");
     auto a16 = [](const void* p) { return ((uintptr_t)p & 15u) == 0; };
-    VKH_REQUIRE(a16(w) && a16(h) && a16(dx), "dense1_backward: operands not 16-byte aligned");
+    VKH_REQUIRE(a16(w) && a16(h) && (!dx || a16(dx)), "dense1_backward: operands not 16-byte aligned");
     VKH_REQUIRE(!dx_bf || ((uintptr_t)dx_bf & 7u) == 0, "dense1_backward: dx_bf not 8-byte aligned");
     VKH_REQUIRE(!dx_bft || (a16(dx_bft) && ldt % 8 == 0 && ldt >= B), "dense1_backward: dx_bft layout");
     uint64_t kc_dw = 0, ndw = 0, kc_db = 0;

@@ -284,7 +284,7 @@ class _Plan:
                 out_bf, out_bft = self.ds_bf[i - 1], self.ds_bft[i - 1]
                 pb = prev.dense.b.grad
                 call("vkh_dense1_backward", h, dy._ptr, d.w.value._ptr, self.hs[i - 1]._ptr,
-                     ACT[prev.act], B, d.input_dim, self.ds[i - 1]._ptr,
+                     ACT[prev.act], B, d.input_dim, self.ds[i - 1]._ptr if self.d1_store else None,
                      out_bf.ptr if out_bf else None, out_bft.ptr if out_bft else None, B,
                      d.w.grad._ptr if d.w.grad is not None else None,
                      pb._ptr if pb is not None else None)
@@ -344,9 +344,12 @@ class _Plan:
         all (not when every consumer takes its BF16 copies)."""
         self.colsum = {}
         B, units = self.B, self.units
-        bf16 = GEMM_MODE["bf16"]
+        last = len(units) - 1
+        # the fused last-layer backward also gives the hidden layer's bias
+        # gradient, so its FP32 output is only for the products
+        self.d1_store = self.fuse_last and self._dy_fp32_read(last - 1)
         for i in range(1, len(units)):
-            if self.fuse_last and i == len(units) - 1:
+            if self.fuse_last and i == last:
                 continue
             d, prev = units[i].dense, units[i - 1].dense
             if prev.b.grad is None:
@@ -356,14 +359,18 @@ class _Plan:
                  ctypes.byref(ok))
             if not ok.value:
                 continue
-            # FP32 dY of unit i - 1 is read by its weight gradient unless that
-            # takes the transposed BF16 copy, and by its own input gradient
-            # (unit i - 1 > 0) unless that takes the BF16 row copy
-            dw_bf = (prev.w.grad is None or (self.mode_dw == bf16 and self.ds_bft[i - 1] is not None
-                                             and B % 64 == 0))
-            dx_bf = i - 1 == 0 or (self.mode_dx == bf16 and self.ds_bf[i - 1] is not None
-                                   and prev.output_dim % 64 == 0)
-            self.colsum[i] = not (dw_bf and dx_bf)
+            self.colsum[i] = self._dy_fp32_read(i - 1)
+
+    def _dy_fp32_read(self, p: int) -> bool:
+        """Whether the products of unit p read its FP32 output gradient ds[p]:
+        its weight gradient does unless it takes the transposed BF16 copy, its
+        input gradient (p > 0) unless it takes the BF16 row copy."""
+        bf16 = GEMM_MODE["bf16"]
+        d = self.units[p].dense
+        dw_bf = (d.w.grad is None or (self.mode_dw == bf16 and self.ds_bft[p] is not None
+                                      and self.B % 64 == 0))
+        dx_bf = p == 0 or (self.mode_dx == bf16 and self.ds_bf[p] is not None and d.output_dim % 64 == 0)
+        return not (dw_bf and dx_bf)
 
     def _capture(self):
         # kernel configuration (when tuning) and the graph's GEMM packing
