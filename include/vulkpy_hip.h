@@ -356,8 +356,9 @@ int vkh_gemm_get_config(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
 int vkh_gemm_set_config(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
                         uint64_t M, uint64_t N, uint64_t K, int index);
 /* times every configuration on the shape (scratch operands from the
- * context's allocator, one warm-up and `reps` timed runs each, operand
- * packing included), keeps the fastest for the shape and returns its index;
+ * context's allocator, one warm-up and `reps` timed runs each; operand
+ * packing included in FP32 and BF16X3, the BF16 operands packed once
+ * beforehand), keeps the fastest for the shape and returns its index;
  * ms_per_config (optional, vkh_gemm_configs entries) receives each mean time
  * in milliseconds.  Synchronises the stream; refused during graph capture. */
 int vkh_gemm_autotune(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
@@ -456,6 +457,13 @@ int vkh_huber_rows(vkh_gpu* gpu, const float* x, const float* y, float* L,
  *        used by the autograd wrapper, no counterpart in the reference */
 int vkh_loss_grad(vkh_gpu* gpu, int kind, const float* x, const float* y,
                   float* dx, uint64_t n, float scale);
+/* vkh_mse_rows (kind 1) or vkh_huber_rows (kind 2) and vkh_loss_grad of the
+ * same kind over the same [rows, cols] x, y, in one launch with the same
+ * bits.  No counterpart in the reference (a captured training step's loss
+ * is one kernel). */
+int vkh_loss_rows_grad(vkh_gpu* gpu, int kind, const float* x, const float* y,
+                       float* L, float* dx, uint64_t rows, uint64_t cols,
+                       float scale);
 /* Adam update in the op order of AdamState.grad2diff + Parameter.update
  * (nn/optimizers.py:235-253, nn/parameters.py:94-95), FP32 arithmetic on
  * scalars formed like the Python ones: beta, 1-beta, 1-beta^t are computed in
@@ -479,6 +487,30 @@ int vkh_adam_device(vkh_gpu* gpu, float* w, float* g, float* m, float* v,
                     uint64_t n, float lr, double beta1, double beta2, float eps,
                     const double* bpow, uint16_t* w_bf16, int reg_kind,
                     float reg_l1, float reg_s2);
+/* one parameter of vkh_adam_device_multi: the arguments of vkh_adam_device
+ * without the BF16 copy */
+#define VKH_ADAM_MULTI_MAX 8
+typedef struct vkh_adam_param {
+  float* w;
+  float* g;
+  float* m;
+  float* v;
+  uint64_t n;
+  float lr;
+  float eps;
+  double beta1;
+  double beta2;
+  const double* bpow;
+  int reg_kind;
+  float reg_l1;
+  float reg_s2;
+} vkh_adam_param;
+/* vkh_adam_device on up to VKH_ADAM_MULTI_MAX parameters in one launch,
+ * each with the bits vkh_adam_device gives it.  The table is read at the
+ * call (captured graphs keep its values).  No counterpart in the reference
+ * (a captured training step updates its parameters in one kernel). */
+int vkh_adam_device_multi(vkh_gpu* gpu, const vkh_adam_param* params,
+                          int count);
 /* SGD (nn/optimizers.py:53) and AdaGrad (nn/optimizers.py:134-140), with the
  * optional regularizer term of vkh_adam_device (reg_kind 0 = none) */
 int vkh_sgd(vkh_gpu* gpu, float* w, float* g, uint64_t n, float lr,

@@ -49,6 +49,9 @@ CASES = [
     ([(8, 16), (16, 3)], ["sigmoid", None], "mse", lambda: nn.Lasso(1e-3), "sgd"),
     ([(12, 40), (40, 4)], ["relu", None], "sce", lambda: nn.Elastic(1e-3, 2e-3), "adagrad"),
     ([(12, 40), (40, 40), (40, 2)], ["relu", "relu", None], "huber", lambda: nn.Elastic(5e-4, 1e-3), "adam"),
+    # ten Adam parameters: two multi-parameter update launches
+    ([(10, 24), (24, 24), (24, 16), (16, 16), (16, 3)], ["relu", "sigmoid", "relu", "relu", None], "mse",
+     lambda: nn.Ridge(1e-3), "adam"),
 ]
 
 

@@ -256,6 +256,69 @@ def test_optimizer_kernels_bit_exact(gpu):
     bits_equal(ada.opt_state.h, h2)
 
 
+def test_loss_rows_grad_equals_separate_kernels(gpu):
+    """vkh_loss_rows_grad (one launch) against vkh_mse_rows / vkh_huber_rows
+    and vkh_loss_grad, bit for bit, and against the oracle."""
+    from vulkpy_amd import _lib
+    h = gpu._handle
+    rng = np.random.default_rng(4)
+    for rows, cols in ((16384, 1), (513, 3)):
+        x = (rng.standard_normal((rows, cols)) * 2).astype(f32)
+        t = (rng.standard_normal((rows, cols)) * 2).astype(f32)
+        X, T = A(gpu, x), A(gpu, t)
+        scale = float(f32(1 / rows))
+        for kind, rows_fn, name in ((1, "vkh_mse_rows", "mse"), (2, "vkh_huber_rows", "huber")):
+            L1, G1, L2, G2 = (vk.Array(gpu, shape=s) for s in ((rows,), (rows, cols), (rows,), (rows, cols)))
+            _lib.call("vkh_loss_rows_grad", h, kind, X._ptr, T._ptr, L1._ptr, G1._ptr, rows, cols, scale)
+            _lib.call(rows_fn, h, X._ptr, T._ptr, L2._ptr, rows, cols)
+            _lib.call("vkh_loss_grad", h, kind, X._ptr, T._ptr, G2._ptr, rows * cols, scale)
+            gpu.wait()
+            bits_equal(L1, np.asarray(L2))
+            bits_equal(G1, np.asarray(G2))
+            bits_equal(L1, O.mse_rows(x, t) if kind == 1 else O.huber_rows(x, t))
+            bits_equal(G1, O.loss_grad(name, x, t, scale))
+
+
+def test_adam_multi_equals_adam_device(gpu):
+    """vkh_adam_device_multi on parameters of several sizes (aligned and
+    ragged, with and without regularizer terms) against one vkh_adam_device
+    each, bit for bit, over three steps with device beta powers."""
+    from vulkpy_amd import _lib
+    h = gpu._handle
+    rng = np.random.default_rng(5)
+    sizes = (1, 3, 4096, 10007, 262144 + 5, 17)
+    regs = ((0, 0.0, 0.0), (2, 0.0, float(f32(2e-4))), (1, 1e-3, 0.0), (3, 1e-3, float(f32(2e-4))),
+            (0, 0.0, 0.0), (2, 0.0, 0.5))
+    def state():
+        return [[A(gpu, rng.standard_normal(n)), A(gpu, np.zeros(n)), A(gpu, np.zeros(n))] for n in sizes]
+    a, b = state(), state()
+    for (wa, ma, va), (wb, mb, vb) in zip(a, b):
+        wb[:] = wa
+    # device beta powers: FP64 pairs held in uint32 buffers
+    bpa = vk.U32Array(gpu, data=np.array([1.0, 1.0]).view(np.uint32))
+    bpb = vk.U32Array(gpu, data=np.array([1.0, 1.0]).view(np.uint32))
+    for step in range(3):
+        grads = [A(gpu, rng.standard_normal(n)) for n in sizes]
+        ga = [vk.Array(gpu, data=np.asarray(g)) for g in grads]
+        gb = [vk.Array(gpu, data=np.asarray(g)) for g in grads]
+        _lib.call("vkh_adam_pow_advance", h, bpa._ptr, 0.9, 0.999)
+        _lib.call("vkh_adam_pow_advance", h, bpb._ptr, 0.9, 0.999)
+        params = [_lib.AdamParam(w._ptr, g._ptr, m._ptr, v._ptr, n, 1e-3, 1e-8, 0.9, 0.999, bpa._ptr, *r)
+                  for (w, m, v), g, n, r in zip(a, ga, sizes, regs)]
+        _lib.call("vkh_adam_device_multi", h, (_lib.AdamParam * len(params))(*params), len(params))
+        for (w, m, v), g, n, r in zip(b, gb, sizes, regs):
+            _lib.call("vkh_adam_device", h, w._ptr, g._ptr, m._ptr, v._ptr, n, 1e-3, 0.9, 0.999, 1e-8,
+                      bpb._ptr, None, *r)
+        for arrs in (a, b, [[g] for g in ga], [[g] for g in gb]):
+            for group in arrs:
+                for x in group:
+                    x._set_job([bpa, bpb])  # written through raw calls: the host mirrors are stale
+        gpu.wait()
+        for (wa, ma, va), (wb, mb, vb), g1, g2 in zip(a, b, ga, gb):
+            for x, y in ((wa, wb), (ma, mb), (va, vb), (g1, g2)):
+                bits_equal(x, np.asarray(y))
+
+
 def test_dense_backward_vs_reference_formula(gpu):
     # vulkpy's Dense.backward (layers.py:126-141) restated by the oracle
     rng = np.random.default_rng(3)

@@ -48,6 +48,16 @@ P = c_void_p
 U64 = c_uint64
 _u64p = POINTER(c_uint64)
 
+ADAM_MULTI_MAX = 8  # VKH_ADAM_MULTI_MAX
+
+
+class AdamParam(ctypes.Structure):
+    """struct vkh_adam_param of include/vulkpy_hip.h"""
+    _fields_ = [("w", c_void_p), ("g", c_void_p), ("m", c_void_p), ("v", c_void_p),
+                ("n", c_uint64), ("lr", c_float), ("eps", c_float), ("beta1", c_double),
+                ("beta2", c_double), ("bpow", c_void_p), ("reg_kind", c_int), ("reg_l1", c_float),
+                ("reg_s2", c_float)]
+
 # name -> (restype, argtypes); every function returns an int status except
 # vkh_last_error.
 _SIGNATURES = {
@@ -136,10 +146,12 @@ _SIGNATURES = {
     "vkh_mse_rows": (c_int, [P, P, P, P, U64, U64]),
     "vkh_huber_rows": (c_int, [P, P, P, P, U64, U64]),
     "vkh_loss_grad": (c_int, [P, c_int, P, P, P, U64, c_float]),
+    "vkh_loss_rows_grad": (c_int, [P, c_int, P, P, P, P, U64, U64, c_float]),
     "vkh_adam": (c_int, [P, P, P, P, P, U64, c_float, c_double, c_double, c_float, c_double, c_double]),
     "vkh_adam_pow_advance": (c_int, [P, P, c_double, c_double]),
     "vkh_adam_device": (c_int, [P, P, P, P, P, U64, c_float, c_double, c_double, c_float, P, P,
                                 c_int, c_float, c_float]),
+    "vkh_adam_device_multi": (c_int, [P, P, c_int]),
     "vkh_sgd": (c_int, [P, P, P, U64, c_float, c_int, c_float, c_float]),
     "vkh_regularizer_grad": (c_int, [P, c_int, P, P, U64, c_float, c_float, c_int]),
     "vkh_dense1_backward": (c_int, [P, P, P, P, c_int, U64, U64, P, P, P, U64, P, P]),

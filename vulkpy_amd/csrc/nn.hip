@@ -234,6 +234,37 @@ k_loss_grad(int kind, const float* x, const float* y, float* dx, uint64_t n, flo
   }
 }
 
+// k_loss_rows<KIND> and k_loss_grad (kind 1 for MSE, 2 for Huber) in one
+// pass over x and y: the same bits, one launch fewer
+template <int KIND>
+__global__ void __launch_bounds__(256)
+k_loss_rows_grad(const float* x, const float* y, float* L, float* dx, uint64_t rows, uint64_t cols,
+                 float scale) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x; r < rows; r += stride) {
+    float acc = 0.0f;
+    for (uint64_t j = 0; j < cols; ++j) {
+      const uint64_t i = r * cols + j;
+      const float xv = x[i], yv = y[i];
+      const float d = yv - xv;
+      float t;
+      if (KIND == 0) {
+        t = d * d;
+      } else {
+        const float a = fabsf(d);
+        t = np_min(a, a * a);
+        t = t * 0.5f;
+      }
+      acc += t;
+      float gd = xv - yv;
+      gd = KIND == 0 ? gd * 2.0f : np_min(np_max(gd, -1.0f), 1.0f);
+      if (scale != 1.0f) gd = gd * scale;
+      dx[i] = gd;
+    }
+    L[r] = acc;
+  }
+}
+
 // nn_cross_entropy_backward.comp:26 then `dx *= scale`
 __global__ void __launch_bounds__(256)
 k_xent_bwd(const float* x, const float* y, float* dx, uint64_t n, float scale) {
@@ -366,6 +397,64 @@ __global__ void k_adam_pow(double* bpow, double beta1, double beta2) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     bpow[0] = bpow[0] * beta1;
     bpow[1] = bpow[1] * beta2;
+  }
+}
+
+// Several parameters' Adam updates in one launch: parameter p takes blocks
+// [first[p], first[p + 1]) and walks its elements as k_adam does with that
+// many blocks (the element arithmetic is k_adam's, so the bits are too).
+constexpr int ADAM_MT = VKH_ADAM_MULTI_MAX;
+struct AdamTensor {
+  float *w, *g, *m, *v;
+  uint64_t n, n4;
+  AdamScalars s;
+  const double* bpow;
+  RegTerms r;
+};
+struct AdamMulti {
+  AdamTensor t[ADAM_MT];
+  uint32_t first[ADAM_MT + 1];
+  int count;
+};
+
+__global__ void __launch_bounds__(256) k_adam_multi(AdamMulti a) {
+  // the block's parameter, by constant indices only (a by-value argument
+  // indexed at run time would be copied to scratch memory)
+  AdamTensor t = a.t[0];
+  uint32_t b0 = a.first[0], b1 = a.first[1];
+#pragma unroll
+  for (int q = 1; q < ADAM_MT; ++q)
+    if (q < a.count && blockIdx.x >= a.first[q]) {
+      t = a.t[q];
+      b0 = a.first[q];
+      b1 = a.first[q + 1];
+    }
+  AdamScalars s = t.s;
+  if (t.bpow) {
+    s.c1 = (float)(1.0 - t.bpow[0]);
+    s.c2 = (float)(1.0 - t.bpow[1]);
+  }
+  float* __restrict__ w = t.w;
+  float* __restrict__ g = t.g;
+  float* __restrict__ m = t.m;
+  float* __restrict__ v = t.v;
+  const RegTerms r = t.r;
+  const uint64_t stride = (uint64_t)(b1 - b0) * 256;
+  const uint64_t i0 = (uint64_t)(blockIdx.x - b0) * 256 + threadIdx.x;
+  for (uint64_t q = i0; q < t.n4; q += stride) {
+    float4 wv = reinterpret_cast<float4*>(w)[q], gv = reinterpret_cast<const float4*>(g)[q];
+    float4 mv = reinterpret_cast<float4*>(m)[q], vv = reinterpret_cast<float4*>(v)[q];
+    adam4(wv, gv, mv, vv, s, r);
+    reinterpret_cast<float4*>(w)[q] = wv;
+    reinterpret_cast<float4*>(m)[q] = mv;
+    reinterpret_cast<float4*>(v)[q] = vv;
+    if (r.kind) reinterpret_cast<float4*>(g)[q] = gv;
+  }
+  for (uint64_t i = 4 * t.n4 + i0; i < t.n; i += stride) {
+    float wi = w[i], gi = with_reg(r, g[i], w[i]), mi = m[i], vi = v[i];
+    adam_elem(wi, gi, mi, vi, s);
+    w[i] = wi; m[i] = mi; v[i] = vi;
+    if (r.kind) g[i] = gi;
   }
 }
 
@@ -548,6 +637,22 @@ int vkh_huber_rows(vkh_gpu* g, const float* x, const float* y, float* L, uint64_
   });
 }
 
+int vkh_loss_rows_grad(vkh_gpu* g, int kind, const float* x, const float* y, float* L, float* dx,
+                       uint64_t rows, uint64_t cols, float scale) {
+  return guarded([&] {
+    use(g);
+    VKH_REQUIRE(kind == 1 || kind == 2, "loss_rows_grad: kind must be 1 (MSE) or 2 (Huber)");
+    if (rows == 0) return;
+    VKH_REQUIRE(x && y && L && dx, "loss_rows_grad: null pointer");
+    unsigned grid = (unsigned)std::min<uint64_t>(ceil_div(rows, 256), (uint64_t)g->num_cus * 8);
+    if (kind == 1)
+      hipLaunchKernelGGL(k_loss_rows_grad<0>, dim3(grid), dim3(256), 0, g->stream, x, y, L, dx, rows, cols, scale);
+    else
+      hipLaunchKernelGGL(k_loss_rows_grad<1>, dim3(grid), dim3(256), 0, g->stream, x, y, L, dx, rows, cols, scale);
+    check_launch("loss_rows_grad");
+  });
+}
+
 int vkh_loss_grad(vkh_gpu* g, int kind, const float* x, const float* y, float* dx, uint64_t n, float scale) {
   return guarded([&] {
     use(g);
@@ -629,6 +734,38 @@ int vkh_adam_device(vkh_gpu* g, float* w, float* gr, float* m, float* v, uint64_
   });
 }
 
+int vkh_adam_device_multi(vkh_gpu* g, const vkh_adam_param* params, int count) {
+  return guarded([&] {
+    use(g);
+    VKH_REQUIRE(count >= 0 && count <= ADAM_MT, "adam_device_multi: count out of range");
+    VKH_REQUIRE(count == 0 || params, "adam_device_multi: null pointer");
+    AdamMulti a{};
+    uint64_t blocks = 0;
+    int k = 0;
+    for (int p = 0; p < count; ++p) {
+      const vkh_adam_param& q = params[p];
+      VKH_REQUIRE(q.w && q.g && q.m && q.v && q.bpow, "adam_device_multi: null pointer");
+      const RegTerms r = reg_terms(q.reg_kind, q.reg_l1, q.reg_s2);
+      if (q.n == 0) continue;
+      const uint64_t n4 = vec4_count(q.n, {q.w, q.g, q.m, q.v});
+      a.t[k] = AdamTensor{q.w, q.g, q.m, q.v, q.n, n4,
+                          AdamScalars{(float)q.beta1, (float)(1.0 - q.beta1), (float)q.beta2,
+                                      (float)(1.0 - q.beta2), q.eps, -q.lr, 1.0f, 1.0f},
+                          q.bpow, r};
+      a.first[k] = (uint32_t)blocks;
+      blocks += grid_for(g, n4 ? n4 : q.n);
+      ++k;
+    }
+    if (k == 0) return;
+    VKH_REQUIRE(blocks < (uint64_t(1) << 31), "adam_device_multi: too many blocks");
+    a.first[k] = (uint32_t)blocks;
+    for (int p = k + 1; p <= ADAM_MT; ++p) a.first[p] = (uint32_t)blocks;
+    a.count = k;
+    hipLaunchKernelGGL(k_adam_multi, dim3((unsigned)blocks), dim3(256), 0, g->stream, a);
+    check_launch("adam_device_multi");
+  });
+}
+
 int vkh_sgd(vkh_gpu* g, float* w, float* gr, uint64_t n, float lr, int reg_kind, float reg_l1,
             float reg_s2) {
   return guarded([&] {
@@ -661,6 +798,11 @@ int vkh_bias_grad(vkh_gpu* g, const float* dy, float* db, uint64_t rows, uint64_
     use(g);
     if (cols == 0) return;
     VKH_REQUIRE(dy && db, "bias_grad: null pointer");
+    if (beta == 0.0f) {  // db = t exactly: the sums go straight to db
+      int r = vkh_reduce_axis(g, VKH_RSUM, dy, db, 1, rows, cols, 0, 1.0f, 1.0f);
+      if (r != VKH_OK) fail(r, vkh_last_error());
+      return;
+    }
     Workspace ws(g, sizeof(float) * cols);
     float* t = (float*)ws.get();
     int r = vkh_reduce_axis(g, VKH_RSUM, dy, t, 1, rows, cols, 0, 1.0f, 1.0f);

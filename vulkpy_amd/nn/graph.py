@@ -37,7 +37,7 @@ from typing import Iterable, List, Optional, Tuple
 
 import numpy as np
 
-from vulkpy_amd._lib import ACT, GEMM_MODE, call
+from vulkpy_amd._lib import ACT, ADAM_MULTI_MAX, GEMM_MODE, AdamParam, call
 from vulkpy_amd import vkarray
 from vulkpy_amd.vkarray import Array, _DeviceBuffer
 from .core import Loss, Module
@@ -244,12 +244,12 @@ class _Plan:
         elif self.loss_kind == "ce":
             call("vkh_cross_entropy", h, out._ptr, self.y._ptr, self.L_rows._ptr, B, O)
             call("vkh_cross_entropy_backward", h, out._ptr, self.y._ptr, gl._ptr, B * O, self.scale)
-        elif self.loss_kind == "mse":
-            call("vkh_mse_rows", h, out._ptr, self.y._ptr, self.L_rows._ptr, B, O)
-            call("vkh_loss_grad", h, 1, out._ptr, self.y._ptr, gl._ptr, B * O, self.scale)
         else:
-            call("vkh_huber_rows", h, out._ptr, self.y._ptr, self.L_rows._ptr, B, O)
-            call("vkh_loss_grad", h, 2, out._ptr, self.y._ptr, gl._ptr, B * O, self.scale)
+            # loss rows and gradient in one kernel (vkh_mse_rows / vkh_huber_rows
+            # and vkh_loss_grad, the same bits)
+            kind = 1 if self.loss_kind == "mse" else 2
+            call("vkh_loss_rows_grad", h, kind, out._ptr, self.y._ptr, self.L_rows._ptr, gl._ptr, B, O,
+                 self.scale)
         mul = float(np.float32(1 / B)) if self.mean else 1.0
         call("vkh_reduce_axis", h, 0, self.L_rows._ptr, self.loss._ptr, 1, B, 1, 0, mul, 1.0)
         if self.g_out is not None:
@@ -305,15 +305,25 @@ class _Plan:
         # parameter (regularizer term, gradient write-back and update)
         for buf, beta1, beta2 in self.bpow_groups:
             call("vkh_adam_pow_advance", h, buf.ptr, beta1, beta2)
+        # Adam parameters: up to ADAM_MULTI_MAX a launch
+        adam = []
+        for p in self.params:
+            st = p.opt_state
+            if isinstance(st, AdamState):
+                o = st.opt
+                kind, l1, s2 = self.reg[id(p)]
+                adam.append(AdamParam(p.value._ptr, p.grad._ptr, st.m._ptr, st.v._ptr, p.value.size,
+                                      float(o.lr), float(o.eps), float(o.beta1), float(o.beta2),
+                                      self.bpow[id(st)].ptr, kind, l1, s2))
+        for k in range(0, len(adam), ADAM_MULTI_MAX):
+            part = adam[k:k + ADAM_MULTI_MAX]
+            call("vkh_adam_device_multi", h, (AdamParam * len(part))(*part), len(part))
         for p in self.params:
             st = p.opt_state
             n = p.value.size
             kind, l1, s2 = self.reg[id(p)]
             if isinstance(st, AdamState):
-                o = st.opt
-                call("vkh_adam_device", h, p.value._ptr, p.grad._ptr, st.m._ptr, st.v._ptr, n,
-                     float(o.lr), float(o.beta1), float(o.beta2), float(o.eps), self.bpow[id(st)].ptr,
-                     None, kind, l1, s2)
+                continue
             elif isinstance(st, SGDState):
                 call("vkh_sgd", h, p.value._ptr, p.grad._ptr, n, float(st.opt.lr), kind, l1, s2)
             else:
