@@ -242,10 +242,11 @@ def test_graph_fused_last_layer_matches_eager(gpu, precision, hidden_act):
             np.testing.assert_array_equal(a, b)
 
 
-def test_dense1_backward_matches_separate_steps(gpu):
+@pytest.mark.parametrize("B,N", [(512, 96), (2048, 4096)])
+def test_dense1_backward_matches_separate_steps(gpu, B, N):
     """vkh_dense1_backward against vkh_gemm (input and weight gradients) and
-    vkh_bias_grad, bit for bit, BF16 copies included, on a 512 x 96 layer."""
-    B, N = 512, 96
+    vkh_bias_grad, bit for bit, BF16 copies included, on a 512 x 96 layer
+    and a 2048 x 4096 one (whose blocks take two row tiles each)."""
     h = gpu._handle
     rng = np.random.default_rng(29)
     dy = vk.Array(gpu, data=rng.standard_normal((B, 1)).astype(np.float32))

@@ -1629,95 +1629,111 @@ static void gemm_smallk(vkh_gpu* g, bool ta, bool tb, const GemmArgs& a) {
 // A block stages a 128-row x 32-column tile of dx and of h in LDS (33 KB, four
 // blocks a compute unit); the chunk sums walk its columns and the transposed
 // copy leaves in 256-byte runs (16 bytes a lane).  kc_dw and kc_db divide 128.
+// Each block takes a column of tiles in turn, with the next tile's loads in
+// flight during the current tile's column walks and stores.
 constexpr int D1_TR = 128, D1_TC = 32, D1_PITCH = D1_TC + 1;  // odd pitch: column walks conflict-free
 constexpr int D1_LOADS = D1_TR * D1_TC / 4 / 256;                 // float4 loads of h a thread
 
 __global__ void __launch_bounds__(256)
 k_dense1_backward(const float* __restrict__ dy, const float* __restrict__ w, const float* __restrict__ h,
-                  int act, uint64_t N, float* __restrict__ dx, uint16_t* __restrict__ dx_bf,
-                  uint16_t* __restrict__ dx_bft, uint64_t ldt, float* __restrict__ part_dw, int kc_dw,
-                  float* __restrict__ part_db, int kc_db) {
+                  int act, uint64_t N, uint64_t row_tiles, float* __restrict__ dx,
+                  uint16_t* __restrict__ dx_bf, uint16_t* __restrict__ dx_bft, uint64_t ldt,
+                  float* __restrict__ part_dw, int kc_dw, float* __restrict__ part_db, int kc_db) {
   __shared__ float D[D1_TR * D1_PITCH], H[D1_TR * D1_PITCH], Y[D1_TR];
-  const uint64_t n0 = (uint64_t)blockIdx.x * D1_TC, b0 = (uint64_t)blockIdx.y * D1_TR;
+  const uint64_t n0 = (uint64_t)blockIdx.x * D1_TC;
   const int t = threadIdx.x, c4 = t % (D1_TC / 4), r0 = t / (D1_TC / 4);
   const uint64_t n = n0 + c4 * 4;
   const float4 wv = *reinterpret_cast<const float4*>(w + n);
   const float wq[4] = {wv.x, wv.y, wv.z, wv.w};
-  // every load of the tile first, then the arithmetic
-  float4 hv[D1_LOADS];
-  float d[D1_LOADS];
-#pragma unroll
-  for (int it = 0; it < D1_LOADS; ++it) {
-    const uint64_t b = b0 + r0 + it * (256 / (D1_TC / 4));
-    hv[it] = ld_stream(reinterpret_cast<const float4*>(h + b * N + n));
-    d[it] = dy[b];
-  }
-  if (t < D1_TR) Y[t] = dy[b0 + t];
-#pragma unroll
-  for (int it = 0; it < D1_LOADS; ++it) {
-    const int r = r0 + it * (256 / (D1_TC / 4));
-    const uint64_t b = b0 + r;
-    const float hq[4] = {hv[it].x, hv[it].y, hv[it].z, hv[it].w};
-    float o[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      o[q] = act_deriv_mul(act, hq[q], d[it] * wq[q]);
-      D[r * D1_PITCH + c4 * 4 + q] = o[q];
-      H[r * D1_PITCH + c4 * 4 + q] = hq[q];
-    }
-    if (dx) st_stream(reinterpret_cast<float4*>(dx + b * N + n), make_float4(o[0], o[1], o[2], o[3]));
-    if (dx_bf) {
-      uint2 hb;
-      hb.x = f2bf(o[0]) | ((uint32_t)f2bf(o[1]) << 16);
-      hb.y = f2bf(o[2]) | ((uint32_t)f2bf(o[3]) << 16);
-      *reinterpret_cast<uint2*>(dx_bf + b * N + n) = hb;
-    }
-  }
-  __syncthreads();
-  // chunk sums: one thread per (chunk, column), rows in order (each sum a
-  // chain of dependent adds; its LDS reads go out 8 at a time)
   const int ndb = part_db ? D1_TR / kc_db : 0, ndw = part_dw ? D1_TR / kc_dw : 0;
-  for (int item = t; item < (ndb + ndw) * D1_TC; item += 256) {
-    const int col = item % D1_TC, j = item / D1_TC;
-    if (j < ndb) {
-      const int rs = j * kc_db;
-      float acc = D[rs * D1_PITCH + col];
-      int r = 1;
-      for (; r + 8 <= kc_db; r += 8) {
-        float v[8];
+  // the block walks row tiles blockIdx.y, + gridDim.y, ...; each tile's
+  // loads are issued before the previous tile's column walks and
+  // transposed stores, so they are in flight meanwhile
+  float4 hv[D1_LOADS];
+  float d[D1_LOADS], yv = 0.f;
+  auto load_tile = [&](uint64_t tile) {
+    const uint64_t b0 = tile * D1_TR;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = D[(rs + r + q) * D1_PITCH + col];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) acc = acc + v[q];
-      }
-      for (; r < kc_db; ++r) acc = acc + D[(rs + r) * D1_PITCH + col];
-      part_db[(b0 / kc_db + j) * N + n0 + col] = acc;
-    } else {
-      const int rs = (j - ndb) * kc_dw;
-      float acc = 0.f;
-      int r = 0;
-      for (; r + 8 <= kc_dw; r += 8) {
-        float v[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = H[(rs + r + q) * D1_PITCH + col] * Y[rs + r + q];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) acc += v[q];
-      }
-      for (; r < kc_dw; ++r) acc += H[(rs + r) * D1_PITCH + col] * Y[rs + r];
-      part_dw[(b0 / kc_dw + j - ndb) * N + n0 + col] = acc;
+    for (int it = 0; it < D1_LOADS; ++it) {
+      const uint64_t b = b0 + r0 + it * (256 / (D1_TC / 4));
+      hv[it] = ld_stream(reinterpret_cast<const float4*>(h + b * N + n));
+      d[it] = dy[b];
     }
-  }
-  if (dx_bft) {
-    // D1_TC columns x 16 runs of 8 rows; 16 consecutive lanes continue one column
-    for (int item = t; item < D1_TC * (D1_TR / 8); item += 256) {
-      const int col = item / (D1_TR / 8), seg = item % (D1_TR / 8);
-      uint32_t wd[4];
+    if (t < D1_TR) yv = dy[b0 + t];
+  };
+  uint64_t tile = blockIdx.y;
+  if (tile < row_tiles) load_tile(tile);
+  for (; tile < row_tiles; tile += gridDim.y) {
+    const uint64_t b0 = tile * D1_TR;
+    if (t < D1_TR) Y[t] = yv;
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        wd[q] = f2bf(D[(seg * 8 + 2 * q) * D1_PITCH + col]) |
-                ((uint32_t)f2bf(D[(seg * 8 + 2 * q + 1) * D1_PITCH + col]) << 16);
-      *reinterpret_cast<uint4*>(dx_bft + (n0 + col) * ldt + b0 + seg * 8) = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+    for (int it = 0; it < D1_LOADS; ++it) {
+      const int r = r0 + it * (256 / (D1_TC / 4));
+      const uint64_t b = b0 + r;
+      const float hq[4] = {hv[it].x, hv[it].y, hv[it].z, hv[it].w};
+      float o[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        o[q] = act_deriv_mul(act, hq[q], d[it] * wq[q]);
+        D[r * D1_PITCH + c4 * 4 + q] = o[q];
+        H[r * D1_PITCH + c4 * 4 + q] = hq[q];
+      }
+      if (dx) st_stream(reinterpret_cast<float4*>(dx + b * N + n), make_float4(o[0], o[1], o[2], o[3]));
+      if (dx_bf) {
+        uint2 hb;
+        hb.x = f2bf(o[0]) | ((uint32_t)f2bf(o[1]) << 16);
+        hb.y = f2bf(o[2]) | ((uint32_t)f2bf(o[3]) << 16);
+        *reinterpret_cast<uint2*>(dx_bf + b * N + n) = hb;
+      }
     }
+    __syncthreads();
+    if (tile + gridDim.y < row_tiles) load_tile(tile + gridDim.y);
+    // chunk sums: one thread per (chunk, column), rows in order (each sum a
+    // chain of dependent adds; its LDS reads go out 8 at a time)
+    for (int item = t; item < (ndb + ndw) * D1_TC; item += 256) {
+      const int col = item % D1_TC, j = item / D1_TC;
+      if (j < ndb) {
+        const int rs = j * kc_db;
+        float acc = D[rs * D1_PITCH + col];
+        int r = 1;
+        for (; r + 8 <= kc_db; r += 8) {
+          float v[8];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) v[q] = D[(rs + r + q) * D1_PITCH + col];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) acc = acc + v[q];
+        }
+        for (; r < kc_db; ++r) acc = acc + D[(rs + r) * D1_PITCH + col];
+        part_db[(b0 / kc_db + j) * N + n0 + col] = acc;
+      } else {
+        const int rs = (j - ndb) * kc_dw;
+        float acc = 0.f;
+        int r = 0;
+        for (; r + 8 <= kc_dw; r += 8) {
+          float v[8];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) v[q] = H[(rs + r + q) * D1_PITCH + col] * Y[rs + r + q];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) acc += v[q];
+        }
+        for (; r < kc_dw; ++r) acc += H[(rs + r) * D1_PITCH + col] * Y[rs + r];
+        part_dw[(b0 / kc_dw + j - ndb) * N + n0 + col] = acc;
+      }
+    }
+    if (dx_bft) {
+      // D1_TC columns x 16 runs of 8 rows; 16 consecutive lanes continue one column
+      for (int item = t; item < D1_TC * (D1_TR / 8); item += 256) {
+        const int col = item / (D1_TR / 8), seg = item % (D1_TR / 8);
+        uint32_t wd[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          wd[q] = f2bf(D[(seg * 8 + 2 * q) * D1_PITCH + col]) |
+                  ((uint32_t)f2bf(D[(seg * 8 + 2 * q + 1) * D1_PITCH + col]) << 16);
+        *reinterpret_cast<uint4*>(dx_bft + (n0 + col) * ldt + b0 + seg * 8) =
+            make_uint4(wd[0], wd[1], wd[2], wd[3]);
+      }
+    }
+    __syncthreads();  // D, H and Y are read before the next tile writes them
   }
 }
 
@@ -2046,9 +2062,13 @@ int vkh_dense1_backward(vkh_gpu* g, const float* dy, const float* w, const float
     Workspace wdw(g, dw ? sizeof(float) * ndw * N : 1), wdb(g, db ? sizeof(float) * ndb * N : 1);
     float* pdw = dw ? (float*)wdw.get() : nullptr;
     float* pdb = db ? (float*)wdb.get() : nullptr;
-    dim3 grid((unsigned)(N / D1_TC), (unsigned)(B / D1_TR));
-    hipLaunchKernelGGL(k_dense1_backward, grid, dim3(256), 0, g->stream, dy, w, h, act, N, dx, dx_bf, dx_bft,
-                       ldt, pdw, (int)kc_dw, pdb, (int)kc_db);
+    // about four blocks a compute unit, each walking B / 128 / gy row tiles
+    const uint64_t col_tiles = N / D1_TC, row_tiles = B / D1_TR;
+    const uint64_t gy = std::min<uint64_t>(row_tiles, std::max<uint64_t>(1, ceil_div(4 * (uint64_t)g->num_cus,
+                                                                                      col_tiles)));
+    dim3 grid((unsigned)col_tiles, (unsigned)gy);
+    hipLaunchKernelGGL(k_dense1_backward, grid, dim3(256), 0, g->stream, dy, w, h, act, N, row_tiles, dx, dx_bf,
+                       dx_bft, ldt, pdw, (int)kc_dw, pdb, (int)kc_db);
     check_launch("dense1_backward");
     // dw: the column walk's combine (0 + chunks in order; its 1 * sum is the sum)
     if (dw) sum_chunks_from_zero(g, (const float*)pdw, dw, N, ndw);
