@@ -364,6 +364,24 @@ int vkh_gemm_set_config(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
 int vkh_gemm_autotune(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
                       uint64_t M, uint64_t N, uint64_t K, int reps, int* best,
                       float* ms_per_config);
+/* vkh_gemm_autotune timing the caller's own product: every configuration
+ * runs the vkh_gemm_bf16io_ext call given (its operands, packed operands and
+ * epilogue: bias, activation, BF16 copies, aux, stored or unstored C; not
+ * the column sums db), one warm-up and `reps` timed runs each, and the
+ * fastest is kept for the shape.  The outputs' contents are unspecified
+ * afterwards.  Synchronises the stream; refused during graph capture.  No
+ * counterpart in the reference (a captured training step tunes its products
+ * with their epilogues, whose cost differs between tile shapes). */
+int vkh_gemm_autotune_ext(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
+                          uint64_t M, uint64_t N, uint64_t K, float alpha,
+                          const float* A, uint64_t lda, const float* B,
+                          uint64_t ldb, float beta, float* C, uint64_t ldc,
+                          const float* bias, int act, uint16_t* c_bf16,
+                          const float* aux, const uint16_t* aux_bf16,
+                          uint64_t ldaux, int aux_act, const uint16_t* a_bf16,
+                          const uint16_t* b_bf16, uint16_t* c_bf16t,
+                          uint64_t ldt, float* db, int store_c, int reps,
+                          int* best, float* ms_per_config);
 /* vkh_gemm whose epilogue finally multiplies by act'(y) with y = aux[m, n]
  * (aux_act: VKH_ACT_RELU -> max(sign(y), 0), VKH_ACT_SIGMOID -> (1-y)*y):
  * the backward of an activation fused into the GEMM that produces its input
@@ -551,6 +569,25 @@ int vkh_gemm_bf16io_colsum(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
                            const uint16_t* a_bf16, const uint16_t* b_bf16,
                            uint16_t* c_bf16t, uint64_t ldt, float* db,
                            int store_c);
+/* The general form of vkh_gemm_bf16io: db (may be NULL) as in
+ * vkh_gemm_bf16io_colsum; store_c = 0 leaves the FP32 C unwritten (C may
+ * then be NULL; beta must be 0; c_bf16, c_bf16t or db must be given);
+ * aux_bf16 (exclusive with aux, aux_act VKH_ACT_RELU only) gives the aux
+ * activation's output as BF16, [M][ldaux] (8-byte rows for the vector path):
+ * the BF16 copies this library writes keep the sign, the zero-ness and the
+ * NaN-ness of their values, so ReLU' read from one equals ReLU' read from the
+ * FP32 value.  A captured training step uses it to drop an FP32 activation
+ * whose only other readers take its BF16 copies.  No counterpart in the
+ * reference. */
+int vkh_gemm_bf16io_ext(vkh_gpu* gpu, int mode, int trans_a, int trans_b,
+                        uint64_t M, uint64_t N, uint64_t K, float alpha,
+                        const float* A, uint64_t lda, const float* B,
+                        uint64_t ldb, float beta, float* C, uint64_t ldc,
+                        const float* bias, int act, uint16_t* c_bf16,
+                        const float* aux, const uint16_t* aux_bf16,
+                        uint64_t ldaux, int aux_act, const uint16_t* a_bf16,
+                        const uint16_t* b_bf16, uint16_t* c_bf16t, uint64_t ldt,
+                        float* db, int store_c);
 /* Backward of a last Dense layer with one output (w: [1, N]) that follows an
  * activation h = act(z) [B, N], in one pass over h, with the results of the
  * separate steps bit for bit (nn/layers.py:126-141 for the layer, 207-210 /
@@ -572,7 +609,9 @@ int vkh_dense1_backward(vkh_gpu* gpu, const float* dy, const float* w,
  * B % 128 == 0, N % 32 == 0 and the column-sum chunk lengths of the
  * separate steps divide 128 rows. */
 int vkh_dense1_backward_supported(vkh_gpu* gpu, uint64_t B, uint64_t N, int* ok);
-/* dst[i] = BF16(src[i]) (round to nearest even) */
+/* dst[i] = BF16(src[i]): round to nearest even, NaN stays NaN, and a nonzero
+ * value that would round to zero (|x| < 2^-134) becomes the smallest BF16 of
+ * its sign; the rounding of every BF16 operand and copy in this library */
 int vkh_to_bf16(vkh_gpu* gpu, const float* src, uint16_t* dst, uint64_t n);
 
 /* ---- graphs ------------------------------------------------------------- */

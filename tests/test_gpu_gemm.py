@@ -1,6 +1,7 @@
 """GEMM on the matrix cores: layouts, modes and epilogues against FP64."""
 import numpy as np
 import pytest
+from bf16ref import bf16_bits
 
 import vulkpy_amd as vk
 from vulkpy_amd import _lib
@@ -212,6 +213,42 @@ def test_autotune_api(gpu):
         _lib.call("vkh_gemm_set_config", gpu._handle, _lib.GEMM_MODE["bf16"], 0, 0, 8, 8, 8, 99)
 
 
+@pytest.mark.parametrize("mode", ["bf16", "fp32"])
+def test_autotune_ext_times_the_call(gpu, mode):
+    """vkh_gemm_autotune_ext runs the given product (bias, ReLU, BF16 copies)
+    in every configuration, keeps the fastest, and the product then gives the
+    bits it gave before tuning."""
+    import ctypes
+    h = gpu._handle
+    md = _lib.GEMM_MODE[mode]
+    M, N, K = 640, 512, 384
+    rng = np.random.default_rng(53)
+    A = vk.Array(gpu, data=rng.standard_normal((M, K)).astype(np.float32))
+    B = vk.Array(gpu, data=rng.standard_normal((N, K)).astype(np.float32))
+    bias = vk.Array(gpu, data=rng.standard_normal(N).astype(np.float32))
+    C, C2 = vk.Array(gpu, shape=(M, N)), vk.Array(gpu, shape=(M, N))
+    cbf, cbft = vk.U32Array(gpu, shape=(M * N // 2,)), vk.U32Array(gpu, shape=(M * N // 2,))
+    args = (md, 0, 1, M, N, K, 1.0, A._ptr, K, B._ptr, K, 0.0)
+    tail = (N, bias._ptr, 1, cbf._ptr, None, None, 0, 0, None, None, cbft._ptr, M, None, 1)
+    _lib.call("vkh_gemm_set_config", h, md, 0, 1, M, N, K, -1)
+    _lib.call("vkh_gemm_bf16io_ext", h, *args, C._ptr, *tail)
+    gpu.wait()
+    before = np.asarray(C).copy()
+    n = len(vk.gemm_configs(mode))
+    ms = (ctypes.c_float * n)()
+    best = ctypes.c_int(-1)
+    _lib.call("vkh_gemm_autotune_ext", h, *args, C2._ptr, *tail, 2, ctypes.byref(best), ms)
+    assert 0 <= best.value < n
+    assert all(t > 0 for t in ms) and ms[best.value] == min(ms)
+    idx, tuned = ctypes.c_int(), ctypes.c_int()
+    _lib.call("vkh_gemm_get_config", h, md, 0, 1, M, N, K, ctypes.byref(idx), ctypes.byref(tuned))
+    assert (idx.value, tuned.value) == (best.value, 1)
+    _lib.call("vkh_gemm_bf16io_ext", h, *args, C2._ptr, *tail)
+    gpu.wait()
+    np.testing.assert_array_equal(np.asarray(C2).view(np.uint32), before.view(np.uint32))
+    _lib.call("vkh_gemm_set_config", h, md, 0, 1, M, N, K, -1)
+
+
 # Split k: few output tiles (BF16 modes: of 256 x 256 with k >= 4096, chunks
 # of at least 1024; FP32: of 128 x 128 with k >= 1024, chunks of at least
 # 256) run k in chunks as separate blocks, then sum the chunk partials in
@@ -338,10 +375,79 @@ def test_prepacked_bf16_operand(gpu):
                                           outs[(mode, True)][1][: M * N // 2], err_msg=mode)
 
 
-def bf16_bits(x):
-    """round-to-nearest-even BF16 bits of finite float32 values"""
-    b = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
-    return ((b + 0x7FFF + ((b >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def test_to_bf16_rounding(gpu):
+    """vkh_to_bf16 on ties, denormals, values that would round to zero, inf
+    and NaN: the library's rounding as the NumPy statement gives it."""
+    rng = np.random.default_rng(41)
+    special = np.array([0.0, -0.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 2 ** -133, 2 ** -134, 1e-45, -1e-45,
+                        -3e-41, 2 ** -126, np.inf, -np.inf, 3.4e38, -3.4e38], dtype=np.float32)
+    x = np.concatenate([special, rng.standard_normal(4096).astype(np.float32),
+                        (rng.standard_normal(4096) * 1e-40).astype(np.float32)])
+    X = vk.Array(gpu, data=x)
+    out = vk.U32Array(gpu, shape=((x.size + 1) // 2,))
+    _lib.call("vkh_to_bf16", gpu._handle, X._ptr, out._ptr, x.size)
+    out._set_job([X])
+    got = np.asarray(out).view(np.uint16)[: x.size]
+    np.testing.assert_array_equal(got, bf16_bits(x))
+    nan = np.array([np.nan, -np.nan], dtype=np.float32)
+    Xn = vk.Array(gpu, data=nan)
+    on = vk.U32Array(gpu, shape=(1,))
+    _lib.call("vkh_to_bf16", gpu._handle, Xn._ptr, on._ptr, 2)
+    on._set_job([Xn])
+    b = np.asarray(on).view(np.uint16)
+    assert all((int(v) & 0x7F80) == 0x7F80 and (int(v) & 0x7F) != 0 for v in b)
+
+
+@pytest.mark.parametrize("mode", ["fp32", "bf16"])
+@pytest.mark.parametrize("M,N,K", [(300, 200, 96), (300, 198, 96), (300, 8, 96), (300, 200, 8),
+                                   (512, 512, 8192)])
+def test_gemm_ext_bf16_aux_and_no_fp32_c(gpu, mode, M, N, K):
+    """vkh_gemm_bf16io_ext: ReLU' read from the BF16 copy of the activation
+    gives C bit for bit as vkh_gemm_ex with the FP32 activation, on every
+    path (matrix cores, ragged rows, skinny, short k, split k), including
+    activations that are -0, denormal or too small for BF16's range; with
+    store_c = 0 C is left as it was (also with C NULL) and the BF16 copy is
+    the library's rounding of the values."""
+    h = gpu._handle
+    md = _lib.GEMM_MODE[mode]
+    rng = np.random.default_rng(M + N + K)
+    A = rng.standard_normal((M, K)).astype(np.float32)
+    B = rng.standard_normal((K, N)).astype(np.float32)
+    aux = np.maximum(rng.standard_normal((M, N)), 0).astype(np.float32)
+    flat = aux.reshape(-1)
+    idx = rng.choice(flat.size, 64, replace=False)
+    flat[idx[:16]] = -0.0
+    flat[idx[16:32]] = 1e-45
+    flat[idx[32:48]] = 3e-41
+    flat[idx[48:]] = 2 ** -126
+    gA, gB, gX = vk.Array(gpu, data=A), vk.Array(gpu, data=B), vk.Array(gpu, data=aux)
+    x16 = vk.U32Array(gpu, shape=((M * N + 1) // 2,))
+    _lib.call("vkh_to_bf16", h, gX._ptr, x16._ptr, M * N)
+    ref = vk.Array(gpu, shape=(M, N))
+    _lib.call("vkh_gemm_ex", h, md, 0, 0, M, N, K, 1.0, gA._ptr, K, gB._ptr, N, 0.0, ref._ptr, N,
+              None, 0, None, gX._ptr, N, 1)
+    C = vk.Array(gpu, shape=(M, N))
+    _lib.call("vkh_gemm_bf16io_ext", h, md, 0, 0, M, N, K, 1.0, gA._ptr, K, gB._ptr, N, 0.0, C._ptr, N,
+              None, 0, None, None, x16._ptr, N, 1, None, None, None, 0, None, 1)
+    gpu.wait()
+    want = np.asarray(ref)
+    np.testing.assert_array_equal(np.asarray(C).view(np.uint32), want.view(np.uint32))
+    for c_arg in ("keep", None):
+        C2 = vk.Array(gpu, data=np.full((M, N), 7.0, dtype=np.float32))
+        cbf = vk.U32Array(gpu, shape=((M * N + 1) // 2,))
+        _lib.call("vkh_gemm_bf16io_ext", h, md, 0, 0, M, N, K, 1.0, gA._ptr, K, gB._ptr, N, 0.0,
+                  C2._ptr if c_arg else None, N, None, 0, cbf._ptr, None, x16._ptr, N, 1, None, None, None, 0,
+                  None, 0)
+        for o in (C2, cbf):
+            o._set_job([gA, gB, x16])
+        gpu.wait()
+        assert np.all(np.asarray(C2) == 7.0)
+        np.testing.assert_array_equal(np.asarray(cbf).view(np.uint16)[: M * N].reshape(M, N), bf16_bits(want))
+    with pytest.raises(ValueError):  # a BF16 aux is for ReLU' only
+        _lib.call("vkh_gemm_bf16io_ext", h, md, 0, 0, M, N, K, 1.0, gA._ptr, K, gB._ptr, N, 0.0, C._ptr, N,
+                  None, 0, None, None, x16._ptr, N, 2, None, None, None, 0, None, 1)
 
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])

@@ -7,6 +7,7 @@ import gc
 
 import numpy as np
 import pytest
+from bf16ref import bf16_bits
 
 import vulkpy_amd as vk
 from vulkpy_amd import _lib, nn
@@ -271,9 +272,8 @@ def test_dense1_backward_matches_separate_steps(gpu, B, N):
     np.testing.assert_array_equal(got_x.view(np.uint32), np.asarray(rx).view(np.uint32))
     np.testing.assert_array_equal(np.asarray(dw).view(np.uint32), np.asarray(rw).view(np.uint32))
     np.testing.assert_array_equal(np.asarray(db).view(np.uint32), np.asarray(rb).view(np.uint32))
-    # BF16 copies: round to nearest even of the FP32 values
-    u = got_x.view(np.uint32).astype(np.uint64)
-    bf = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    # BF16 copies: the library's rounding of the FP32 values
+    bf = bf16_bits(got_x)
     np.testing.assert_array_equal(np.asarray(dxbf).view(np.uint16).reshape(B, N), bf)
     np.testing.assert_array_equal(np.asarray(dxbft).view(np.uint16).reshape(N, B), bf.T)
     # without the FP32 dx: the same copies, weight and bias gradients
@@ -321,9 +321,8 @@ def test_gemm_colsum_matches_separate_steps(gpu, mode):
             np.testing.assert_array_equal(got.view(np.uint32), np.asarray(ref).view(np.uint32))
         else:
             assert np.all(got == 7.0)
-        # the transposed BF16 copy: round to nearest even of the FP32 values
-        u = np.asarray(ref).view(np.uint32).astype(np.uint64)
-        bf = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+        # the transposed BF16 copy: the library's rounding of the FP32 values
+        bf = bf16_bits(np.asarray(ref))
         np.testing.assert_array_equal(np.asarray(cbft).view(np.uint16).reshape(N, M), bf.T)
         del C, db, cbft, got
 

@@ -160,6 +160,12 @@ _SIGNATURES = {
                                        c_float, P, U64, P, c_int, P, P, U64, c_int, P, P, P, U64, P,
                                        c_int]),
     "vkh_dense1_backward_supported": (c_int, [P, U64, U64, POINTER(c_int)]),
+    "vkh_gemm_bf16io_ext": (c_int, [P, c_int, c_int, c_int, U64, U64, U64, c_float, P, U64, P, U64,
+                                    c_float, P, U64, P, c_int, P, P, P, U64, c_int, P, P, P, U64, P,
+                                    c_int]),
+    "vkh_gemm_autotune_ext": (c_int, [P, c_int, c_int, c_int, U64, U64, U64, c_float, P, U64, P, U64,
+                                      c_float, P, U64, P, c_int, P, P, P, U64, c_int, P, P, P, U64, P,
+                                      c_int, c_int, POINTER(c_int), POINTER(c_float)]),
     "vkh_adagrad": (c_int, [P, P, P, P, U64, c_float, c_float, c_int, c_float, c_float]),
     "vkh_bias_grad": (c_int, [P, P, P, U64, U64, c_float]),
     "vkh_to_bf16": (c_int, [P, P, P, U64]),

@@ -193,6 +193,20 @@ inline unsigned stream_grid(const vkh_gpu* g, uint64_t work_items, unsigned bloc
 // ordinary accesses in the same one-float4-per-thread kernels.
 typedef float vkh_f4v __attribute__((ext_vector_type(4)));
 
+// FP32 -> BF16 bits: round to nearest even, NaN kept NaN (quiet), and a
+// nonzero value that would round to zero (|x| < 2^-134) keeps the smallest
+// BF16 magnitude of its sign instead.  So a BF16 copy has the sign, the
+// zero-ness and the NaN-ness of its value, and a ReLU' factor read from it
+// equals the one read from the FP32 value.
+__device__ __forceinline__ uint16_t f2bf(float x) {
+  const uint32_t u = __float_as_uint(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  uint32_t b = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+  if ((b & 0x7fffu) == 0 && (u & 0x7fffffffu) != 0) b |= 1u;
+  return (uint16_t)b;
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
 __device__ __forceinline__ float4 ld_stream(const float4* p) {
   const vkh_f4v v = __builtin_nontemporal_load(reinterpret_cast<const vkh_f4v*>(p));
   return make_float4(v.x, v.y, v.z, v.w);

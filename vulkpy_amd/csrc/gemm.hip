@@ -40,14 +40,6 @@ __device__ __forceinline__ void static_for(F&& f) {
   static_for_seq(f, std::make_integer_sequence<int, N>{});
 }
 
-__device__ __forceinline__ uint16_t f2bf(float x) {
-  // round to nearest even (NaN kept quiet)
-  uint32_t u = __float_as_uint(x);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 typedef __attribute__((ext_vector_type(4))) float f32x4_t;
 
@@ -72,6 +64,9 @@ struct GemmArgs {
   // epilogues with 128-row wave slabs only)
   float* cs;
   int no_c;  // 1: the FP32 C is not stored (only Cbf / Cbft / cs)
+  // the aux activation's output as BF16 instead of aux (ReLU' only: the BF16
+  // copy has the sign, zero-ness and NaN-ness of the value)
+  const uint16_t* aux16;
 };
 constexpr int CS_ROWS = 128;
 
@@ -294,6 +289,30 @@ __device__ __forceinline__ f32x4_t load4(const float* p, bool full, int cnt) {
   return v;
 }
 
+// 4 consecutive BF16 values p[0 .. cnt) as floats (one 8-byte load when
+// `full`), zeros beyond.
+__device__ __forceinline__ f32x4_t load4_bf(const uint16_t* p, bool full, int cnt) {
+  f32x4_t v;
+  if (full) {
+    const uint2 t = *reinterpret_cast<const uint2*>(p);
+    v[0] = __uint_as_float(t.x << 16);
+    v[1] = __uint_as_float(t.x & 0xffff0000u);
+    v[2] = __uint_as_float(t.y << 16);
+    v[3] = __uint_as_float(t.y & 0xffff0000u);
+    return v;
+  }
+  v[0] = cnt > 0 ? bf2f(p[0]) : 0.f;
+  v[1] = cnt > 1 ? bf2f(p[1]) : 0.f;
+  v[2] = cnt > 2 ? bf2f(p[2]) : 0.f;
+  v[3] = cnt > 3 ? bf2f(p[3]) : 0.f;
+  return v;
+}
+
+// the aux activation's outputs at C[m, n..n+3] (FP32 or BF16 aux)
+__device__ __forceinline__ f32x4_t aux4(const GemmArgs& e, uint64_t m, uint64_t n, bool full, int cnt) {
+  return e.aux16 ? load4_bf(e.aux16 + m * e.ldaux + n, full, cnt) : load4(e.aux + m * e.ldaux + n, full, cnt);
+}
+
 // The final values of 4 consecutive outputs C[m, n..n+3] from their
 // accumulators, given C's old values (beta != 0), bias[n..n+3] and the aux
 // activation's outputs, in the order alpha*acc (+ beta*C) (+ bias[n]),
@@ -312,7 +331,7 @@ __device__ __forceinline__ float4 finish4(const GemmArgs& e, uint64_t m, uint64_
     if (e.beta != 0.f) t += e.beta * c[q];
     if (e.bias) t += b[q];
     t = act_apply<SIG>(e.act, t);
-    if (e.aux) t = act_deriv_mul(e.aux_act, x[q], t);
+    if (e.aux || e.aux16) t = act_deriv_mul(e.aux_act, x[q], t);
     v[q] = t;
   }
   if (full) {
@@ -348,7 +367,7 @@ __device__ __forceinline__ float4 epilogue4(const GemmArgs& e, uint64_t m, uint6
   const f32x4_t z = {0.f, 0.f, 0.f, 0.f};
   const f32x4_t c = e.beta != 0.f ? load4(e.C + m * e.ldc + n, full, cnt) : z;
   const f32x4_t b = e.bias ? load4(e.bias + n, full, cnt) : z;
-  const f32x4_t x = e.aux ? load4(e.aux + m * e.ldaux + n, full, cnt) : z;
+  const f32x4_t x = e.aux || e.aux16 ? aux4(e, m, n, full, cnt) : z;
   return finish4(e, m, n, acc, full, cnt, c, b, x, write_t);
 }
 
@@ -373,14 +392,22 @@ __device__ __forceinline__ void store_part4(float* part, uint64_t N, uint64_t m,
 // 32 x (WN+4) float staging slice of `smem`, then reads them back as 4-float
 // row segments (WN/4 lanes per row), so C is accessed in whole row segments.
 // With `part` the raw sums go to that k chunk's partial matrix instead.  The
-// transposed BF16 copy (e.Cbft) takes a second pass over the slice: the row
-// pass puts the final values back, then each group of four lanes converts
-// one column of 32 rows and writes it as one 64-byte run of C^T (16 bytes a
-// lane, so a store instruction covers 16 whole runs).  The caller's operand tiles
-// in `smem` must be dead (a barrier before the call).
-template <int TM, int TN, int WM, int WN>
+// transposed BF16 copy (e.Cbft): with TST (the caller's LDS holds a second
+// area `tbase` of WN x TT_PITCH BF16 a wave after the staging slices) the
+// row pass also writes each final value as BF16 into that wave's transposed
+// tile, and after every second pass each group of eight lanes writes one
+// column's 64 rows as one 128-byte run of C^T (16 bytes a lane).  Without
+// it the row pass puts the final values back into the slice and each group
+// of four lanes converts one column of 32 rows into a 64-byte run.  The
+// caller's operand tiles in `smem` must be dead (a barrier before the call).
+constexpr int TT_PITCH = 72;  // transposed tile row: 64 rows of C + 8, 16-byte aligned
+template <int WAVES, int WN>
+constexpr int tt_bytes() { return WAVES * WN * TT_PITCH * 2; }
+
+template <int TM, int TN, int WM, int WN, bool TST = false>
 __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM][TN], const GemmArgs& e,
-                                                float* part, uint64_t m0, uint64_t n0, int wm, int wn) {
+                                                float* part, uint64_t m0, uint64_t n0, int wm, int wn,
+                                                uint16_t* tbase = nullptr) {
   constexpr int SROW = WN + 4, F4R = WN / 4, RPI = 64 / F4R;
   static_assert(64 % F4R == 0, "whole rows per wave pass");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -388,6 +415,17 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
   float* st = smem + wave * 32 * SROW;
   const uint64_t nb = n0 + wn * WN;
   const bool tr = e.Cbft && !part;
+  const bool tst = TST && tr;
+  uint16_t* tt = TST ? tbase + wave * WN * TT_PITCH : nullptr;
+  // the final values of row segment (rr, cc..cc+3) of pass i into the
+  // transposed tile, rows of even passes first
+  auto put_t = [&](int i, int rr, int cc, float4 f) {
+    uint16_t* d = tt + cc * TT_PITCH + (i & 1) * 32 + rr;
+    d[0] = f2bf(f.x);
+    d[TT_PITCH] = f2bf(f.y);
+    d[2 * TT_PITCH] = f2bf(f.z);
+    d[3 * TT_PITCH] = f2bf(f.w);
+  };
   const bool tr_vec = e.ldt % 8 == 0 && ((uintptr_t)e.Cbft & 15u) == 0;
   // column sums: the wave's WM = 128 rows are one chunk; lane c < WN walks
   // column c down the staged slices in row order, 32 rows per pass
@@ -409,7 +447,7 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
   const bool lane_fast = !part && e.act != VKH_ACT_SIGMOID && e.vec_c && n + 3 < e.N;
   const f32x4_t z = {0.f, 0.f, 0.f, 0.f};
   const f32x4_t bq = lane_fast && e.bias ? *reinterpret_cast<const f32x4_t*>(e.bias + n) : z;
-  const bool aux = lane_fast && e.aux;
+  const bool aux = lane_fast && (e.aux || e.aux16);
   f32x4_t xq[PF];
 #pragma unroll
   for (int k = 0; k < PF; ++k) xq[k] = z;
@@ -417,7 +455,7 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
 #pragma unroll
     for (int k = 0; k < PF; ++k) {
       const uint64_t m = m0 + wm * WM + i * 32 + r0 + (k0 + k) * RPI;
-      xq[k] = m < e.M ? *reinterpret_cast<const f32x4_t*>(e.aux + m * e.ldaux + n) : z;
+      xq[k] = m < e.M ? aux4(e, m, n, true, 4) : z;
     }
   };
   if (aux && AHEAD) load_aux(0, 0);
@@ -442,7 +480,8 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
           const float4 v = *reinterpret_cast<const float4*>(st + rr * SROW + cc);
           const f32x4_t c = e.beta != 0.f ? *reinterpret_cast<const f32x4_t*>(e.C + (mb + rr) * e.ldc + n) : z;
           const float4 f = finish4<false>(e, mb + rr, n, v, true, 4, c, bq, xq[k], false);
-          if (tr || csum) *reinterpret_cast<float4*>(st + rr * SROW + cc) = f;
+          if (tst) put_t(i, rr, cc, f);
+          if (csum || (tr && !tst)) *reinterpret_cast<float4*>(st + rr * SROW + cc) = f;
         }
       }
     } else {
@@ -454,7 +493,8 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
             store_part4(part, e.N, m, n, v);
           } else {
             const float4 f = epilogue4(e, m, n, v, false);
-            if (tr || csum) *reinterpret_cast<float4*>(st + rr * SROW + cc) = f;
+            if (tst) put_t(i, rr, cc, f);
+            if (csum || (tr && !tst)) *reinterpret_cast<float4*>(st + rr * SROW + cc) = f;
           }
         }
       }
@@ -480,8 +520,27 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
         }
       }
     }
-    if (csum && !tr) __syncthreads();  // the slice is read before the next pass writes it
-    if (tr) {
+    if (tst && ((i & 1) || i == TM - 1)) {
+      // rows (i & ~1) * 32 .. of the slab: eight lanes a column, 16 bytes each
+      const uint64_t mp = m0 + wm * WM + (i & ~1) * 32;
+      const int q = lane & 7, rows = (i & 1) ? 64 : 32;
+      if (q * 8 < rows) {
+        for (int c = lane >> 3; c < WN; c += 8) {
+          const uint64_t nt = nb + c;
+          if (nt >= e.N) continue;
+          uint16_t* dst = e.Cbft + nt * e.ldt + mp + q * 8;
+          const uint16_t* src = tt + c * TT_PITCH + q * 8;
+          if (tr_vec && mp + q * 8 + 7 < e.M) {
+            *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(src);
+          } else {
+            for (int r = 0; r < 8; ++r)
+              if (mp + q * 8 + r < e.M) dst[r] = src[r];
+          }
+        }
+      }
+    }
+    if (csum && !(tr && !tst)) __syncthreads();  // st (and tt) are read before the next pass writes them
+    if (tr && !tst) {
       // four lanes per column, 8 rows (16 bytes) each: one store covers 16
       // columns in whole 64-byte runs
       const int q = lane & 3;
@@ -615,7 +674,9 @@ k_gemm_f32(GemmArgs p, float* part, uint64_t kc) {
   using SA = F32Stager<BM, AK, THREADS>;
   using SB = F32Stager<BN, BKM, THREADS>;
   constexpr int MAIN = 2 * (SA::FLOATS + SB::FLOATS), STAGE = WAVES * 32 * (WN + 4);
-  __shared__ __attribute__((aligned(16))) float smem[MAIN > STAGE ? MAIN : STAGE];
+  constexpr int SMEM_FLOATS = MAIN > STAGE ? MAIN : STAGE;
+  constexpr bool TST = 4 * STAGE + tt_bytes<WAVES, WN>() <= 4 * SMEM_FLOATS;
+  __shared__ __attribute__((aligned(16))) float smem[SMEM_FLOATS];
   auto As = [&](int b) { return smem + b * SA::FLOATS; };
   auto Bs = [&](int b) { return smem + 2 * SA::FLOATS + b * SB::FLOATS; };
 
@@ -708,8 +769,8 @@ k_gemm_f32(GemmArgs p, float* part, uint64_t kc) {
     frags(cur, 0, 0);
   }
   __syncthreads();  // every wave is done with the operand tiles before staging
-  staged_epilogue<TM, TN, WM, WN>(smem, acc, p, part ? part + (uint64_t)blockIdx.y * p.M * p.N : nullptr,
-                                  m0, n0, wm, wn);
+  staged_epilogue<TM, TN, WM, WN, TST>(smem, acc, p, part ? part + (uint64_t)blockIdx.y * p.M * p.N : nullptr,
+                                       m0, n0, wm, wn, reinterpret_cast<uint16_t*>(smem + STAGE));
 }
 
 // Kernel configurations of the FP32 path (tile and wave grid).  They differ
@@ -798,6 +859,7 @@ k_gemm_packed(PackedArgs p) {
   constexpr int SROW = WN + 4;                                      // staging row (floats)
   constexpr int STAGE_BYTES = WAVES * 32 * SROW * 4;
   constexpr int SMEM_BYTES = MAIN_BYTES > STAGE_BYTES ? MAIN_BYTES : STAGE_BYTES;
+  constexpr bool TST = STAGE_BYTES + tt_bytes<WAVES, WN>() <= SMEM_BYTES;
   __shared__ __attribute__((aligned(16))) uint16_t smem[SMEM_BYTES / 2];
   // operand tile (buffer, part): As(buf, q) / Bs(buf, q)
   auto As = [&](int buf, int q) { return smem + (buf * PARTS + q) * A_TILE; };
@@ -907,9 +969,9 @@ k_gemm_packed(PackedArgs p) {
     if (kt + 1 < nk) frags(cur, 0, 0);
   }
   __syncthreads();  // every wave is done with the operand tiles before staging
-  staged_epilogue<TM, TN, WM, WN>(reinterpret_cast<float*>(smem), acc, e,
-                                  p.part ? p.part + (uint64_t)blockIdx.y * e.M * e.N : nullptr,
-                                  m0, n0, wm, wn);
+  staged_epilogue<TM, TN, WM, WN, TST>(reinterpret_cast<float*>(smem), acc, e,
+                                       p.part ? p.part + (uint64_t)blockIdx.y * e.M * e.N : nullptr,
+                                       m0, n0, wm, wn, reinterpret_cast<uint16_t*>(smem) + STAGE_BYTES / 2);
 }
 
 // BF16 variant whose operand slices go from global memory straight into LDS
@@ -921,7 +983,7 @@ k_gemm_packed(PackedArgs p) {
 // chunk touches 8 distinct positions.  The next slice is requested at the
 // start of a slice and awaited, with one barrier, at its end.  Every output
 // accumulates k in the same order as k_gemm_packed.
-constexpr int DIRECT_SMEM_BYTES = 128 * 1024;
+constexpr int DIRECT_SMEM_BYTES = 144 * 1024;  // operands 128 KB; the epilogue's staging 140 KB
 
 template <int BM, int BN, int WGM, int WGN>
 __global__ void __launch_bounds__(64 * WGM * WGN)
@@ -942,6 +1004,7 @@ k_gemm_direct(PackedArgs p) {
   // dependent size, the host pass of this template emits no launch stub once
   // the LDS-load builtin writes into the array.
   static_assert(SMEM_BYTES <= DIRECT_SMEM_BYTES, "LDS of the direct-load kernel");
+  constexpr bool TST = STAGE_BYTES + tt_bytes<WAVES, WN>() <= DIRECT_SMEM_BYTES;
   __shared__ __attribute__((aligned(16))) char smem[DIRECT_SMEM_BYTES];
   auto Abuf = [&](int b) { return smem + b * A_BYTES; };
   auto Bbuf = [&](int b) { return smem + 2 * A_BYTES + b * B_BYTES; };
@@ -1028,9 +1091,9 @@ k_gemm_direct(PackedArgs p) {
     if (kt + 1 < nk) frags(cur, 0, 0);
   }
   __syncthreads();  // every wave is done with the operand tiles before staging
-  staged_epilogue<TM, TN, WM, WN>(reinterpret_cast<float*>(smem), acc, e,
-                                  p.part ? p.part + (uint64_t)blockIdx.y * e.M * e.N : nullptr,
-                                  m0, n0, wm, wn);
+  staged_epilogue<TM, TN, WM, WN, TST>(reinterpret_cast<float*>(smem), acc, e,
+                                       p.part ? p.part + (uint64_t)blockIdx.y * e.M * e.N : nullptr,
+                                       m0, n0, wm, wn, reinterpret_cast<uint16_t*>(smem) + STAGE_BYTES / 2);
 }
 
 // Tile choice of the packed kernel: 256 x 256 when both sides fill at least
@@ -1314,7 +1377,8 @@ __device__ __forceinline__ void skinny_store(const SkinnyArgs& a, uint64_t p, in
   if (e.bias) v += e.bias[n];
   v = act_apply(e.act, v);
   if (e.aux) v = act_deriv_mul(e.aux_act, e.aux[m * e.ldaux + n], v);
-  *cp = v;
+  if (e.aux16) v = act_deriv_mul(e.aux_act, bf2f(e.aux16[m * e.ldaux + n]), v);
+  if (!e.no_c) *cp = v;
   if (e.Cbf) e.Cbf[m * e.ldc + n] = f2bf(v);
   if (e.Cbft) e.Cbft[n * e.ldt + m] = f2bf(v);
 }
@@ -1754,17 +1818,19 @@ static void gemm_impl(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M
                       uint16_t* c_bf16, const float* aux, uint64_t ldaux, int aux_act,
                       const uint16_t* a_bf16 = nullptr, const uint16_t* b_bf16 = nullptr,
                       uint16_t* c_bf16t = nullptr, uint64_t ldt = 0, float* cs = nullptr,
-                      int no_c = 0) {
+                      int no_c = 0, const uint16_t* aux16 = nullptr) {
   use(g);
   if (M == 0 || N == 0) return;
-  VKH_REQUIRE(Cp, "gemm: null C");
+  VKH_REQUIRE(Cp || no_c, "gemm: null C");
+  VKH_REQUIRE(!(aux && aux16), "gemm: aux given twice");
+  VKH_REQUIRE(!aux16 || aux_act == VKH_ACT_RELU, "gemm: a BF16 aux is for ReLU' only");
   VKH_REQUIRE(K == 0 || (A && B), "gemm: null operand");
   VKH_REQUIRE(ldc >= N, "gemm: ldc < N");
   VKH_REQUIRE(trans_a ? lda >= M : lda >= K, "gemm: lda too small");
   VKH_REQUIRE(trans_b ? ldb >= K : ldb >= N, "gemm: ldb too small");
   VKH_REQUIRE(act >= VKH_ACT_NONE && act <= VKH_ACT_SIGMOID, "gemm: unknown activation");
   VKH_REQUIRE(aux_act >= VKH_ACT_NONE && aux_act <= VKH_ACT_SIGMOID, "gemm: unknown aux activation");
-  VKH_REQUIRE(!aux || ldaux >= N, "gemm: ldaux < N");
+  VKH_REQUIRE(!(aux || aux16) || ldaux >= N, "gemm: ldaux < N");
   VKH_REQUIRE(!c_bf16t || ldt >= M, "gemm: ldt < M");
   VKH_REQUIRE(!c_bf16t || ((uintptr_t)c_bf16t & 1u) == 0, "gemm: c_bf16t not 2-byte aligned");
   VKH_REQUIRE(!a_bf16 || ((uintptr_t)a_bf16 & 15u) == 0, "gemm: a_bf16 not 16-byte aligned");
@@ -1775,10 +1841,11 @@ static void gemm_impl(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M
   a.alpha = alpha; a.beta = beta; a.act = act;
   a.aux = aux; a.ldaux = ldaux; a.aux_act = aux_act;
   a.Cbft = c_bf16t; a.ldt = ldt;
-  a.cs = cs; a.no_c = no_c;
+  a.cs = cs; a.no_c = no_c; a.aux16 = aux16;
   VKH_REQUIRE(!no_c || beta == 0.0f, "gemm: beta * C needs C");
   a.vec_c = ((uintptr_t)Cp & 15u) == 0 && ldc % 4 == 0 && (!bias || ((uintptr_t)bias & 15u) == 0) &&
             (!aux || (((uintptr_t)aux & 15u) == 0 && ldaux % 4 == 0)) &&
+            (!aux16 || (((uintptr_t)aux16 & 7u) == 0 && ldaux % 4 == 0)) &&
             (!c_bf16 || ((uintptr_t)c_bf16 & 7u) == 0);
   VKH_REQUIRE(mode >= VKH_GEMM_FP32 && mode <= VKH_GEMM_BF16X3, "gemm: unknown mode");
   if (std::min(M, N) <= SKINNY_MAX && K > 0) {
@@ -1939,6 +2006,66 @@ int vkh_gemm_autotune(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M
   });
 }
 
+int vkh_gemm_autotune_ext(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M, uint64_t N, uint64_t K,
+                          float alpha, const float* A, uint64_t lda, const float* B, uint64_t ldb, float beta,
+                          float* Cp, uint64_t ldc, const float* bias, int act, uint16_t* c_bf16,
+                          const float* aux, const uint16_t* aux_bf16, uint64_t ldaux, int aux_act,
+                          const uint16_t* a_bf16, const uint16_t* b_bf16, uint16_t* c_bf16t, uint64_t ldt,
+                          float* db, int store_c, int reps, int* best, float* ms_per_config) {
+  return guarded([&] {
+    use(g);
+    VKH_REQUIRE(mode >= VKH_GEMM_FP32 && mode <= VKH_GEMM_BF16X3, "unknown mode");
+    VKH_REQUIRE(!g->capturing, "gemm_autotune: not allowed during graph capture");
+    VKH_REQUIRE(M > 0 && N > 0 && K > 0, "gemm_autotune: empty product");
+    if (reps < 1) reps = 3;
+    (void)db;  // the column sums are not part of the timing
+    const int n = mode_cfgs(mode);
+    const auto key = cfg_key(mode, trans_a, trans_b, M, N, K);
+    // the call's own epilogue; without its column sums an unstored C would
+    // leave no output, so then C is stored
+    const int no_c = (store_c || !(c_bf16 || c_bf16t)) ? 0 : 1;
+    auto run = [&] {
+      gemm_impl(g, mode, trans_a, trans_b, M, N, K, alpha, A, lda, B, ldb, beta, Cp, ldc, bias, act, c_bf16, aux,
+                ldaux, aux_act, a_bf16, b_bf16, c_bf16t, ldt, nullptr, no_c, aux_bf16);
+    };
+    if (std::min(M, N) <= SKINNY_MAX || K <= SMALLK_MAX) {  // a single kernel choice
+      if (ms_per_config)
+        for (int i = 0; i < n; ++i) ms_per_config[i] = -1.0f;
+      if (best) *best = default_cfg(mode, M, N, K);
+      return;
+    }
+    hipEvent_t e0, e1;
+    VKH_HIP(hipEventCreate(&e0));
+    VKH_HIP(hipEventCreate(&e1));
+    int bi = -1;
+    float bt = 0.0f;
+    try {
+      for (int i = 0; i < n; ++i) {
+        g->gemm_cfg[key] = i;
+        run();  // warm-up (and packing-area growth)
+        VKH_HIP(hipEventRecord(e0, g->stream));
+        for (int r = 0; r < reps; ++r) run();
+        VKH_HIP(hipEventRecord(e1, g->stream));
+        VKH_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        VKH_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms /= (float)reps;
+        if (ms_per_config) ms_per_config[i] = ms;
+        if (bi < 0 || ms < bt) { bi = i; bt = ms; }
+      }
+    } catch (...) {
+      g->gemm_cfg.erase(key);
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g->gemm_cfg[key] = bi;
+    if (best) *best = bi;
+  });
+}
+
 int vkh_gemm_reserve(vkh_gpu* g, int mode, uint64_t M, uint64_t N, uint64_t K) {
   return guarded([&] {
     use(g);
@@ -2020,20 +2147,48 @@ int vkh_gemm_colsum_supported(vkh_gpu* g, int mode, int trans_a, int trans_b, ui
   });
 }
 
+// vkh_gemm_bf16io with the column sums of C (db, optional), the FP32 C
+// optional (store_c) and the aux activation's output as BF16 (aux_bf16)
+static void gemm_ext(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M, uint64_t N, uint64_t K,
+                     float alpha, const float* A, uint64_t lda, const float* B, uint64_t ldb, float beta, float* Cp,
+                     uint64_t ldc, const float* bias, int act, uint16_t* c_bf16, const float* aux,
+                     const uint16_t* aux_bf16, uint64_t ldaux, int aux_act, const uint16_t* a_bf16,
+                     const uint16_t* b_bf16, uint16_t* c_bf16t, uint64_t ldt, float* db, int store_c) {
+  use(g);
+  VKH_REQUIRE(store_c || c_bf16 || c_bf16t || db, "gemm: no output");
+  if (!db) {
+    gemm_impl(g, mode, trans_a, trans_b, M, N, K, alpha, A, lda, B, ldb, beta, Cp, ldc, bias, act, c_bf16, aux,
+              ldaux, aux_act, a_bf16, b_bf16, c_bf16t, ldt, nullptr, store_c ? 0 : 1, aux_bf16);
+    return;
+  }
+  VKH_REQUIRE(colsum_ok(g, mode, trans_a, trans_b, M, N, K), "gemm_colsum: shape or configuration not supported");
+  const uint64_t nchunk = ceil_div(M, (uint64_t)CS_ROWS);
+  Workspace ws(g, sizeof(float) * nchunk * N);
+  gemm_impl(g, mode, trans_a, trans_b, M, N, K, alpha, A, lda, B, ldb, beta, Cp, ldc, bias, act, c_bf16, aux,
+            ldaux, aux_act, a_bf16, b_bf16, c_bf16t, ldt, (float*)ws.get(), store_c ? 0 : 1, aux_bf16);
+  sum_chunks(g, (const float*)ws.get(), db, N, nchunk);
+}
+
 int vkh_gemm_bf16io_colsum(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M, uint64_t N,
                            uint64_t K, float alpha, const float* A, uint64_t lda, const float* B, uint64_t ldb,
                            float beta, float* Cp, uint64_t ldc, const float* bias, int act, uint16_t* c_bf16,
                            const float* aux, uint64_t ldaux, int aux_act, const uint16_t* a_bf16,
                            const uint16_t* b_bf16, uint16_t* c_bf16t, uint64_t ldt, float* db, int store_c) {
   return guarded([&] {
-    use(g);
     VKH_REQUIRE(db, "gemm_colsum: null db");
-    VKH_REQUIRE(colsum_ok(g, mode, trans_a, trans_b, M, N, K), "gemm_colsum: shape or configuration not supported");
-    const uint64_t nchunk = ceil_div(M, (uint64_t)CS_ROWS);
-    Workspace ws(g, sizeof(float) * nchunk * N);
-    gemm_impl(g, mode, trans_a, trans_b, M, N, K, alpha, A, lda, B, ldb, beta, Cp, ldc, bias, act, c_bf16, aux,
-              ldaux, aux_act, a_bf16, b_bf16, c_bf16t, ldt, (float*)ws.get(), store_c ? 0 : 1);
-    sum_chunks(g, (const float*)ws.get(), db, N, nchunk);
+    gemm_ext(g, mode, trans_a, trans_b, M, N, K, alpha, A, lda, B, ldb, beta, Cp, ldc, bias, act, c_bf16, aux,
+             nullptr, ldaux, aux_act, a_bf16, b_bf16, c_bf16t, ldt, db, store_c);
+  });
+}
+
+int vkh_gemm_bf16io_ext(vkh_gpu* g, int mode, int trans_a, int trans_b, uint64_t M, uint64_t N, uint64_t K,
+                        float alpha, const float* A, uint64_t lda, const float* B, uint64_t ldb, float beta,
+                        float* Cp, uint64_t ldc, const float* bias, int act, uint16_t* c_bf16, const float* aux,
+                        const uint16_t* aux_bf16, uint64_t ldaux, int aux_act, const uint16_t* a_bf16,
+                        const uint16_t* b_bf16, uint16_t* c_bf16t, uint64_t ldt, float* db, int store_c) {
+  return guarded([&] {
+    gemm_ext(g, mode, trans_a, trans_b, M, N, K, alpha, A, lda, B, ldb, beta, Cp, ldc, bias, act, c_bf16, aux,
+             aux_bf16, ldaux, aux_act, a_bf16, b_bf16, c_bf16t, ldt, db, store_c);
   });
 }
 

@@ -344,12 +344,6 @@ __device__ __forceinline__ void adam_elem(float& w, float g, float& m, float& v,
   w = w + mhat;
 }
 
-__device__ __forceinline__ uint16_t f2bf_nn(float x) {
-  uint32_t u = __float_as_uint(x);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
 
 // One Adam step of parameter element(s): the regularizer term (if any) is
 // added to the gradient first and the sum written back, as
@@ -379,7 +373,7 @@ k_adam(float* __restrict__ w, float* __restrict__ g, float* __restrict__ m, floa
     if (r.kind) reinterpret_cast<float4*>(g)[q] = gv;
     if (wbf) {
       ushort4 b;
-      b.x = f2bf_nn(wv.x); b.y = f2bf_nn(wv.y); b.z = f2bf_nn(wv.z); b.w = f2bf_nn(wv.w);
+      b.x = f2bf(wv.x); b.y = f2bf(wv.y); b.z = f2bf(wv.z); b.w = f2bf(wv.w);
       reinterpret_cast<ushort4*>(wbf)[q] = b;
     }
   }, [&](uint64_t i) {
@@ -387,7 +381,7 @@ k_adam(float* __restrict__ w, float* __restrict__ g, float* __restrict__ m, floa
     adam_elem(wi, gi, mi, vi, s);
     w[i] = wi; m[i] = mi; v[i] = vi;
     if (r.kind) g[i] = gi;
-    if (wbf) wbf[i] = f2bf_nn(wi);
+    if (wbf) wbf[i] = f2bf(wi);
   });
 }
 

@@ -88,7 +88,8 @@ def _packs_a(M: int, N: int, K: int) -> bool:
 
 
 class _Plan:
-    def __init__(self, seq: "GraphSequence", x_shape, y_shape):
+    def __init__(self, seq: "GraphSequence", x_shape, y_shape, x: Optional[Array] = None,
+                 y: Optional[Array] = None):
         self.key = (tuple(x_shape), tuple(y_shape))
         self.seq = seq
         gpu = seq.gpu
@@ -216,23 +217,46 @@ class _Plan:
                 self.bpow[id(st)] = groups[key]
         # the inputs copied into the step's buffers last time, and their jobs
         self._src = {"x": (None, None), "y": (None, None)}
+        self._events = None  # (product key, start, end) timer events of a timed tuning pass
         gpu.wait()
         self.graph = None
-        self._capture()
+        self._capture(x, y)
 
     # ---- the step -------------------------------------------------------------------
-    def _enqueue(self):
+    def _enqueue(self, tune: bool = False, update: bool = True):
+        """Issue the step's kernels.  tune: a pass that times every product's
+        kernel configurations with its own operands and epilogue
+        (vkh_gemm_autotune_ext) and records the products' shapes; update
+        False (implied by tune): stop before the optimizer update."""
         h, B = self.h, self.B
         units = self.units
+
+        def gemm(*args):  # the arguments of vkh_gemm_bf16io_ext
+            key = tuple(args[1:7])  # mode, trans_a, trans_b, M, N, K
+            if tune:
+                call("vkh_gemm_autotune_ext", *args, 3, None, None)
+                if _mfma(*key[3:]) and key not in self._products:
+                    self._products.append(key)
+                    self._takes_colsum[key] = args[26] is not None
+            elif self._events is not None:  # timed pass of the step tuning
+                e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+                call("vkh_timer_record", h, ctypes.byref(e0))
+                call("vkh_gemm_bf16io_ext", *args)
+                call("vkh_timer_record", h, ctypes.byref(e1))
+                self._events.append((key, e0, e1))
+            else:
+                call("vkh_gemm_bf16io_ext", *args)
+
         # forward
         src, src_bf = self.x, None
         for i, (u, hout) in enumerate(zip(units, self.hs)):
             d = u.dense
             hb, hbt = self.hs_bf[i], self.hs_bft[i]
-            call("vkh_gemm_bf16io", h, self.mode_fwd, 0, 1, B, d.output_dim, d.input_dim, 1.0, src._ptr,
-                 d.input_dim, d.w.value._ptr, d.input_dim, 0.0, hout._ptr, d.output_dim, d.b.value._ptr,
-                 ACT[u.act], hb.ptr if hb else None, None, 0, 0, src_bf.ptr if src_bf else None,
-                 None, hbt.ptr if hbt else None, B)
+            store = self.h_store[i]
+            gemm(h, self.mode_fwd, 0, 1, B, d.output_dim, d.input_dim, 1.0, src._ptr,
+                 d.input_dim, d.w.value._ptr, d.input_dim, 0.0, hout._ptr if store else None, d.output_dim,
+                 d.b.value._ptr, ACT[u.act], hb.ptr if hb else None, None, None, 0, 0,
+                 src_bf.ptr if src_bf else None, None, hbt.ptr if hbt else None, B, None, int(store))
             src, src_bf = hout, hb
         out = self.hs[-1]
         O = units[-1].dense.output_dim
@@ -273,10 +297,10 @@ class _Plan:
                 beta = 0.0
                 if is_w:
                     dyt, xt = self.ds_bft[i], (self.hs_bft[i - 1] if i > 0 else None)
-                    call("vkh_gemm_bf16io", h, self.mode_dw, 1, 0, d.output_dim, d.input_dim, B, 1.0, dy._ptr,
+                    gemm(h, self.mode_dw, 1, 0, d.output_dim, d.input_dim, B, 1.0, dy._ptr,
                          d.output_dim, xin._ptr, d.input_dim, beta, p.grad._ptr, d.input_dim,
-                         None, 0, None, None, 0, 0, dyt.ptr if dyt else None, xt.ptr if xt else None,
-                         None, 0)
+                         None, 0, None, None, None, 0, 0, dyt.ptr if dyt else None, xt.ptr if xt else None,
+                         None, 0, None, 1)
                 else:
                     call("vkh_bias_grad", h, dy._ptr, p.grad._ptr, B, d.output_dim, beta)
             if fused:
@@ -291,16 +315,20 @@ class _Plan:
             elif i > 0:
                 prev = units[i - 1]
                 out_bf, dy_bf, out_bft = self.ds_bf[i - 1], self.ds_bf[i], self.ds_bft[i - 1]
-                args = (h, self.mode_dx, 0, 0, B, d.input_dim, d.output_dim, 1.0, dy._ptr,
-                        d.output_dim, d.w.value._ptr, d.input_dim, 0.0, self.ds[i - 1]._ptr,
-                        d.input_dim, None, 0, out_bf.ptr if out_bf else None,
-                        self.hs[i - 1]._ptr if prev.act else None, d.input_dim, ACT[prev.act],
-                        dy_bf.ptr if dy_bf else None, None, out_bft.ptr if out_bft else None, B)
-                if i in self.colsum:
-                    # with the bias gradient of unit i - 1 from its epilogue
-                    call("vkh_gemm_bf16io_colsum", *args, prev.dense.b.grad._ptr, int(self.colsum[i]))
-                else:
-                    call("vkh_gemm_bf16io", *args)
+                # ReLU' from the BF16 copy of h when there is one (the same
+                # factor: the copy keeps the sign and zero-ness of h)
+                aux16 = self.hs_bf[i - 1] if prev.act == "relu" else None
+                aux32 = self.hs[i - 1]._ptr if prev.act and aux16 is None else None
+                # with the bias gradient of unit i - 1 from its epilogue
+                db = prev.dense.b.grad._ptr if i in self.colsum else None
+                store = int(self.colsum[i]) if i in self.colsum else 1
+                gemm(h, self.mode_dx, 0, 0, B, d.input_dim, d.output_dim, 1.0, dy._ptr,
+                     d.output_dim, d.w.value._ptr, d.input_dim, 0.0, self.ds[i - 1]._ptr, d.input_dim, None, 0,
+                     out_bf.ptr if out_bf else None, aux32, aux16.ptr if aux16 else None, d.input_dim,
+                     ACT[prev.act], dy_bf.ptr if dy_bf else None, None, out_bft.ptr if out_bft else None, B,
+                     db, store)
+        if tune or not update:
+            return
         # optimizer: beta powers once per group, then one kernel per
         # parameter (regularizer term, gradient write-back and update)
         for buf, beta1, beta2 in self.bpow_groups:
@@ -347,7 +375,7 @@ class _Plan:
         for d in sorted({u.dense.output_dim for u in self.units if u.dense.b.grad is not None}):
             yield 1, B, d                                 # bias gradient (vkh_bias_grad)
 
-    def _plan_colsums(self):
+    def _plan_colsums(self, check: bool = True):
         """Input-gradient products that also give the previous layer's bias
         gradient (vkh_gemm_bf16io_colsum), decided once the GEMM
         configurations are chosen; and whether their FP32 output is read at
@@ -355,6 +383,7 @@ class _Plan:
         self.colsum = {}
         B, units = self.B, self.units
         last = len(units) - 1
+        self.h_store = [self._h_fp32_read(i) for i in range(len(units))]
         # the fused last-layer backward also gives the hidden layer's bias
         # gradient, so its FP32 output is only for the products
         self.d1_store = self.fuse_last and self._dy_fp32_read(last - 1)
@@ -364,12 +393,80 @@ class _Plan:
             d, prev = units[i].dense, units[i - 1].dense
             if prev.b.grad is None:
                 continue
-            ok = ctypes.c_int(0)
-            call("vkh_gemm_colsum_supported", self.h, self.mode_dx, 0, 0, B, d.input_dim, d.output_dim,
-                 ctypes.byref(ok))
-            if not ok.value:
-                continue
+            if check:  # (without: as if every product took the column sums, for the tuning pass)
+                ok = ctypes.c_int(0)
+                call("vkh_gemm_colsum_supported", self.h, self.mode_dx, 0, 0, B, d.input_dim, d.output_dim,
+                     ctypes.byref(ok))
+                if not ok.value:
+                    continue
             self.colsum[i] = self._dy_fp32_read(i - 1)
+
+    def _h_fp32_read(self, i: int) -> bool:
+        """Whether anything reads the FP32 output hs[i] of unit i's forward
+        product: the prediction and the fused last-layer backward do; the next
+        unit's forward, weight-gradient and input-gradient products do unless
+        they take BF16 copies (ReLU' from the row copy, no other activation)."""
+        last = len(self.units) - 1
+        if i == last or (self.fuse_last and i == last - 1):
+            return True
+        nxt = self.units[i + 1].dense
+        fwd_bf = self.hs_bf[i] is not None
+        dw_bf = nxt.w.grad is None or self.hs_bft[i] is not None
+        act = self.units[i].act
+        aux_bf = act is None or (act == "relu" and self.hs_bf[i] is not None)
+        return not (fwd_bf and dw_bf and aux_bf)
+
+    def _step_tune(self, rounds: int = 2, passes: int = 2):
+        """Second round of the GEMM tuning, in the step's own context.  A
+        product alone and repeated keeps its operands in cache; in the step
+        its neighbours' traffic decides too.  So the forward and backward
+        pass (no update) runs with every product in each of its candidate
+        configurations in turn, timing each product's call between events,
+        `rounds` times over; each product keeps the configuration with the
+        lowest median.  A product whose epilogue takes the column sums
+        only tries the configurations that can."""
+        h = self.h
+        cands = {}
+        for key in self._products:
+            n = ctypes.c_int(0)
+            call("vkh_gemm_configs", key[0], ctypes.byref(n))
+            allowed = []
+            for c in range(n.value):
+                call("vkh_gemm_set_config", h, *key, c)
+                if self._takes_colsum[key]:
+                    ok = ctypes.c_int(0)
+                    call("vkh_gemm_colsum_supported", h, *key, ctypes.byref(ok))
+                    if not ok.value:
+                        continue
+                allowed.append(c)
+            cands[key] = allowed or [0]
+        times = {key: {c: [] for c in cands[key]} for key in cands}
+        trials = max((len(v) for v in cands.values()), default=0)
+        ms = ctypes.c_float()
+        for _ in range(rounds):
+            for t in range(trials):
+                chosen = {key: cands[key][t % len(cands[key])] for key in cands}
+                for key, c in chosen.items():
+                    call("vkh_gemm_set_config", h, *key, c)
+                self._plan_colsums()
+                self._enqueue(update=False)
+                self._events = []
+                try:
+                    for _ in range(passes):
+                        self._enqueue(update=False)
+                    self.gpu.wait()
+                    for key, e0, e1 in self._events:
+                        call("vkh_event_elapsed_ms", e0, e1, ctypes.byref(ms))
+                        if key in chosen:
+                            times[key][chosen[key]].append(ms.value)
+                finally:
+                    for _, e0, e1 in self._events:
+                        call("vkh_event_destroy", e0)
+                        call("vkh_event_destroy", e1)
+                    self._events = None
+        for key, per in times.items():
+            best = min(per, key=lambda c: float(np.median(per[c])) if per[c] else float("inf"))
+            call("vkh_gemm_set_config", h, *key, best)
 
     def _dy_fp32_read(self, p: int) -> bool:
         """Whether the products of unit p read its FP32 output gradient ds[p]:
@@ -382,18 +479,26 @@ class _Plan:
         dx_bf = p == 0 or (self.mode_dx == bf16 and self.ds_bf[p] is not None and d.output_dim % 64 == 0)
         return not (dw_bf and dx_bf)
 
-    def _capture(self):
-        # kernel configuration (when tuning) and the graph's GEMM packing
-        # region for every product of the step
+    def _capture(self, x: Optional[Array] = None, y: Optional[Array] = None):
+        # the graph's GEMM packing region for every product of the step
         for mode, ta, tb, M, N, K in self._gemm_shapes():
-            if self.seq.autotune:
-                call("vkh_gemm_autotune", self.h, mode, ta, tb, M, N, K, 3, None, None)
             call("vkh_gemm_reserve", self.h, mode, M, N, K)
         if self.seq.autotune:
             for prev, K, post in self._reduce_shapes():
                 # only configurations with the default summation order, so
                 # the step's results never depend on the timings
                 call("vkh_reduce_autotune", self.h, 1, prev, K, post, 3, 1, None, None)
+            # GEMM configurations: a pass of the step (on the first batch when
+            # given, no optimizer update) that times each product with its own
+            # operands and epilogue, whose cost differs between tile shapes
+            for name, src, dst in (("x", x, self.x), ("y", y, self.y)):
+                if src is not None:
+                    call("vkh_copy", self.h, dst._ptr, src._ptr, dst.buffer.nbytes)
+                    self._src[name] = (src, src.job)
+            self._plan_colsums(check=False)
+            self._products, self._takes_colsum = [], {}
+            self._enqueue(tune=True)
+            self._step_tune()
         self._plan_colsums()
         self.gpu.wait()
         if not self.seq.capture:
@@ -521,12 +626,13 @@ class GraphSequence:
         never does (vulkpy/nn/models.py:55-78).
     autotune : bool
         Before capturing, time every GEMM kernel configuration on each
-        product shape of the step (``vulkpy_amd.autotune_gemm``)
-        and the reduction configurations of the bias gradients and the batch
-        loss that keep the default summation order
-        (``vulkpy_amd.autotune_reduce(..., same_order=True)``), and use the
-        fastest.  Neither choice changes a result.  The choice is kept on the
-        GPU context, so eager calls of the same shapes use it too.
+        product of the step with its own operands and epilogue (a pass of the
+        step on the first batch without the optimizer update, through
+        ``vkh_gemm_autotune_ext``) and the reduction configurations of the
+        bias gradients and the batch loss that keep the default summation
+        order (``vulkpy_amd.autotune_reduce(..., same_order=True)``), and use
+        the fastest.  Neither choice changes a result.  The choice is kept on
+        the GPU context, so eager calls of the same shapes use it too.
     capture : bool
         Capture the step as a hipGraph (default).  With False every ``train``
         call issues the same kernels, with the same arguments and results,
@@ -560,7 +666,7 @@ class GraphSequence:
         key = (tuple(x.shape), tuple(y.shape))
         if self._plan is None or self._plan.key != key:
             self._plan = None
-            self._plan = _Plan(self, x.shape, y.shape)
+            self._plan = _Plan(self, x.shape, y.shape, x, y)
         return self._plan.run(x, y)
 
     def predict(self, x: Array, y: Optional[Array] = None):
