@@ -433,12 +433,14 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
   float csacc = 0.f;
   // A lane's row segments: columns n .. n+3 of rows r0 + k*RPI of each pass.
   // In the common case (a whole 4-column group, every row of the pass in
-  // range, no sigmoid) there are no global load waits between the pass's
-  // stores (a wait on a load also waits for every earlier store): bias is
-  // read once, and the aux values of the pass are in registers before its
-  // row pass, read during the previous pass where they fit (ITER <= 8), else
-  // in groups of PF at its start.  Other segments (and the split-k partial
-  // sums) take one loop iteration each, with their own loads.
+  // range, no sigmoid) no global load waits between the passes' stores (a
+  // wait on a load also waits for every store issued before it): bias is
+  // read once; a ReLU' aux is read for every pass before the first store and
+  // kept as a 2-bit code per value (positive, NaN, else), which is all ReLU'
+  // needs (the BF16 aux); other aux values are in registers before their pass,
+  // read during the previous pass where they fit (ITER <= 8), else in groups
+  // of PF at its start.  Other segments (and the split-k partial sums) take
+  // one loop iteration each, with their own loads.
   constexpr int ITER = 32 / RPI, PF = ITER < 8 ? ITER : 8;
   constexpr bool AHEAD = ITER == PF;
   const int cc = (lane % F4R) * 4, r0 = lane / F4R;
@@ -448,6 +450,52 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
   const f32x4_t z = {0.f, 0.f, 0.f, 0.f};
   const f32x4_t bq = lane_fast && e.bias ? *reinterpret_cast<const f32x4_t*>(e.bias + n) : z;
   const bool aux = lane_fast && (e.aux || e.aux16);
+  const bool relu_codes = aux && e.aux16;  // (a BF16 aux is ReLU' only)
+  const bool xpre = aux && !relu_codes;
+  // codes: 8 bits a row segment (2 a value), four segments a word; read as
+  // raw bits, a pass a batch
+  constexpr int NSEG = TM * ITER;
+  uint32_t cw[(NSEG + 3) / 4];
+#pragma unroll
+  for (int w = 0; w < (NSEG + 3) / 4; ++w) cw[w] = 0u;
+  auto put_code = [&](int seg, uint32_t byte) { cw[seg / 4] |= byte << (8 * (seg % 4)); };
+  if (relu_codes) {
+    constexpr int BP = 1;
+#pragma unroll
+    for (int i0 = 0; i0 < TM; i0 += BP) {
+      uint2 raw[BP * ITER];
+#pragma unroll
+      for (int g = 0; g < BP * ITER; ++g) {
+        const int i = i0 + g / ITER, k = g % ITER;
+        const uint64_t m = m0 + wm * WM + i * 32 + r0 + k * RPI;
+        raw[g] = (i < TM && m < e.M) ? *reinterpret_cast<const uint2*>(e.aux16 + m * e.ldaux + n)
+                                     : make_uint2(0u, 0u);
+      }
+#pragma unroll
+      for (int g = 0; g < BP * ITER; ++g) {
+        if (i0 * ITER + g < NSEG) {
+          uint32_t byte = 0u;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const uint32_t hb = ((q < 2 ? raw[g].x : raw[g].y) >> (16 * (q & 1))) & 0xffffu, mag = hb & 0x7fffu;
+            byte |= (mag > 0x7f80u ? 2u : ((hb & 0x8000u) == 0u && mag != 0u ? 1u : 0u)) << (2 * q);
+          }
+          put_code(i0 * ITER + g, byte);
+        }
+      }
+    }
+  }
+  // the aux values ReLU' acts like for segment seg: 1 (positive), NaN or 0
+  auto relu_x = [&](int seg) {
+    const uint32_t byte = (cw[seg / 4] >> (8 * (seg % 4))) & 0xffu;
+    f32x4_t xv;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t c = (byte >> (2 * q)) & 3u;
+      xv[q] = c == 1u ? 1.f : (c == 2u ? __builtin_nanf("") : 0.f);
+    }
+    return xv;
+  };
   f32x4_t xq[PF];
 #pragma unroll
   for (int k = 0; k < PF; ++k) xq[k] = z;
@@ -458,7 +506,7 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
       xq[k] = m < e.M ? aux4(e, m, n, true, 4) : z;
     }
   };
-  if (aux && AHEAD) load_aux(0, 0);
+  if (xpre && AHEAD) load_aux(0, 0);
   // pass i: rows i*32 .. i*32+31 of the sub-tile (a compile-time i, so the
   // accumulators stay in registers however large the pass's code)
   auto pass = [&](auto ic) __attribute__((always_inline)) {
@@ -473,13 +521,14 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
     if (lane_fast && mb + 31 < e.M) {
 #pragma unroll
       for (int k0 = 0; k0 < ITER; k0 += PF) {
-        if (aux && !AHEAD) load_aux(i, k0);
+        if (xpre && !AHEAD) load_aux(i, k0);
 #pragma unroll
         for (int k = 0; k < PF; ++k) {
           const int rr = r0 + (k0 + k) * RPI;
           const float4 v = *reinterpret_cast<const float4*>(st + rr * SROW + cc);
           const f32x4_t c = e.beta != 0.f ? *reinterpret_cast<const f32x4_t*>(e.C + (mb + rr) * e.ldc + n) : z;
-          const float4 f = finish4<false>(e, mb + rr, n, v, true, 4, c, bq, xq[k], false);
+          const f32x4_t xv = relu_codes ? relu_x(i * ITER + k0 + k) : xq[k];
+          const float4 f = finish4<false>(e, mb + rr, n, v, true, 4, c, bq, xv, false);
           if (tst) put_t(i, rr, cc, f);
           if (csum || (tr && !tst)) *reinterpret_cast<float4*>(st + rr * SROW + cc) = f;
         }
@@ -499,7 +548,7 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
         }
       }
     }
-    if (aux && AHEAD && i + 1 < TM) load_aux(i + 1, 0);
+    if (xpre && AHEAD && i + 1 < TM) load_aux(i + 1, 0);
     __syncthreads();
     if (csum && lane < WN) {
       if (mb + 31 < e.M) {
