@@ -320,7 +320,8 @@ __device__ __forceinline__ f32x4_t aux4(const GemmArgs& e, uint64_t m, uint64_t 
 // accesses when `full` (cnt outputs of the group in range otherwise).  The
 // transposed BF16 copy is written here, one element at a time, unless write_t
 // is false (the staged epilogue writes it in whole runs).
-template <bool SIG = true>
+// D: x holds the derivative factors themselves (ReLU' decoded from codes)
+template <bool SIG = true, bool D = false>
 __device__ __forceinline__ float4 finish4(const GemmArgs& e, uint64_t m, uint64_t n, float4 acc, bool full,
                                           int cnt, f32x4_t c, f32x4_t b, f32x4_t x, bool write_t) {
   float* cp = e.C + m * e.ldc + n;
@@ -331,7 +332,7 @@ __device__ __forceinline__ float4 finish4(const GemmArgs& e, uint64_t m, uint64_
     if (e.beta != 0.f) t += e.beta * c[q];
     if (e.bias) t += b[q];
     t = act_apply<SIG>(e.act, t);
-    if (e.aux || e.aux16) t = act_deriv_mul(e.aux_act, x[q], t);
+    if (e.aux || e.aux16) t = D ? x[q] * t : act_deriv_mul(e.aux_act, x[q], t);
     v[q] = t;
   }
   if (full) {
@@ -485,7 +486,8 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
       }
     }
   }
-  // the aux values ReLU' acts like for segment seg: 1 (positive), NaN or 0
+  // ReLU' of segment seg's aux values, as act_deriv_mul finds it: 1
+  // (positive), NaN (NaN) or 0
   auto relu_x = [&](int seg) {
     const uint32_t byte = (cw[seg / 4] >> (8 * (seg % 4))) & 0xffu;
     f32x4_t xv;
@@ -527,8 +529,9 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
           const int rr = r0 + (k0 + k) * RPI;
           const float4 v = *reinterpret_cast<const float4*>(st + rr * SROW + cc);
           const f32x4_t c = e.beta != 0.f ? *reinterpret_cast<const f32x4_t*>(e.C + (mb + rr) * e.ldc + n) : z;
-          const f32x4_t xv = relu_codes ? relu_x(i * ITER + k0 + k) : xq[k];
-          const float4 f = finish4<false>(e, mb + rr, n, v, true, 4, c, bq, xv, false);
+          const float4 f = relu_codes
+                               ? finish4<false, true>(e, mb + rr, n, v, true, 4, c, bq, relu_x(i * ITER + k0 + k), false)
+                               : finish4<false>(e, mb + rr, n, v, true, 4, c, bq, xq[k], false);
           if (tst) put_t(i, rr, cc, f);
           if (csum || (tr && !tst)) *reinterpret_cast<float4*>(st + rr * SROW + cc) = f;
         }
