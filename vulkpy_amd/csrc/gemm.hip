@@ -321,9 +321,12 @@ __device__ __forceinline__ f32x4_t aux4(const GemmArgs& e, uint64_t m, uint64_t 
 // transposed BF16 copy is written here, one element at a time, unless write_t
 // is false (the staged epilogue writes it in whole runs).
 // D: x holds the derivative factors themselves (ReLU' decoded from codes)
+// bf (full groups): receives the four values as BF16, two a word, when it
+// converts them for e.Cbf or when `want_bf` (the caller writes them again)
 template <bool SIG = true, bool D = false>
 __device__ __forceinline__ float4 finish4(const GemmArgs& e, uint64_t m, uint64_t n, float4 acc, bool full,
-                                          int cnt, f32x4_t c, f32x4_t b, f32x4_t x, bool write_t) {
+                                          int cnt, f32x4_t c, f32x4_t b, f32x4_t x, bool write_t,
+                                          uint2* bf = nullptr, bool want_bf = false) {
   float* cp = e.C + m * e.ldc + n;
   float v[4] = {acc.x, acc.y, acc.z, acc.w};
 #pragma unroll
@@ -337,11 +340,12 @@ __device__ __forceinline__ float4 finish4(const GemmArgs& e, uint64_t m, uint64_
   }
   if (full) {
     if (!e.no_c) *reinterpret_cast<float4*>(cp) = make_float4(v[0], v[1], v[2], v[3]);
-    if (e.Cbf) {
+    if (e.Cbf || want_bf) {
       uint2 h;
       h.x = f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
       h.y = f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
-      *reinterpret_cast<uint2*>(e.Cbf + m * e.ldc + n) = h;
+      if (e.Cbf) *reinterpret_cast<uint2*>(e.Cbf + m * e.ldc + n) = h;
+      if (bf) *bf = h;
     }
   } else {
 #pragma unroll
@@ -419,13 +423,17 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
   const bool tst = TST && tr;
   uint16_t* tt = TST ? tbase + wave * WN * TT_PITCH : nullptr;
   // the final values of row segment (rr, cc..cc+3) of pass i into the
-  // transposed tile, rows of even passes first
-  auto put_t = [&](int i, int rr, int cc, float4 f) {
+  // transposed tile, rows of even passes first (as BF16 pairs already
+  // converted, or converted here)
+  auto put_t_bf = [&](int i, int rr, int cc, uint2 h) {
     uint16_t* d = tt + cc * TT_PITCH + (i & 1) * 32 + rr;
-    d[0] = f2bf(f.x);
-    d[TT_PITCH] = f2bf(f.y);
-    d[2 * TT_PITCH] = f2bf(f.z);
-    d[3 * TT_PITCH] = f2bf(f.w);
+    d[0] = (uint16_t)h.x;
+    d[TT_PITCH] = (uint16_t)(h.x >> 16);
+    d[2 * TT_PITCH] = (uint16_t)h.y;
+    d[3 * TT_PITCH] = (uint16_t)(h.y >> 16);
+  };
+  auto put_t = [&](int i, int rr, int cc, float4 f) {
+    put_t_bf(i, rr, cc, make_uint2(f2bf(f.x) | ((uint32_t)f2bf(f.y) << 16), f2bf(f.z) | ((uint32_t)f2bf(f.w) << 16)));
   };
   const bool tr_vec = e.ldt % 8 == 0 && ((uintptr_t)e.Cbft & 15u) == 0;
   // column sums: the wave's WM = 128 rows are one chunk; lane c < WN walks
@@ -529,10 +537,12 @@ __device__ __forceinline__ void staged_epilogue(float* smem, f32x16_t (&acc)[TM]
           const int rr = r0 + (k0 + k) * RPI;
           const float4 v = *reinterpret_cast<const float4*>(st + rr * SROW + cc);
           const f32x4_t c = e.beta != 0.f ? *reinterpret_cast<const f32x4_t*>(e.C + (mb + rr) * e.ldc + n) : z;
+          uint2 hb = make_uint2(0u, 0u);
           const float4 f = relu_codes
-                               ? finish4<false, true>(e, mb + rr, n, v, true, 4, c, bq, relu_x(i * ITER + k0 + k), false)
-                               : finish4<false>(e, mb + rr, n, v, true, 4, c, bq, xq[k], false);
-          if (tst) put_t(i, rr, cc, f);
+                               ? finish4<false, true>(e, mb + rr, n, v, true, 4, c, bq, relu_x(i * ITER + k0 + k), false,
+                                                      &hb, tst)
+                               : finish4<false>(e, mb + rr, n, v, true, 4, c, bq, xq[k], false, &hb, tst);
+          if (tst) put_t_bf(i, rr, cc, hb);
           if (csum || (tr && !tst)) *reinterpret_cast<float4*>(st + rr * SROW + cc) = f;
         }
       }
