@@ -83,8 +83,9 @@ __device__ __forceinline__ float act_apply(int act, float v) {
 // nn/layers.py:207-210 (ReLU) and 264-267 (Sigmoid)
 __device__ __forceinline__ float act_deriv_mul(int act, float y, float dy) {
   if (act == VKH_ACT_RELU) {
-    const float s = y > 0.f ? 1.f : (y < 0.f ? -1.f : (y == 0.f ? 0.f : y));
-    const float d = (s >= 0.f || s != s) ? s : 0.f;
+    // max(sign(y), 0) (np.maximum keeps a NaN): 1 for y > 0, y itself for a
+    // NaN, +0 otherwise
+    const float d = y > 0.f ? 1.f : (y != y ? y : 0.f);
     return d * dy;
   }
   if (act == VKH_ACT_SIGMOID) {
