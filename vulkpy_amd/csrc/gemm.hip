@@ -1186,7 +1186,14 @@ static const PackedCfg BF16_CFGS[] = {
     {256, 128, 64, 4, 128, launch_packed<256, 128, 64, false, 2, 2>},
     {128, 256, 64, 4, 64, launch_packed<128, 256, 64, false, 2, 2>},
     {256, 256, 64, 8, 128, launch_direct<256, 256, 2, 4>, 1},
-    // (direct loads with 4 waves of 128 x 128, a third less LDS fragment
+    // 8 waves of 64 x 32: twice the waves of configuration 2 in the same
+    // LDS, for short k, where the operand latency of each tile's few slices
+    // is exposed (16384 x 4096 x 256 with its BF16 row and transposed
+    // copies: 109 against 143-151 us)
+    {128, 128, 64, 8, 64, launch_packed<128, 128, 64, false, 2, 4>},
+    // (128 x 256 with 8 waves of 64 x 64: 110 KB of LDS, one block a
+    // compute unit, fastest on none of the MLP shapes, so not kept;
+    // direct loads with 4 waves of 128 x 128, a third less LDS fragment
     // traffic, one wave per SIMD: 894 against 977 TFLOP/s at 8192^3 and
     // slower on every MLP shape, so not kept)
 };
