@@ -184,16 +184,16 @@ k_pack_rows(const float* src, uint64_t ld, uint64_t R, uint64_t K, uint16_t* hi,
   }
 }
 
-// Transposed source: a PK_TK (k) x PK_TR (r) tile goes through LDS (rows of
-// PK_TR + 1 floats: the column reads of the second phase fall in distinct
-// banks), so the reads are 16-byte loads along r (512-byte row segments) and
-// the writes 16-byte stores along k (256-byte row segments of the packed
-// buffer).  Rp is a multiple of PK_TR; the last k tile may pass Kp.
+// Transposed source: a PK_TK (k) x PK_TR (r) tile goes through LDS (BF16
+// pairs for the plain pack, rows of PK_TR + 1 floats for the split one: the
+// column reads of the second phase spread over the banks), so the reads are
+// 16-byte loads along r (512-byte row segments) and the writes 16-byte
+// stores along k (256-byte row segments of the packed buffer).  Rp is a
+// multiple of PK_TR; the last k tile may pass Kp.
 template <bool SPLIT>
 __global__ void __launch_bounds__(256)
 k_pack_cols(const float* src, uint64_t ld, uint64_t R, uint64_t K, uint16_t* hi, uint16_t* lo,
             uint64_t Rp, uint64_t Kp, int vec) {
-  __shared__ float t[PK_TK][PK_TR + 1];
   const uint64_t r0 = (uint64_t)blockIdx.x * PK_TR, k0 = (uint64_t)blockIdx.y * PK_TK;
   constexpr int R4 = PK_TR / 4;
   constexpr int NL = PK_TK * R4 / 256;  // 16-byte loads per thread, all issued before the LDS writes
@@ -210,24 +210,60 @@ k_pack_cols(const float* src, uint64_t ld, uint64_t R, uint64_t K, uint16_t* hi,
       for (int q = 0; q < 4; ++q) v[i][q] = (k < K && r + q < R) ? src[k * ld + r + q] : 0.f;
     }
   }
+  if constexpr (!SPLIT) {
+    // BF16 tile: the values are rounded on the way in (the rounding the
+    // packed row would apply), so the tile takes 33 KB and four blocks fit
+    // a compute unit.  Word w of row kk holds r = 2w (low half) and 2w + 1;
+    // a thread of the second phase reads one word of 8 consecutive k and
+    // writes the packed rows of both r.
+    constexpr int PW = PK_TR / 2 + 1;  // words a row: odd, so the column reads spread over banks
+    __shared__ uint32_t tb[PK_TK][PW];
 #pragma unroll
-  for (int i = 0; i < NL; ++i) {
-    const int f = threadIdx.x + i * 256;
-    const int kk = f / R4, rr = (f % R4) * 4;
+    for (int i = 0; i < NL; ++i) {
+      const int f = threadIdx.x + i * 256;
+      const int kk = f / R4, w = (f % R4) * 2;
+      tb[kk][w] = bf2x(v[i][0], v[i][1]);
+      tb[kk][w + 1] = bf2x(v[i][2], v[i][3]);
+    }
+    __syncthreads();
+    constexpr int KC = PK_TK / 8;
+    for (int f = threadIdx.x; f < (PK_TR / 2) * KC; f += 256) {
+      const int rp = f / KC, kc = f % KC;
+      const uint64_t k = k0 + kc * 8;
+      if (k >= Kp) continue;
+      uint32_t wd[8];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) t[kk][rr + q] = v[i][q];
-  }
-  __syncthreads();
-  constexpr int KC = PK_TK / 8;  // 8-element chunks per packed row in the tile
-  for (int f = threadIdx.x; f < PK_TR * KC; f += 256) {
-    const int rr = f / KC, kc = f % KC;
-    const uint64_t k = k0 + kc * 8;
-    if (k >= Kp) continue;
-    float v[8];
+      for (int q = 0; q < 8; ++q) wd[q] = tb[kc * 8 + q][rp];
+      uint4 ev, od;  // rows 2 rp (low halves) and 2 rp + 1 (high halves), k ascending
+      ev.x = (wd[0] & 0xffffu) | (wd[1] << 16); od.x = (wd[0] >> 16) | (wd[1] & 0xffff0000u);
+      ev.y = (wd[2] & 0xffffu) | (wd[3] << 16); od.y = (wd[2] >> 16) | (wd[3] & 0xffff0000u);
+      ev.z = (wd[4] & 0xffffu) | (wd[5] << 16); od.z = (wd[4] >> 16) | (wd[5] & 0xffff0000u);
+      ev.w = (wd[6] & 0xffffu) | (wd[7] << 16); od.w = (wd[6] >> 16) | (wd[7] & 0xffff0000u);
+      const uint64_t off = (r0 + 2 * rp) * Kp + k;
+      *reinterpret_cast<uint4*>(hi + off) = ev;
+      *reinterpret_cast<uint4*>(hi + off + Kp) = od;
+    }
+  } else {  // FP32 tile: the split needs each value's low part too
+    __shared__ float t[PK_TK][PK_TR + 1];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) v[q] = t[kc * 8 + q][rr];
-    const uint64_t off = (r0 + rr) * Kp + k;
-    pack8<SPLIT>(v, hi + off, SPLIT ? lo + off : nullptr);
+    for (int i = 0; i < NL; ++i) {
+      const int f = threadIdx.x + i * 256;
+      const int kk = f / R4, rr = (f % R4) * 4;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) t[kk][rr + q] = v[i][q];
+    }
+    __syncthreads();
+    constexpr int KC = PK_TK / 8;  // 8-element chunks per packed row in the tile
+    for (int f = threadIdx.x; f < PK_TR * KC; f += 256) {
+      const int rr = f / KC, kc = f % KC;
+      const uint64_t k = k0 + kc * 8;
+      if (k >= Kp) continue;
+      float v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = t[kc * 8 + q][rr];
+      const uint64_t off = (r0 + rr) * Kp + k;
+      pack8<SPLIT>(v, hi + off, SPLIT ? lo + off : nullptr);
+    }
   }
 }
 
