@@ -506,3 +506,49 @@ def test_bf16io_transposed_output_and_packed_b(gpu, ta, tb):
                       gA._ptr, A.shape[1], gB._ptr, Bm.shape[1], 0.0, C._ptr, N, None, 0, None, None, 0,
                       0, a_p, b_p, c_p, M)
     gpu.wait()
+
+
+@pytest.mark.parametrize("M,N,K", [(300, 260, 256), (1024, 512, 256)])
+def test_every_bf16_configuration_with_bf16_copies(gpu, M, N, K):
+    """Every BF16 kernel configuration with the forward epilogue (bias, ReLU)
+    and both BF16 copies of C, row-major and transposed (each configuration's
+    own store path for C^T): C bit for bit the same in every configuration,
+    each copy the library's rounding of C, and C within the FP32 accumulation
+    bound of relu(A W^T + b) computed in FP64 from the BF16-rounded operands."""
+    rng = np.random.default_rng(M + 3 * N + K)
+    A = rng.standard_normal((M, K)).astype(np.float32)
+    W = rng.standard_normal((N, K)).astype(np.float32)
+    bias = rng.standard_normal(N).astype(np.float32)
+    gA, gW, gb = vk.Array(gpu, data=A), vk.Array(gpu, data=W), vk.Array(gpu, data=bias)
+    h, md = gpu._handle, _lib.GEMM_MODE["bf16"]
+    ldt = (M + 7) // 8 * 8
+    cfgs = vk.gemm_configs("bf16")
+    outs = []
+    try:
+        for i in range(len(cfgs)):
+            _lib.call("vkh_gemm_set_config", h, md, 0, 1, M, N, K, i)
+            C = vk.Array(gpu, shape=(M, N))
+            cbf = vk.U32Array(gpu, shape=((M * N + 1) // 2,))
+            ct = vk.U32Array(gpu, shape=((N * ldt + 1) // 2,))
+            _lib.call("vkh_gemm_bf16io", h, md, 0, 1, M, N, K, 1.0, gA._ptr, K, gW._ptr, K, 0.0, C._ptr, N,
+                      gb._ptr, _lib.ACT["relu"], cbf._ptr, None, 0, 0, None, None, ct._ptr, ldt)
+            C._set_job([gA, gW, gb])
+            cbf._set_job([])   # written by the product: drop the host mirrors
+            ct._set_job([])
+            c = np.asarray(C).copy()
+            outs.append(c)
+            want = bf16_bits(c)
+            np.testing.assert_array_equal(np.asarray(cbf).view(np.uint16)[: M * N].reshape(M, N), want,
+                                          err_msg=f"row copy, configuration {i} {cfgs[i]}")
+            got_t = np.asarray(ct).view(np.uint16)[: N * ldt].reshape(N, ldt)[:, :M]
+            np.testing.assert_array_equal(got_t, want.T, err_msg=f"transposed copy, configuration {i} {cfgs[i]}")
+    finally:
+        _lib.call("vkh_gemm_set_config", h, md, 0, 1, M, N, K, -1)
+    for i, o in enumerate(outs[1:], 1):
+        np.testing.assert_array_equal(o.view(np.uint32), outs[0].view(np.uint32),
+                                      err_msg=f"configuration {i} {cfgs[i]}")
+    a16 = (bf16_bits(A).astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    w16 = (bf16_bits(W).astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    ref = np.maximum(a16 @ w16.T + bias.astype(np.float64), 0.0)
+    bound = np.abs(a16) @ np.abs(w16).T + np.abs(bias.astype(np.float64))
+    assert (np.abs(outs[0] - ref) / bound).max() <= (K + 2) * 2.0 ** -24
